@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Benchmark of the Kino-PAX planner on one MI355X.
+
+A step is one complete plan() of the 12D quadcopter forest workload (400
+columns in [0,10]^2 x [0,5], RK4 dt = 0.02, 2^20-node tree cap, 16^3 regions,
+64^3 collision grid), each step with its own seed.  Inputs (obstacles, grid)
+are resident in HBM before timing starts.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "kino-pax-fork_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+import kpax  # noqa: E402
+from workloads import workload  # noqa: E402
+
+METRIC = "p50 time-to-first-solution & plans/sec at ≥95% success; node-expansions/sec"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+# Algorithmic bytes per expansion: parent state read (4S), child sample write
+# (4(S+C+1)), parent index (4), admission flag (1), two region counters (8).
+def bytes_per_expansion(S, C):
+    return 8 * S + 4 * C + 17
+
+
+# FP32 operations per integration step as written in the kernel (add, sub,
+# mul, div, sqrt, sin, cos = 1 each; compares not counted).
+FLOP_PER_STEP = {kpax.MODEL_DOUBLE_INTEGRATOR: 60, kpax.MODEL_DUBINS_AIRPLANE: 67, kpax.MODEL_QUADCOPTER: 624}
+
+
+def cpu_baseline(k, env_seed, plans, seeds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # checker only: times the CPU restatement of the same planner
+    threads = min(os.cpu_count() or 1, 16)
+    cfg, obs, start, goal = workload(k, seed=env_seed)
+    exp = 0
+    ms = []
+    t0 = time.perf_counter()
+    for s in seeds[:plans]:
+        t = time.perf_counter()
+        r = oracle.plan(cfg, obs, start, goal, seed=s, threads=threads, want_tree=False)
+        ms.append((time.perf_counter() - t) * 1e3)
+        exp += r["expansions"]
+    dt = time.perf_counter() - t0
+    return {"value": exp / dt, "unit": "node-expansions/s", "cores": threads, "kind": "port",
+            "sample": f"{plans} full plan(s) of the same workload (seeds {seeds[:plans]}) on {threads} host threads "
+                      f"(OpenMP over expansions); p50 time-to-solution {statistics.median(ms):.0f} ms"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--env-seed", type=int, default=0)
+    ap.add_argument("--cpu-plans", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (each GPU box here has one MI355X); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    cfg, obs, start, goal = workload(args.workload, seed=args.env_seed)
+    planner = kpax.KPAX(cfg)
+    planner.set_obstacles(obs)
+    seeds = [1000 + i for i in range(args.warmup + args.steps)]
+    for s in seeds[: args.warmup]:
+        planner.plan(start, goal, seed=s)
+
+    planner.set_profiling(True)
+    planner.reset_stats()
+    results = []
+    t0 = time.perf_counter()
+    for s in seeds[args.warmup:]:
+        results.append(planner.plan(start, goal, seed=s))
+    wall = time.perf_counter() - t0
+    st = planner.stats()
+    planner.close()
+
+    ok = [r for r in results if r.status == kpax.OK and r.verified == 1]
+    success = len(ok) / len(results)
+    expansions = sum(r.expansions for r in results)
+    steps_int = sum(r.integrated_steps for r in results)
+    p50 = statistics.median(r.plan_ms for r in ok) if ok else None
+    p99 = float(np.percentile([r.plan_ms for r in ok], 99)) if ok else None
+
+    S, C = cfg.S, cfg.C
+    exp_ms = st["phase_ms"]["expand"]
+    launches = max(1, st["phase_launches"]["expand"])
+    per_launch_exp = st["expansions"] / launches
+    per_launch_ms = exp_ms / launches
+    achieved_gbs = per_launch_exp * bytes_per_expansion(S, C) / (per_launch_ms * 1e-3) / 1e9
+    flops = st["integrated_steps"] * FLOP_PER_STEP[cfg.model]
+    achieved_tflops = flops / (exp_ms * 1e-3) / 1e12
+
+    line = {
+        "metric": METRIC,
+        "value": expansions / wall,
+        "unit": "node-expansions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "12D quadcopter, 400-column forest, [0,10]^2x[0,5], RK4 dt=0.02, "
+                               "2^20-node tree cap, 16^3 regions x 4^3 sub-regions, 64^3 grid",
+                   "plans_per_step": 1, "env_seed": args.env_seed, "max_tree": int(cfg.max_tree)},
+        "p50_time_to_first_solution_ms": p50,
+        "p99_time_to_first_solution_ms": p99,
+        "plans_per_sec": len(results) / wall,
+        "success_rate": success,
+        "mean_iterations": statistics.mean(r.iterations for r in results),
+        "expansions_per_plan": expansions / len(results),
+        "integrated_steps": steps_int,
+        "phase_ms_per_plan": {k: v / max(1, st["plans"]) for k, v in st["phase_ms"].items()},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_expand<quadcopter>", "bytes_per_expansion": bytes_per_expansion(S, C),
+                     "expansions_per_launch": per_launch_exp, "avg_launch_ms": per_launch_ms},
+        "roofline_valu": {"bound": "valu_fp32", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": achieved_tflops / FP32_PEAK_TFLOPS,
+                          "flop_per_step": FLOP_PER_STEP[cfg.model]},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.env_seed, args.cpu_plans, seeds[args.warmup:])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

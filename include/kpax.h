@@ -1,0 +1,269 @@
+/*
+ * kpax.h — C ABI of the MI355X-native Kino-PAX planner (libkpax.so).
+ *
+ * This is the drop-in boundary for the reference's planner hot path.  The
+ * reference (kranzowe/Kino-PAX-Fork) exposes a C++ class hierarchy compiled
+ * with nvcc; every entry point below names the reference interface it
+ * replaces.  Plain pointers and sizes only: host buffers in, host buffers out.
+ * The library owns its device memory and one HIP stream per handle.
+ *
+ * Conventions
+ *   S = state dimension, C = control dimension of the configured model.
+ *   "SoA" arrays are dimension-major: x[d * n + i] is dimension d of item i.
+ *   Every function returns a kpax_status; on failure kpax_last_error()
+ *   describes the cause.  Nothing calls exit() and nothing prints.
+ */
+#ifndef KPAX_H
+#define KPAX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KPAX_ABI_VERSION 1
+
+/* Status codes.  The reference has none (errors are unchecked and a missing
+ * obstacle file calls exit(1), src/helper/helper.cu:18-22). */
+typedef enum kpax_status {
+    KPAX_OK               = 0,
+    KPAX_NOT_FOUND        = 1,  /* plan finished without reaching the goal   */
+    KPAX_BUDGET_EXHAUSTED = 2,  /* tree cap reached before the goal          */
+    KPAX_INVALID_ARG      = -1,
+    KPAX_DEVICE_ERROR     = -2,
+    KPAX_NO_MEMORY        = -3,
+    KPAX_NO_DEVICE        = -4,
+    KPAX_IO_ERROR         = -5
+} kpax_status;
+
+/* Dynamics models.  Numbering follows the reference's MODEL macro
+ * (include/config/config.h:209; dispatch src/statePropagator/statePropagator.cu:375-390). */
+typedef enum kpax_model {
+    KPAX_MODEL_DOUBLE_INTEGRATOR = 1, /* S=6  C=3  statePropagatorSpatialHash.cu:72-134 */
+    KPAX_MODEL_DUBINS_AIRPLANE   = 2, /* S=6  C=3  statePropagatorSpatialHash.cu:139-229 */
+    KPAX_MODEL_QUADCOPTER        = 3  /* S=12 C=4  statePropagatorSpatialHash.cu:234-308 + statePropagator.cu:325-370 */
+} kpax_model;
+
+/* Runtime configuration.  Replaces the compile-time macros of
+ * include/config/config.h:205-308 (one model per binary there). */
+typedef struct kpax_config {
+    int32_t model;             /* kpax_model                                            */
+    int32_t di_textbook;       /* double integrator only: 0 = reference update
+                                  x += (5/6)v dt + (1/3)a dt^2 (statePropagatorSpatialHash.cu:92-94),
+                                  1 = textbook x += v dt + a dt^2/2                      */
+    float   dt;                /* STEP_SIZE                                             */
+    int32_t steps_min;         /* integration steps per edge drawn uniformly in          */
+    int32_t steps_max;         /*   [steps_min, steps_max] (reference: 1 + (int)(u*10))  */
+    float   ws_min[3];         /* workspace box per axis (reference: W_MIN..W_MAX cube)  */
+    float   ws_max[3];
+    float   ctrl_min[4];       /* control box; quad: Zc, Lc, Mc, Nc; DI: ax, ay, az;     */
+    float   ctrl_max[4];       /*   Dubins: a, yawRate, pitchRate (reference draw order) */
+    float   vel_min;           /* per-axis velocity box: DI vx,vy,vz; quad body u,v,w;  */
+    float   vel_max;           /*   Dubins airspeed v (V_MIN/V_MAX)                      */
+    float   pitch_min;         /* Dubins pitch box (DUBINS_AIRPLANE_MIN/MAX_PITCH)       */
+    float   pitch_max;
+    float   mass;              /* quad: MASS, IX, IY, IZ, NU, MU, GRAVITY                */
+    float   inertia[3];
+    float   nu;
+    float   mu;
+    float   gravity;
+    int32_t r1_len[3];         /* R1 regions per workspace axis (W_R1_LENGTH)            */
+    int32_t r2_len[3];         /* R2 sub-regions per R1 per axis (W_R2_LENGTH)           */
+    int32_t grid_dims[3];      /* uniform collision grid cells per axis                  */
+    uint32_t max_tree;         /* MAX_TREE_SIZE                                          */
+    uint32_t max_iter;         /* MAX_ITER                                               */
+    float   goal_radius;       /* GOAL_THRESH (strict <, position distance)              */
+    uint32_t fanout;           /* children per frontier repeat entry (h_activeBlockSize_ = 32) */
+    uint32_t root_repeat;      /* repeat count of the root (KPAX.cu:77, = 5)             */
+    uint32_t sparse_repeat;    /* repeat count of a new node in a sparse region (= 15)   */
+    uint32_t dense_repeat;     /* repeat count otherwise (= 1)                            */
+    int32_t sparse_valid_below;/* region is sparse while valid count < this (= 10)       */
+    float   eps;               /* EPSILON (= 1e-2)                                        */
+    int32_t verify;            /* 1: re-propagate every returned trajectory in FP64 and
+                                  re-check it against all obstacles on the host          */
+    int32_t device;            /* HIP device ordinal                                      */
+} kpax_config;
+
+/* Trajectory returned by kpax_plan.  Replaces the public fields
+ * h_pathToGoal_ / h_controlPathToGoal_ / h_treeSize_ / h_itr_ of
+ * include/planners/Planner.cuh:31-34 (rows there run goal -> root). */
+typedef struct kpax_result {
+    int32_t  status;           /* KPAX_OK, KPAX_NOT_FOUND or KPAX_BUDGET_EXHAUSTED       */
+    uint32_t n_rows;           /* path nodes, root first, goal node last (0 if not found) */
+    uint32_t state_dim;
+    uint32_t control_dim;
+    float*   rows;             /* n_rows x (S + C + 1): [state, control, duration]; the
+                                  root row has zero control and duration (reference row
+                                  format, KPAX.cu:485-500)                               */
+    int32_t* steps;            /* n_rows: integer steps of the edge into each node       */
+    int32_t* node_index;       /* n_rows: tree index of each node                        */
+    int32_t  goal_index;       /* tree index of the goal node (h_pathToGoal_), -1 if none */
+    uint32_t iterations;       /* h_itr_                                                  */
+    uint32_t tree_size;        /* h_treeSize_                                             */
+    uint64_t expansions;       /* children propagated in total                            */
+    uint64_t integrated_steps; /* integration steps those children took                   */
+    double   plan_ms;          /* wall time of kpax_plan including initialisation         */
+    int32_t  verified;         /* -1 not run, 0 failed, 1 passed the FP64 re-check        */
+    float    verify_max_dev;   /* max |FP64 re-propagated - stored FP32| over all rows    */
+} kpax_result;
+
+/* Per-phase device time, accumulated with HIP events on the handle's stream
+ * while profiling is on (kpax_set_profiling).  Phases: 0 expand, 1 score,
+ * 2 append, 3 frontier, 4 init. */
+#define KPAX_N_PHASES 5
+typedef struct kpax_stats {
+    double   phase_ms[KPAX_N_PHASES];
+    uint64_t phase_launches[KPAX_N_PHASES];
+    uint64_t expansions;       /* children propagated by the timed expand launches       */
+    uint64_t integrated_steps; /* integration steps taken by those children               */
+    uint64_t plans;
+} kpax_stats;
+
+/* ---- configuration ---------------------------------------------------- */
+
+/* Fill *out with the built-in workload `workload`:
+ *   1  6D double integrator, [0,1]^3, N = 65,536
+ *   2  12D quadcopter forest, [0,10]^2 x [0,5], N = 2^20
+ *   3  6D Dubins airplane maze, [0,1]^3, N = 2^21
+ * Replaces editing include/config/config.h and recompiling. */
+int kpax_config_workload(int32_t workload, kpax_config* out);
+
+/* Dimensions of a model: *state_dim = S, *control_dim = C. */
+int kpax_model_dims(int32_t model, int32_t* state_dim, int32_t* control_dim);
+
+/* ---- planner lifecycle (Planner / KPAX classes) ------------------------ */
+
+typedef struct kpax_handle kpax_handle;
+
+/* Replaces KPAX::KPAX() (src/planners/KPAX.cu:5-44) and Planner::Planner()
+ * (src/planners/Planner.cu:4-35): allocates the tree and work buffers at the
+ * configured capacity on cfg->device. */
+int kpax_create(const kpax_config* cfg, kpax_handle** out);
+
+/* Replaces KPAX::~KPAX() (KPAX.cu:46-49); frees everything. */
+int kpax_destroy(kpax_handle* h);
+
+/* Message of the last failure on this handle (never NULL). */
+const char* kpax_last_error(const kpax_handle* h);
+
+/* Copy n obstacles [minx,miny,minz,maxx,maxy,maxz] (host array, row format of
+ * helper.cu:13-38) into the handle and build the collision grid on the GPU.
+ * Replaces the caller's cudaMalloc of d_obstacles (examples/gpu/main.cu:25-26)
+ * plus updateSpatialHashGrid (src/collisionCheck/spatialHash.cu:123-139),
+ * which the reference re-runs every iteration (KPAX.cu:306). */
+int kpax_set_obstacles(kpax_handle* h, const float* aabb6, uint32_t n);
+
+/* Plan from start (S floats) to the goal ball goal[0..2] with the configured
+ * radius.  Replaces KPAX::plan(h_initial, h_goal, d_obstacles, n, saveTree)
+ * (include/planners/KPAX.cuh:14, src/planners/KPAX.cu:51-115), with an
+ * explicit seed instead of the wall clock (KPAX.cu:90-91).  On return *out
+ * owns its arrays until kpax_result_free. */
+int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t seed,
+              kpax_result* out);
+
+/* Free the arrays of a result filled by kpax_plan. */
+void kpax_result_free(kpax_result* r);
+
+/* Copy the tree of the last plan: tree_size nodes, SoA states (S x cap),
+ * controls (C x cap), steps, parents, R1 region of each node.  Any pointer
+ * may be NULL.  `cap` is the column stride of the SoA outputs and must be
+ * >= tree_size.  Replaces the per-iteration dumps of KPAX::planBench
+ * (KPAX.cu:545-628). */
+int kpax_get_tree(kpax_handle* h, uint32_t cap, float* states, float* controls,
+                  int32_t* steps, int32_t* parents, int32_t* regions, uint32_t* tree_size);
+
+/* Copy the region graph of the last plan: valid and total child counts per R1
+ * region, covered-R2 count per R1 region, normalised scores per R1 region.
+ * Replaces reading graph_.d_validCounterArray_ / d_counterArray_ /
+ * d_vertexScoreArray_ (include/graphs/Graph.cuh:25-26). */
+int kpax_get_regions(kpax_handle* h, int32_t* valid, int32_t* total, int32_t* covered,
+                     float* scores);
+
+/* Profiling of device phases with HIP events (planDebug, KPAX.cu:117-241). */
+int kpax_set_profiling(kpax_handle* h, int32_t on);
+int kpax_get_stats(kpax_handle* h, kpax_stats* out);
+int kpax_reset_stats(kpax_handle* h);
+
+/* ---- hot-path operators (parity entry points, host buffers) ------------- */
+
+/* Propagate n edges with injected controls and step counts: the body of
+ * propagateAndCheckSpatialHash (statePropagatorSpatialHash.cu:72-308) with
+ * the cuRAND draws replaced by u_soa (C x n) and steps (n).  x0_soa is S x n.
+ * Outputs: x1_soa (S x n, the state after the last step taken; on failure the
+ * failing step's state as in the reference), valid (n), fail_step (n: number
+ * of the step that failed, 1-based, or 0 when valid). */
+int kpax_propagate_batch(kpax_handle* h, const float* x0_soa, const float* u_soa,
+                         const int32_t* steps, uint32_t n, float* x1_soa, uint8_t* valid,
+                         int32_t* fail_step);
+
+/* Segment-vs-obstacle verdicts through the collision grid for n segment
+ * boxes: free[i] = 1 iff the box [bbmin_i, bbmax_i] is strictly separated from
+ * every obstacle on some axis.  Replaces isMotionValidSpatialHash
+ * (spatialHash.cu:145-194) / isMotionValid (collisionCheck.cu:49-62). */
+int kpax_collide_batch(kpax_handle* h, const float* bbmin_soa, const float* bbmax_soa,
+                       uint32_t n, uint8_t* free_out);
+
+/* R1 and R2 region index of n states (S x n SoA).  Replaces getRegion /
+ * getSubRegion (src/graphs/Graph.cu:99-205). */
+int kpax_region_batch(kpax_handle* h, const float* states_soa, uint32_t n, int32_t* r1,
+                      int32_t* r2);
+
+/* Region counters of n children taken as one iteration on an empty coverage
+ * map: total[r1] += 1 for every child, valid[r1] += 1 for valid ones, and
+ * covered[r1] += the number of distinct R2 cells of r1 that valid children
+ * reach.  Counters accumulate into the arrays passed in (R1 entries each).
+ * Replaces the atomics of propagateFrontier_kernel1 (KPAX.cu:386-396). */
+int kpax_region_histogram(kpax_handle* h, const float* states_soa, const uint8_t* valid,
+                          uint32_t n, int32_t* total, int32_t* valid_count, int32_t* covered);
+
+/* Normalised R1 scores from counters (R1 entries each).  Replaces
+ * Graph::updateVertices (src/graphs/Graph.cu:207-358). */
+int kpax_scores(kpax_handle* h, const int32_t* valid_count, const int32_t* total,
+                const int32_t* covered, float* scores);
+
+/* Weighted repeat-expansion: out lists index i weights[i] times, in increasing
+ * i; *total = sum of weights.  out must hold the total.  Replaces the frontier
+ * scan + findInd + repeatInd chain (KPAX.cu:310-321, Planner.cu:52-80). */
+int kpax_repeat_expand(kpax_handle* h, const uint32_t* weights, uint32_t n, uint32_t* out,
+                       uint32_t out_cap, uint32_t* total);
+
+/* Copy the collision grid built by kpax_set_obstacles: cell_start holds
+ * G + 1 offsets (cells x fastest, gridToIndex of spatialHash.cuh:57-60),
+ * items holds the obstacle ids of every cell in ascending order.  Replaces
+ * the fixed 16-slot SpatialHashGrid (spatialHash.cuh:25-37), which drops
+ * obstacles past the 16th of a cell (spatialHash.cu:41-48). */
+int kpax_get_grid(kpax_handle* h, int32_t* cell_start, int32_t* items, uint32_t items_cap,
+                  uint32_t* n_cells, uint32_t* n_items);
+
+/* ---- environments (input formats either side of the path) -------------- */
+
+/* Read an obstacle CSV (rows of 6 comma-separated floats) like
+ * readObstaclesFromCSV (src/helper/helper.cu:13-38).  Call with out = NULL to
+ * get the count, then with an array of 6 * n floats. */
+int kpax_read_obstacles_csv(const char* path, float* out, uint32_t cap, uint32_t* n);
+
+/* Seeded environment generators for the built-in workloads:
+ *   1  random boxes: n AABBs, centres U[ws], edges U[0.05,0.15], clipped to the
+ *      workspace, rejected if they touch the start point or the goal ball
+ *   2  forest: n vertical columns of footprint U[0.1,0.3] over the full height
+ *   3  maze: a dividing wall of tiles with one 0.1-wide passage plus slabs
+ * `start` and `goal` (xyz + radius) are kept clear.  Call with out = NULL to
+ * get the count. */
+int kpax_generate_environment(int32_t kind, uint64_t seed, uint32_t n, const float* ws_min,
+                              const float* ws_max, const float* start, const float* goal,
+                              float* out, uint32_t cap, uint32_t* n_out);
+
+/* Host FP64 re-verification of a trajectory (rows as in kpax_result): each
+ * edge is re-propagated in double from its parent row's stored state and
+ * re-checked against all obstacles; the last row must lie in the goal ball.
+ * *max_dev = max |FP64 state - stored FP32 state|.  Returns KPAX_OK when the
+ * trajectory passes, KPAX_NOT_FOUND when it does not. */
+int kpax_verify_path(const kpax_config* cfg, const float* obstacles, uint32_t n_obs,
+                     const float* rows, const int32_t* steps, uint32_t n_rows,
+                     const float* goal, float* max_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KPAX_H */

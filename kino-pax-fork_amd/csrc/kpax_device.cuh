@@ -1,0 +1,532 @@
+// kpax_device.cuh — device building blocks of the MI355X planner:
+// counter-based RNG, float sin/cos, dynamics with per-step checks, collision
+// through the uniform grid, region location, wave/block scans and the
+// decoupled look-back used by every compaction.
+//
+// Floating-point expressions keep the reference's operation order and the
+// library is compiled with -ffp-contract=off, so a child's state, its region
+// and its verdict are reproducible bit for bit on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "kpax_internal.h"
+
+namespace kpax {
+namespace dev {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Counter = (slot,
+// iteration, query, purpose), key = the 64-bit plan seed.  Zero bytes of
+// generator state in HBM (the reference keeps a 48-byte curandState per tree
+// slot, read and written by every expansion: Planner.cu:18, KPAX.cu:384,398).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                        uint32_t k1)
+{
+#pragma unroll
+    for(int r = 0; r < 10; ++r)
+    {
+        if(r)
+        {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+enum : uint32_t { PURPOSE_CONTROL = 0, PURPOSE_STEPS_ADMIT = 1, PURPOSE_READMIT = 2 };
+
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+// sin and cos of x: Cody-Waite reduction by pi/2 (three float parts applied
+// with fma), then Taylor polynomials (degree 13 / 12) on [-pi/4, pi/4].
+__device__ __forceinline__ void sincos_f32(float x, float* s, float* c)
+{
+    const float k  = __builtin_rintf(x * 0x1.45f306p-1f);
+    const int q    = (int)k;
+    float r        = __builtin_fmaf(-k, 0x1.921fb6p+0f, x);
+    r              = __builtin_fmaf(-k, -0x1.777a5cp-25f, r);
+    r              = __builtin_fmaf(-k, -0x1.ee59dap-50f, r);
+    const float r2 = r * r;
+    float ps       = __builtin_fmaf(r2, 0x1.612462p-33f, -0x1.ae6456p-26f);
+    ps             = __builtin_fmaf(r2, ps, 0x1.71de3ap-19f);
+    ps             = __builtin_fmaf(r2, ps, -0x1.a01a02p-13f);
+    ps             = __builtin_fmaf(r2, ps, 0x1.111112p-7f);
+    ps             = __builtin_fmaf(r2, ps, -0x1.555556p-3f);
+    const float sp = __builtin_fmaf(r * r2, ps, r);
+    float pc       = __builtin_fmaf(r2, 0x1.1eed8ep-29f, -0x1.27e4fcp-22f);
+    pc             = __builtin_fmaf(r2, pc, 0x1.a01a02p-16f);
+    pc             = __builtin_fmaf(r2, pc, -0x1.6c16c2p-10f);
+    pc             = __builtin_fmaf(r2, pc, 0x1.555556p-5f);
+    pc             = __builtin_fmaf(r2, pc, -0.5f);
+    const float cp = __builtin_fmaf(r2, pc, 1.0f);
+    const int qq   = q & 3;
+    const float sa = (qq & 1) ? cp : sp;
+    const float ca = (qq & 1) ? sp : cp;
+    *s             = (qq & 2) ? -sa : sa;
+    *c             = ((qq + 1) & 2) ? -ca : ca;
+}
+
+// Truncation toward zero clamped to [0, len-1], NaN -> 0; identical to the
+// reference's (int) cast + clamp for finite inputs (Graph.cu:107-109).
+__device__ __forceinline__ int clamp_index(float f, int len)
+{
+    if(!(f >= 0.0f)) return 0;
+    if(f >= (float)len) return len - 1;
+    return (int)f;
+}
+
+__device__ __forceinline__ bool inb(float v, float lo, float hi) { return v >= lo && v <= hi; }
+
+// ---------------------------------------------------------------------------
+// Uniform grid in CSR form: start[G+1], items[K] (obstacle ids ascending per
+// cell), obstacle boxes as two float4 arrays.  Built once per environment.
+// ---------------------------------------------------------------------------
+struct GridView {
+    GridParams gp;
+    const int32_t* start;
+    const int32_t* items;
+    const float4* lo;
+    const float4* hi;
+};
+
+__device__ __forceinline__ int cell_of(float x, float mn, float cell, int D)
+{
+    return clamp_index((x - mn) / cell, D);
+}
+
+// Segment box vs all obstacles in the cells it overlaps
+// (isMotionValidSpatialHash, spatialHash.cu:145-194; broad phase
+// collisionCheck.cu:3-10).  Any obstacle that overlaps the box shares at
+// least one cell with it, so the verdict equals the brute-force one.
+__device__ __forceinline__ bool seg_free(const GridView& g, const float bmin[3], const float bmax[3])
+{
+    int lo[3], hi[3];
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        lo[d] = cell_of(bmin[d], g.gp.mn[d], g.gp.cell[d], g.gp.D[d]);
+        hi[d] = cell_of(bmax[d], g.gp.mn[d], g.gp.cell[d], g.gp.D[d]);
+    }
+    for(int z = lo[2]; z <= hi[2]; ++z)
+        for(int y = lo[1]; y <= hi[1]; ++y)
+            for(int x = lo[0]; x <= hi[0]; ++x)
+            {
+                const int c = x + g.gp.D[0] * (y + g.gp.D[1] * z);
+                const int e = g.start[c + 1];
+                for(int k = g.start[c]; k < e; ++k)
+                {
+                    const int o    = g.items[k];
+                    const float4 a = g.lo[o];
+                    const float4 b = g.hi[o];
+                    const bool sep = bmax[0] <= a.x || b.x <= bmin[0] || bmax[1] <= a.y || b.y <= bmin[1] ||
+                                     bmax[2] <= a.z || b.z <= bmin[2];
+                    if(!sep) return false;
+                }
+            }
+    return true;
+}
+
+__device__ __forceinline__ bool ws_and_collision(const ModelParams& P, const GridView& G, const float p0[3],
+                                                 const float p1[3])
+{
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+        if(!inb(p1[d], P.ws_min[d], P.ws_max[d])) return false;
+    float bmin[3], bmax[3];
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        const bool sw = p0[d] > p1[d];
+        bmin[d]       = sw ? p1[d] : p0[d];
+        bmax[d]       = sw ? p0[d] : p1[d];
+    }
+    return seg_free(G, bmin, bmax);
+}
+
+// ---------------------------------------------------------------------------
+// Dynamics.  Each propagator returns the 1-based failing step (0 = valid) and
+// leaves in x1 the state after the last step taken, as the reference does.
+// ---------------------------------------------------------------------------
+template<int MODEL> struct Dims;
+template<> struct Dims<KPAX_MODEL_DOUBLE_INTEGRATOR> {
+    static constexpr int S = 6, C = 3;
+};
+template<> struct Dims<KPAX_MODEL_DUBINS_AIRPLANE> {
+    static constexpr int S = 6, C = 3;
+};
+template<> struct Dims<KPAX_MODEL_QUADCOPTER> {
+    static constexpr int S = 12, C = 4;
+};
+
+// Double integrator: statePropagatorSpatialHash.cu:88-129.
+__device__ __forceinline__ int prop_di(const ModelParams& P, const GridView& G, const float* x0, const float* u,
+                                       int steps, float* x1)
+{
+    const float dt = P.dt;
+    float x = x0[0], y = x0[1], z = x0[2], vx = x0[3], vy = x0[4], vz = x0[5];
+    const float ax = u[0], ay = u[1], az = u[2];
+    int fail = 0;
+    for(int i = 0; i < steps; ++i)
+    {
+        const float p0[3] = {x, y, z};
+        if(!P.di_textbook)
+        {
+            x += (vx + (vx + 2.0f * (vx + ax * dt / 2.0f) + (vx + ax * dt))) * dt / 6.0f;
+            y += (vy + (vy + 2.0f * (vy + ay * dt / 2.0f) + (vy + ay * dt))) * dt / 6.0f;
+            z += (vz + (vz + 2.0f * (vz + az * dt / 2.0f) + (vz + az * dt))) * dt / 6.0f;
+            vx += (ax + 2.0f * ax + 2.0f * ax + ax) * dt / 6.0f;
+            vy += (ay + 2.0f * ay + 2.0f * ay + ay) * dt / 6.0f;
+            vz += (az + 2.0f * az + 2.0f * az + az) * dt / 6.0f;
+        }
+        else
+        {
+            x += vx * dt + 0.5f * ax * dt * dt;
+            y += vy * dt + 0.5f * ay * dt * dt;
+            z += vz * dt + 0.5f * az * dt * dt;
+            vx += ax * dt;
+            vy += ay * dt;
+            vz += az * dt;
+        }
+        if(!inb(vx, P.vmin, P.vmax) || !inb(vy, P.vmin, P.vmax) || !inb(vz, P.vmin, P.vmax))
+        {
+            fail = i + 1;
+            break;
+        }
+        const float p1[3] = {x, y, z};
+        if(!ws_and_collision(P, G, p0, p1))
+        {
+            fail = i + 1;
+            break;
+        }
+    }
+    x1[0] = x, x1[1] = y, x1[2] = z, x1[3] = vx, x1[4] = vy, x1[5] = vz;
+    return fail;
+}
+
+// Dubins airplane: statePropagatorSpatialHash.cu:156-215.
+__device__ __forceinline__ int prop_dubins(const ModelParams& P, const GridView& G, const float* x0, const float* u,
+                                           int steps, float* x1)
+{
+    const float dt = P.dt, h = P.half, d6 = P.d6;
+    float x = x0[0], y = x0[1], z = x0[2], yaw = x0[3], pitch = x0[4], v = x0[5];
+    const float a = u[0], yr = u[1], pr = u[2];
+    int fail = 0;
+    for(int i = 0; i < steps; ++i)
+    {
+        const float p0[3] = {x, y, z};
+        float cp, sp, cy, sy, cpm, spm, cym, sym, cpe, spe, cye, sye;
+        sincos_f32(pitch, &sp, &cp);
+        sincos_f32(yaw, &sy, &cy);
+        sincos_f32(pitch + h * pr, &spm, &cpm);
+        sincos_f32(yaw + h * yr, &sym, &cym);
+        sincos_f32(pitch + dt * pr, &spe, &cpe);
+        sincos_f32(yaw + dt * yr, &sye, &cye);
+        const float A  = v + h * a;
+        const float B  = v + dt * a;
+        const float Tx = A * cpm * cym;
+        const float Ty = A * cpm * sym;
+        const float Tz = A * spm;
+        x += d6 * (v * cp * cy + 2.0f * (Tx + Tx) + B * cpe * cye);
+        y += d6 * (v * cp * sy + 2.0f * (Ty + Ty) + B * cpe * sye);
+        z += d6 * (v * sp + 2.0f * (Tz + Tz) + B * spe);
+        yaw += dt * yr;
+        pitch += dt * pr;
+        v += d6 * (a + 2.0f * (a + a) + a);
+        if(!inb(v, P.vmin, P.vmax) || !inb(pitch, P.pmin, P.pmax))
+        {
+            fail = i + 1;
+            break;
+        }
+        const float p1[3] = {x, y, z};
+        if(!ws_and_collision(P, G, p0, p1))
+        {
+            fail = i + 1;
+            break;
+        }
+    }
+    x1[0] = x, x1[1] = y, x1[2] = z, x1[3] = yaw, x1[4] = pitch, x1[5] = v;
+    return fail;
+}
+
+// 12D quadcopter ode: statePropagator.cu:325-370.
+__device__ __forceinline__ void quad_ode(float* out, const float* x, const float* hh, float delta, const float* u,
+                                         const ModelParams& k)
+{
+    const float phi = x[3] + delta * hh[3], theta = x[4] + delta * hh[4], psi = x[5] + delta * hh[5];
+    const float uu = x[6] + delta * hh[6], v = x[7] + delta * hh[7], w = x[8] + delta * hh[8];
+    const float p = x[9] + delta * hh[9], q = x[10] + delta * hh[10], r = x[11] + delta * hh[11];
+    float sf, cf, st, ct, ss, cs;
+    sincos_f32(phi, &sf, &cf);
+    sincos_f32(theta, &st, &ct);
+    sincos_f32(psi, &ss, &cs);
+    const float tt = st / ct;
+    out[0]         = ct * cs * uu + (sf * st * cs - cf * ss) * v + (cf * st * cs + sf * ss) * w;
+    out[1]         = ct * ss * uu + (sf * st * ss + cf * cs) * v + (cf * st * ss - sf * cs) * w;
+    out[2]         = -st * uu + sf * ct * v + cf * ct * w;
+    out[3]         = p + (q * sf + r * cf) * tt;
+    out[4]         = q * cf - r * sf;
+    out[5]         = (q * sf + r * cf) / ct;
+    const float XYZ = k.negnu * __builtin_sqrtf(uu * uu + v * v + w * w);
+    out[6]          = (r * v - q * w) - k.g * st + k.mi * XYZ * uu;
+    out[7]          = (p * w - r * uu) + k.g * ct * sf + k.mi * XYZ * v;
+    out[8]          = (q * uu - p * v) + k.g * ct * cf + k.mi * XYZ * w + k.mi * u[0];
+    const float LMN = k.negmu * __builtin_sqrtf(p * p + q * q + r * r);
+    out[9]          = k.k1x * q * r + k.ixi * LMN * p + k.ixi * u[1];
+    out[10]         = k.k1y * p * r + k.iyi * LMN * q + k.iyi * u[2];
+    out[11]         = k.k1z * p * q + k.izi * LMN * r + k.izi * u[3];
+}
+
+// Quadcopter RK4: statePropagatorSpatialHash.cu:246-307.
+__device__ __forceinline__ int prop_quad(const ModelParams& P, const GridView& G, const float* x0, const float* u,
+                                         int steps, float* x1)
+{
+    const float dt = P.dt, half = P.half, d6 = P.d6;
+    float h0[12], k1[12], k2[12], k3[12], k4[12];
+    const float zero[12] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for(int j = 0; j < 12; ++j) h0[j] = x0[j];
+    int fail = 0;
+    for(int i = 0; i < steps; ++i)
+    {
+        const float p0[3] = {h0[0], h0[1], h0[2]};
+        quad_ode(k1, h0, zero, 1.0f, u, P);
+        quad_ode(k2, h0, k1, half, u, P);
+        quad_ode(k3, h0, k2, half, u, P);
+        quad_ode(k4, h0, k3, dt, u, P);
+#pragma unroll
+        for(int j = 0; j < 12; ++j) h0[j] += d6 * (k1[j] + 2.0f * k2[j] + 2.0f * k3[j] + k4[j]);
+        if(!inb(h0[6], P.vmin, P.vmax) || !inb(h0[7], P.vmin, P.vmax) || !inb(h0[8], P.vmin, P.vmax))
+        {
+            fail = i + 1;
+            break;
+        }
+        const float p1[3] = {h0[0], h0[1], h0[2]};
+        if(!ws_and_collision(P, G, p0, p1))
+        {
+            fail = i + 1;
+            break;
+        }
+    }
+#pragma unroll
+    for(int j = 0; j < 12; ++j) x1[j] = h0[j];
+    return fail;
+}
+
+template<int MODEL>
+__device__ __forceinline__ int propagate(const ModelParams& P, const GridView& G, const float* x0, const float* u,
+                                         int steps, float* x1)
+{
+    if constexpr(MODEL == KPAX_MODEL_DOUBLE_INTEGRATOR)
+        return prop_di(P, G, x0, u, steps, x1);
+    else if constexpr(MODEL == KPAX_MODEL_DUBINS_AIRPLANE)
+        return prop_dubins(P, G, x0, u, steps, x1);
+    else
+        return prop_quad(P, G, x0, u, steps, x1);
+}
+
+// ---------------------------------------------------------------------------
+// Regions: getRegion / getSubRegion (Graph.cu:99-205) on the workspace axes,
+// x most significant, with the R1 min corner min + idx * size
+// (initializeRegions_kernel, Graph.cu:67-70) computed inline.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void locate(const RegionParams& R, const float* p, int* r1, int* r2)
+{
+    int idx[3], jdx[3];
+#pragma unroll
+    for(int d = 0; d < 3; ++d) idx[d] = clamp_index((float)R.L[d] * (p[d] - R.mn[d]) / R.ext[d], R.L[d]);
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        const float lo = R.mn[d] + (float)idx[d] * R.size[d];
+        jdx[d]         = clamp_index((float)R.L2[d] * (p[d] - lo) / R.size[d], R.L2[d]);
+    }
+    *r1 = (idx[0] * R.L[1] + idx[1]) * R.L[2] + idx[2];
+    *r2 = *r1 * R.r2_per + (jdx[0] * R.L2[1] + jdx[1]) * R.L2[2] + jdx[2];
+}
+
+// Region counters of one child (KPAX.cu:386-396).  Sub-region novelty is the
+// outcome of the reference's read-then-exchange (KPAX.cu:394-395) when the
+// children run one at a time in slot order: only the smallest slot among the
+// valid children that reach an uncovered R2 cell sees it empty.  Each cell
+// keeps (first iteration << 32 | smallest slot), lowered with atomicMin; a
+// cell stamped with an earlier iteration is settled, so most children decide
+// with one plain read.  Returns kValid | kCandidate; a candidate is confirmed
+// after the launch by comparing the cell's word with its own key.
+constexpr uint32_t kValid = 1u, kCandidate = 2u, kScoreAdmit = 4u;
+
+__device__ __forceinline__ uint64_t cell_key(uint32_t t, uint32_t slot) { return ((uint64_t)t << 32) | slot; }
+
+__device__ __forceinline__ uint32_t count_child(int32_t* total, int32_t* valid, int32_t* covered,
+                                                uint64_t* r2_state, int r1, int r2, bool ok, uint32_t t,
+                                                uint32_t slot)
+{
+    atomicAdd(&total[r1], 1);
+    if(!ok) return 0u;
+    atomicAdd(&valid[r1], 1);
+    const uint64_t seen = __hip_atomic_load(&r2_state[r2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if((uint32_t)(seen >> 32) < t) return kValid;
+    const uint64_t old = atomicMin((unsigned long long*)&r2_state[r2], (unsigned long long)cell_key(t, slot));
+    if(old == ~0ull) atomicAdd(&covered[r1], 1);
+    return kValid | kCandidate;
+}
+
+// ---------------------------------------------------------------------------
+// Scans.  256-thread blocks (4 waves of 64) scan 1024-item tiles; tiles are
+// handed out in order by an atomic ticket and chained with a decoupled
+// look-back whose status words carry the aggregate (flag 1) or inclusive
+// prefix (flag 2) in their low 32 bits.  A tile only waits on tiles with
+// smaller tickets, which are already resident, so every wait ends.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFlagAgg    = 1ull << 62;
+constexpr uint64_t kFlagPrefix = 2ull << 62;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for(int off = 1; off < 64; off <<= 1)
+    {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if(lane >= off) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+#pragma unroll
+    for(int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block; *agg gets
+// the block total.  lds must hold 4 words; contains two barriers.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* agg, uint32_t* lds)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if(lane == 63) lds[wid] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for(int w = 0; w < 4; ++w)
+        if(w < wid) before += lds[w];
+    *agg = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return before + inc - v;
+}
+
+__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t status_load(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by all 64 lanes of wave 0; returns the exclusive prefix of `tile`.
+__device__ __forceinline__ uint32_t lookback(uint64_t* status, uint32_t tile, uint32_t agg)
+{
+    const int lane = threadIdx.x & 63;
+    if(tile == 0)
+    {
+        if(lane == 0) status_store(&status[0], kFlagPrefix | agg);
+        return 0;
+    }
+    if(lane == 0) status_store(&status[tile], kFlagAgg | agg);
+    uint32_t excl = 0;
+    int64_t base  = (int64_t)tile - 1;
+    while(true)
+    {
+        const int64_t j = base - lane;
+        uint64_t s      = kFlagPrefix;  // before tile 0: an empty prefix
+        if(j >= 0)
+        {
+            s = status_load(&status[j]);
+            while((s >> 62) == 0) s = status_load(&status[j]);
+        }
+        const uint64_t pm = __ballot((s >> 62) == 2);
+        const int stop    = pm ? __builtin_ctzll(pm) : 64;
+        excl += wave_sum(lane <= stop ? (uint32_t)s : 0u);
+        if(pm) break;
+        base -= 64;
+    }
+    if(lane == 0) status_store(&status[tile], kFlagPrefix | (excl + agg));
+    return excl;
+}
+
+// Block-wide: exclusive prefix of this tile given its aggregate.
+__device__ __forceinline__ uint32_t tile_prefix(uint64_t* status, uint32_t tile, uint32_t agg, uint32_t* lds_word)
+{
+    if(threadIdx.x < 64)
+    {
+        const uint32_t e = lookback(status, tile, agg);
+        if(threadIdx.x == 0) *lds_word = e;
+    }
+    __syncthreads();
+    const uint32_t e = *lds_word;
+    __syncthreads();
+    return e;
+}
+
+// Next dynamic tile for the whole block (one atomic per block).
+__device__ __forceinline__ uint32_t next_tile(uint32_t* counter, uint32_t* lds_word)
+{
+    if(threadIdx.x == 0) *lds_word = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t t = *lds_word;
+    __syncthreads();
+    return t;
+}
+
+// fAccept of updateFrontier (KPAX.cu:527-528): float iteration * eps times the
+// double fifth power of the remaining tree fraction.
+__device__ __forceinline__ float f_accept(uint32_t t, uint32_t tree_plus_new, const PlanParams& PP)
+{
+    const float tas = 1.0f - ((float)tree_plus_new / (float)PP.N);
+    const double x  = (double)tas;
+    const double p5 = x * x * x * x * x;
+    return (float)((double)((float)t * PP.eps) * p5);
+}
+
+// Fan-out of the next iteration (KPAX.cu:323-333), capped so the children
+// never exceed the remaining tree capacity.
+__device__ __forceinline__ Iter make_iter(uint32_t t, uint32_t ts, uint32_t W, const PlanParams& PP)
+{
+    Iter n;
+    n.t                      = t;
+    n.ts                     = ts;
+    n.W                      = W;
+    const uint32_t remaining = PP.N - ts;
+    uint32_t fan             = PP.fanout;
+    uint64_t M               = (uint64_t)W * fan;
+    if(M > remaining)
+    {
+        fan = (uint32_t)(int)((float)remaining / (float)W);
+        if(fan > PP.fanout) fan = PP.fanout;
+        if(fan == 0) fan = 1;
+        M = (uint64_t)W * fan;
+        if(M > remaining) M = remaining;
+    }
+    n.M      = (uint32_t)M;
+    n.fan    = fan;
+    n.status = t > PP.max_iter ? KPAX_NOT_FOUND : (remaining == 0 ? KPAX_BUDGET_EXHAUSTED : 0);
+    return n;
+}
+
+}  // namespace dev
+}  // namespace kpax

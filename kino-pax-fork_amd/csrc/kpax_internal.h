@@ -1,0 +1,92 @@
+// kpax_internal.h — structures shared by the HIP planner (kpax_device.hip)
+// and the host-only parts of libkpax (kpax_host.cpp).  Not part of the ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/kpax.h"
+
+namespace kpax {
+
+constexpr int kMaxS = 12;
+constexpr int kMaxC = 4;
+
+// Per-iteration control record.  The frontier kernel of iteration t writes
+// `nxt`; the one-block advance kernel copies it to `cur` at the start of
+// iteration t+1, so no kernel ever changes a field its own blocks read.
+struct Iter {
+    uint32_t t;       // iteration number, 1-based
+    uint32_t ts;      // tree size at the start of the iteration
+    uint32_t W;       // length of the frontier repeat list (may exceed its capacity)
+    uint32_t M;       // children expanded in this iteration
+    uint32_t fan;     // children per repeat entry
+    int32_t status;   // 0 = running, else the final kpax_status
+};
+
+struct Ctl {
+    Iter cur;
+    Iter nxt;
+    int32_t goal;          // smallest tree index inside the goal ball, INT32_MAX if none
+    uint32_t Fn;           // children appended in the current iteration
+    float f_accept;        // re-admission slack of the current iteration
+    float score_total;     // sum of raw region scores (current iteration)
+    uint64_t expansions;   // children propagated since the plan started
+    uint64_t steps_done;   // integration steps taken by those children
+    uint32_t iterations;   // last iteration that ran
+    uint32_t tile_expand;  // dynamic tile counters, cleared by the advance kernel
+    uint32_t tile_append;
+    uint32_t tile_frontier;
+    uint32_t path_len;     // rows written by the path kernel
+    uint32_t pad[3];
+};
+
+// Dynamics and checks, with every constant folded on the host exactly as the
+// reference's macros fold in float.
+struct ModelParams {
+    int32_t model;
+    int32_t di_textbook;
+    float dt, half, d6;  // STEP_SIZE, 0.5f * STEP_SIZE, STEP_SIZE / 6.0f
+    int32_t steps_min, span;
+    float ws_min[3], ws_max[3];
+    float ctrl_min[4], ctrl_rng[4];
+    float vmin, vmax, pmin, pmax;
+    float negnu, negmu, g, mi, k1x, k1y, k1z, ixi, iyi, izi;
+};
+
+struct RegionParams {
+    int32_t L[3], L2[3];
+    float mn[3], ext[3], size[3];
+    int32_t r1_count, r2_per;
+    float vol;
+};
+
+struct GridParams {
+    int32_t D[3];
+    float mn[3], cell[3];
+};
+
+// Planner parameters that are not per-model.
+struct PlanParams {
+    uint32_t N;             // tree cap
+    uint32_t max_iter;
+    uint32_t fanout, root_repeat, sparse_repeat, dense_repeat;
+    int32_t sparse_valid_below;
+    float eps;
+    float goal_radius;
+};
+
+int state_dim(int model);
+int control_dim(int model);
+std::string validate_config(const kpax_config& c);
+ModelParams make_model_params(const kpax_config& c);
+RegionParams make_region_params(const kpax_config& c);
+GridParams make_grid_params(const kpax_config& c);
+PlanParams make_plan_params(const kpax_config& c);
+
+// Host FP64 re-verification (kpax_host.cpp).
+bool verify_path_fp64(const kpax_config& c, const float* obs, uint32_t n_obs, const float* rows,
+                      const int32_t* steps, uint32_t n_rows, const float* goal, float* max_dev);
+
+}  // namespace kpax

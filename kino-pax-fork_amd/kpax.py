@@ -1,0 +1,414 @@
+"""Python host mirror of the Kino-PAX planner over the libkpax.so C ABI.
+
+The class `KPAX` keeps the reference's planner interface
+(include/planners/KPAX.cuh:14: plan(h_initial, h_goal, d_obstacles, n)) and
+exposes the hot-path operators the C ABI exports.  All arrays are numpy;
+state/control batches are SoA (dimension-major, shape (S, n)).
+
+The library is loaded from this directory and nothing falls back to another
+implementation: if libkpax.so is missing or a call fails, an exception says so.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkpax.so")
+
+STATUS = {0: "OK", 1: "NOT_FOUND", 2: "BUDGET_EXHAUSTED", -1: "INVALID_ARG", -2: "DEVICE_ERROR",
+          -3: "NO_MEMORY", -4: "NO_DEVICE", -5: "IO_ERROR"}
+OK, NOT_FOUND, BUDGET_EXHAUSTED = 0, 1, 2
+MODEL_DOUBLE_INTEGRATOR, MODEL_DUBINS_AIRPLANE, MODEL_QUADCOPTER = 1, 2, 3
+N_PHASES = 5
+PHASES = ("expand", "score", "append", "frontier", "init")
+
+
+class Config(ctypes.Structure):
+    """kpax_config (include/kpax.h)."""
+    _fields_ = [
+        ("model", ctypes.c_int32), ("di_textbook", ctypes.c_int32), ("dt", ctypes.c_float),
+        ("steps_min", ctypes.c_int32), ("steps_max", ctypes.c_int32),
+        ("ws_min", ctypes.c_float * 3), ("ws_max", ctypes.c_float * 3),
+        ("ctrl_min", ctypes.c_float * 4), ("ctrl_max", ctypes.c_float * 4),
+        ("vel_min", ctypes.c_float), ("vel_max", ctypes.c_float),
+        ("pitch_min", ctypes.c_float), ("pitch_max", ctypes.c_float),
+        ("mass", ctypes.c_float), ("inertia", ctypes.c_float * 3), ("nu", ctypes.c_float),
+        ("mu", ctypes.c_float), ("gravity", ctypes.c_float),
+        ("r1_len", ctypes.c_int32 * 3), ("r2_len", ctypes.c_int32 * 3), ("grid_dims", ctypes.c_int32 * 3),
+        ("max_tree", ctypes.c_uint32), ("max_iter", ctypes.c_uint32), ("goal_radius", ctypes.c_float),
+        ("fanout", ctypes.c_uint32), ("root_repeat", ctypes.c_uint32), ("sparse_repeat", ctypes.c_uint32),
+        ("dense_repeat", ctypes.c_uint32), ("sparse_valid_below", ctypes.c_int32), ("eps", ctypes.c_float),
+        ("verify", ctypes.c_int32), ("device", ctypes.c_int32),
+    ]
+
+    def copy(self):
+        c = Config()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(self), ctypes.sizeof(Config))
+        return c
+
+    @property
+    def S(self):
+        return 12 if self.model == MODEL_QUADCOPTER else 6
+
+    @property
+    def C(self):
+        return 4 if self.model == MODEL_QUADCOPTER else 3
+
+    @property
+    def r1_count(self):
+        return int(np.prod(list(self.r1_len)))
+
+    @property
+    def r2_per(self):
+        return int(np.prod(list(self.r2_len)))
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32), ("n_rows", ctypes.c_uint32), ("state_dim", ctypes.c_uint32),
+        ("control_dim", ctypes.c_uint32), ("rows", ctypes.POINTER(ctypes.c_float)),
+        ("steps", ctypes.POINTER(ctypes.c_int32)), ("node_index", ctypes.POINTER(ctypes.c_int32)),
+        ("goal_index", ctypes.c_int32), ("iterations", ctypes.c_uint32), ("tree_size", ctypes.c_uint32),
+        ("expansions", ctypes.c_uint64), ("integrated_steps", ctypes.c_uint64), ("plan_ms", ctypes.c_double),
+        ("verified", ctypes.c_int32),
+        ("verify_max_dev", ctypes.c_float),
+    ]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("phase_ms", ctypes.c_double * N_PHASES), ("phase_launches", ctypes.c_uint64 * N_PHASES),
+                ("expansions", ctypes.c_uint64), ("integrated_steps", ctypes.c_uint64), ("plans", ctypes.c_uint64)]
+
+
+class KpaxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libkpax.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KpaxError(f"{LIB_PATH} is missing: build it with kino-pax-fork_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    u32, i32, u64 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
+    sig = {
+        "kpax_config_workload": (i32, [i32, P]),
+        "kpax_model_dims": (i32, [i32, P, P]),
+        "kpax_create": (i32, [P, P]),
+        "kpax_destroy": (i32, [P]),
+        "kpax_last_error": (ctypes.c_char_p, [P]),
+        "kpax_set_obstacles": (i32, [P, P, u32]),
+        "kpax_plan": (i32, [P, P, P, u64, P]),
+        "kpax_result_free": (None, [P]),
+        "kpax_get_tree": (i32, [P, u32, P, P, P, P, P, P]),
+        "kpax_get_regions": (i32, [P, P, P, P, P]),
+        "kpax_get_grid": (i32, [P, P, P, u32, P, P]),
+        "kpax_set_profiling": (i32, [P, i32]),
+        "kpax_get_stats": (i32, [P, P]),
+        "kpax_reset_stats": (i32, [P]),
+        "kpax_propagate_batch": (i32, [P, P, P, P, u32, P, P, P]),
+        "kpax_collide_batch": (i32, [P, P, P, u32, P]),
+        "kpax_region_batch": (i32, [P, P, u32, P, P]),
+        "kpax_region_histogram": (i32, [P, P, P, u32, P, P, P]),
+        "kpax_scores": (i32, [P, P, P, P, P]),
+        "kpax_repeat_expand": (i32, [P, P, u32, P, u32, P]),
+        "kpax_read_obstacles_csv": (i32, [ctypes.c_char_p, P, u32, P]),
+        "kpax_generate_environment": (i32, [i32, u64, u32, P, P, P, P, P, u32, P]),
+        "kpax_verify_path": (i32, [P, P, u32, P, P, u32, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def config_workload(k):
+    """Built-in workload 1 (double integrator), 2 (quadcopter) or 3 (Dubins)."""
+    c = Config()
+    rc = lib().kpax_config_workload(int(k), ctypes.byref(c))
+    if rc != OK:
+        raise KpaxError(f"unknown workload {k}")
+    return c
+
+
+def model_dims(model):
+    s, c = ctypes.c_int32(), ctypes.c_int32()
+    if lib().kpax_model_dims(int(model), ctypes.byref(s), ctypes.byref(c)) != OK:
+        raise KpaxError(f"unknown model {model}")
+    return s.value, c.value
+
+
+def read_obstacles_csv(path):
+    """readObstaclesFromCSV (src/helper/helper.cu:13-38) -> (n, 6) float32."""
+    n = ctypes.c_uint32()
+    rc = lib().kpax_read_obstacles_csv(str(path).encode(), None, 0, ctypes.byref(n))
+    if rc != OK:
+        raise KpaxError(f"cannot read obstacles from {path}: {STATUS.get(rc, rc)}")
+    out = np.zeros((n.value, 6), np.float32)
+    rc = lib().kpax_read_obstacles_csv(str(path).encode(), _p(out), n.value, ctypes.byref(n))
+    if rc != OK:
+        raise KpaxError(f"cannot read obstacles from {path}: {STATUS.get(rc, rc)}")
+    return out
+
+
+def generate_environment(kind, seed, n, ws_min, ws_max, start=None, goal=None):
+    """Seeded obstacle sets: 1 random boxes, 2 forest, 3 maze (see include/kpax.h)."""
+    wmin, wmax = _f32(ws_min), _f32(ws_max)
+    st = None if start is None else _f32(start)
+    gl = None if goal is None else _f32(goal)
+    cnt = ctypes.c_uint32()
+    args = (int(kind), int(seed), int(n), _p(wmin), _p(wmax), _p(st), _p(gl))
+    rc = lib().kpax_generate_environment(*args, None, 0, ctypes.byref(cnt))
+    if rc != OK:
+        raise KpaxError(f"generate_environment failed: {STATUS.get(rc, rc)}")
+    out = np.zeros((cnt.value, 6), np.float32)
+    rc = lib().kpax_generate_environment(*args, _p(out), cnt.value, ctypes.byref(cnt))
+    if rc != OK:
+        raise KpaxError(f"generate_environment failed: {STATUS.get(rc, rc)}")
+    return out
+
+
+def verify_path(cfg, obstacles, rows, steps, goal):
+    """Host FP64 re-propagation and re-collision-check of a trajectory."""
+    obs = _f32(obstacles).reshape(-1, 6)
+    rows = _f32(rows)
+    steps = np.ascontiguousarray(steps, np.int32)
+    g = _f32(goal)
+    dev = ctypes.c_float()
+    rc = lib().kpax_verify_path(ctypes.byref(cfg), _p(obs), obs.shape[0], _p(rows), _p(steps), steps.shape[0],
+                                _p(g), ctypes.byref(dev))
+    if rc not in (OK, NOT_FOUND):
+        raise KpaxError(f"verify_path: {STATUS.get(rc, rc)}")
+    return rc == OK, float(dev.value)
+
+
+@dataclass
+class PlanResult:
+    """What KPAX::plan leaves in h_pathToGoal_ / h_controlPathToGoal_ / h_treeSize_ / h_itr_
+    (include/planners/Planner.cuh:31-34), plus timing and verification."""
+    status: int
+    rows: np.ndarray          # (n_rows, S + C + 1) root -> goal
+    steps: np.ndarray         # (n_rows,) int32
+    node_index: np.ndarray    # (n_rows,) tree index of each row
+    goal_index: int
+    iterations: int
+    tree_size: int
+    expansions: int
+    integrated_steps: int
+    plan_ms: float
+    verified: int
+    verify_max_dev: float
+    state_dim: int = 0
+    control_dim: int = 0
+
+    @property
+    def found(self):
+        return self.status == OK
+
+    @property
+    def path_to_goal(self):
+        """h_pathToGoal_: tree index of the goal node, 0 when none (the reference's convention)."""
+        return self.goal_index if self.goal_index > 0 else 0
+
+    @property
+    def control_path_to_goal(self):
+        """h_controlPathToGoal_ rows, goal -> root, each [state, control, duration]."""
+        return self.rows[::-1].copy()
+
+
+class KPAX:
+    """MI355X Kino-PAX planner (mirrors the reference's KPAX class)."""
+
+    def __init__(self, config=2):
+        cfg = config_workload(config) if isinstance(config, int) else config.copy()
+        self.config = cfg
+        self.S, self.C = model_dims(cfg.model)
+        h = ctypes.c_void_p()
+        rc = lib().kpax_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != OK:
+            raise KpaxError(f"kpax_create failed: {STATUS.get(rc, rc)}")
+        self._h = h
+        self._obs = None
+        self._obs_src = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kpax_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != OK:
+            msg = lib().kpax_last_error(self._h)
+            raise KpaxError(f"{what}: {STATUS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    # ---- planner ----------------------------------------------------------
+    def set_obstacles(self, obstacles):
+        obs = _f32(obstacles).reshape(-1, 6)
+        self._check(lib().kpax_set_obstacles(self._h, _p(obs), obs.shape[0]), "set_obstacles")
+        self._obs = obs
+
+    def plan(self, initial, goal, obstacles=None, seed=0):
+        """plan(h_initial, h_goal, obstacles) -> PlanResult (KPAX.cu:51-115).
+
+        `initial` holds at least S floats (the reference passes SAMPLE_DIM);
+        `goal` at least the xyz centre of the goal ball."""
+        if obstacles is not None and (self._obs is None or obstacles is not self._obs_src):
+            self.set_obstacles(obstacles)
+            self._obs_src = obstacles
+        if self._obs is None:
+            self.set_obstacles(np.zeros((0, 6), np.float32))
+        x0 = _f32(np.asarray(initial, np.float32).ravel()[: self.S])
+        if x0.size < self.S:
+            raise KpaxError(f"initial state needs {self.S} values")
+        g = _f32(np.asarray(goal, np.float32).ravel()[:3])
+        r = _Result()
+        self._check(lib().kpax_plan(self._h, _p(x0), _p(g), int(seed) & (2**64 - 1), ctypes.byref(r)), "plan")
+        D = self.S + self.C + 1
+        n = r.n_rows
+        rows = np.ctypeslib.as_array(r.rows, (n * D,)).reshape(n, D).copy() if n else np.zeros((0, D), np.float32)
+        steps = np.ctypeslib.as_array(r.steps, (n,)).copy() if n else np.zeros(0, np.int32)
+        idx = np.ctypeslib.as_array(r.node_index, (n,)).copy() if n else np.zeros(0, np.int32)
+        out = PlanResult(r.status, rows, steps, idx, r.goal_index, r.iterations, r.tree_size, r.expansions,
+                         r.integrated_steps, r.plan_ms, r.verified, r.verify_max_dev, self.S, self.C)
+        lib().kpax_result_free(ctypes.byref(r))
+        return out
+
+    def tree(self):
+        """Tree of the last plan: states (S, n), controls (C, n), steps, parents, regions."""
+        ts = ctypes.c_uint32()
+        self._check(lib().kpax_get_tree(self._h, 0, None, None, None, None, None, ctypes.byref(ts)), "get_tree")
+        n = ts.value
+        st = np.zeros((self.S, n), np.float32)
+        ct = np.zeros((self.C, n), np.float32)
+        steps = np.zeros(n, np.int32)
+        par = np.zeros(n, np.int32)
+        reg = np.zeros(n, np.int32)
+        self._check(lib().kpax_get_tree(self._h, n, _p(st), _p(ct), _p(steps), _p(par), _p(reg), ctypes.byref(ts)),
+                    "get_tree")
+        return {"states": st, "controls": ct, "steps": steps, "parents": par, "regions": reg}
+
+    def regions(self):
+        n = self.config.r1_count
+        v, t, c = (np.zeros(n, np.int32) for _ in range(3))
+        s = np.zeros(n, np.float32)
+        self._check(lib().kpax_get_regions(self._h, _p(v), _p(t), _p(c), _p(s)), "get_regions")
+        return {"valid": v, "total": t, "covered": c, "scores": s}
+
+    def grid(self):
+        nc, nk = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(lib().kpax_get_grid(self._h, None, None, 0, ctypes.byref(nc), ctypes.byref(nk)), "get_grid")
+        start = np.zeros(nc.value + 1, np.int32)
+        items = np.zeros(max(nk.value, 1), np.int32)
+        self._check(lib().kpax_get_grid(self._h, _p(start), _p(items), items.size, ctypes.byref(nc),
+                                        ctypes.byref(nk)), "get_grid")
+        return start, items[: nk.value]
+
+    def set_profiling(self, on=True):
+        self._check(lib().kpax_set_profiling(self._h, int(bool(on))), "set_profiling")
+
+    def stats(self):
+        s = _Stats()
+        self._check(lib().kpax_get_stats(self._h, ctypes.byref(s)), "get_stats")
+        return {"phase_ms": dict(zip(PHASES, list(s.phase_ms))),
+                "phase_launches": dict(zip(PHASES, list(s.phase_launches))),
+                "expansions": s.expansions, "integrated_steps": s.integrated_steps, "plans": s.plans}
+
+    def reset_stats(self):
+        self._check(lib().kpax_reset_stats(self._h), "reset_stats")
+
+    # ---- hot-path operators -------------------------------------------------
+    def propagate_batch(self, x0, u, steps):
+        """Injected-control propagation: x0 (S, n), u (C, n), steps (n,) ->
+        x1 (S, n), valid (n,) bool, fail_step (n,)."""
+        steps = np.ascontiguousarray(steps, np.int32)
+        n = steps.shape[0]
+        x0 = _f32(x0, (self.S, n))
+        u = _f32(u, (self.C, n))
+        x1 = np.zeros((self.S, n), np.float32)
+        valid = np.zeros(n, np.uint8)
+        fail = np.zeros(n, np.int32)
+        self._check(lib().kpax_propagate_batch(self._h, _p(x0), _p(u), _p(steps), n, _p(x1), _p(valid), _p(fail)),
+                    "propagate_batch")
+        return x1, valid.astype(bool), fail
+
+    def collide_batch(self, bbmin, bbmax):
+        bbmin = _f32(bbmin)
+        n = bbmin.shape[1] if bbmin.ndim == 2 else 0
+        bbmin = bbmin.reshape(3, n)
+        bbmax = _f32(bbmax, (3, n))
+        out = np.zeros(n, np.uint8)
+        self._check(lib().kpax_collide_batch(self._h, _p(bbmin), _p(bbmax), n, _p(out)), "collide_batch")
+        return out.astype(bool)
+
+    def region_batch(self, states):
+        st = _f32(states)
+        n = st.shape[1]
+        st = np.ascontiguousarray(st[:3])
+        r1 = np.zeros(n, np.int32)
+        r2 = np.zeros(n, np.int32)
+        self._check(lib().kpax_region_batch(self._h, _p(st), n, _p(r1), _p(r2)), "region_batch")
+        return r1, r2
+
+    def region_histogram(self, states, valid, total=None, valid_count=None, covered=None):
+        st = _f32(states)
+        n = st.shape[1]
+        st = np.ascontiguousarray(st[:3])
+        v = np.ascontiguousarray(valid, np.uint8)
+        R1 = self.config.r1_count
+        total = np.zeros(R1, np.int32) if total is None else np.array(total, np.int32)
+        valid_count = np.zeros(R1, np.int32) if valid_count is None else np.array(valid_count, np.int32)
+        covered = np.zeros(R1, np.int32) if covered is None else np.array(covered, np.int32)
+        self._check(lib().kpax_region_histogram(self._h, _p(st), _p(v), n, _p(total), _p(valid_count), _p(covered)),
+                    "region_histogram")
+        return total, valid_count, covered
+
+    def scores(self, valid_count, total, covered):
+        R1 = self.config.r1_count
+        v = np.ascontiguousarray(valid_count, np.int32)
+        t = np.ascontiguousarray(total, np.int32)
+        c = np.ascontiguousarray(covered, np.int32)
+        out = np.zeros(R1, np.float32)
+        self._check(lib().kpax_scores(self._h, _p(v), _p(t), _p(c), _p(out)), "scores")
+        return out
+
+    def repeat_expand(self, weights, cap=None):
+        w = np.ascontiguousarray(weights, np.uint32)
+        cap = int(w.sum(dtype=np.uint64)) if cap is None else int(cap)
+        out = np.zeros(max(cap, 1), np.uint32)
+        total = ctypes.c_uint32()
+        self._check(lib().kpax_repeat_expand(self._h, _p(w), w.shape[0], _p(out), cap, ctypes.byref(total)),
+                    "repeat_expand")
+        return out[: min(cap, total.value)], total.value

@@ -1,0 +1,881 @@
+// kpax_oracle.cpp — CPU oracle of the Kino-PAX hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load liboracle_kpax.so.  The product (libkpax.so) never links or calls it.
+//
+// It restates, in plain sequential C++17, the reference's planner iteration
+// (kranzowe/Kino-PAX-Fork; citations are path:line in that repository):
+//   propagators   src/statePropagator/statePropagatorSpatialHash.cu:72-308,
+//                 src/statePropagator/statePropagator.cu:325-370 (quad ode)
+//   collision     src/collisionCheck/collisionCheck.cu:3-10,49-62 (broad phase)
+//                 src/collisionCheck/spatialHash.cu:9-54,145-194 (grid)
+//                 include/collisionCheck/spatialHash.cuh:42-60
+//   regions       src/graphs/Graph.cu:99-205, 53-97
+//   scores        src/graphs/Graph.cu:233-358 (the evident intent, see below)
+//   compaction    src/planners/Planner.cu:52-80, KPAX.cu:303-351
+//   iteration     src/planners/KPAX.cu:51-115, 358-543
+//   goal metric   include/helper/helper.cuh:138-152
+// Every floating-point expression keeps the reference's operation order and is
+// compiled without contraction (-ffp-contract=off), templated on float (the
+// precision the planner runs in) and double (the truth the float path is
+// measured against).
+//
+// Deliberate, documented departures from the reference (each one makes the
+// planner deterministic or removes undefined behaviour):
+//   * RNG: Philox4x32-10 (Salmon et al., SC'11) keyed by the seed and counted
+//     by (slot, iteration, query, purpose), replacing per-slot cuRAND XORWOW
+//     state seeded from the wall clock (Planner.cu:37-50, KPAX.cu:90-91).
+//   * steps = steps_min + floor(u * (steps_max - steps_min + 1)), u in [0,1)
+//     (reference: 1 + (int)(u * 10) with u in (0,1], statePropagatorSpatialHash.cu:77).
+//   * float sin/cos: the project's own Cody-Waite + Taylor routine below (the
+//     reference calls CUDA's cosf/sinf, whose last-ulp results cannot be
+//     reproduced on any other machine).  tan(t) = sin(t)/cos(t).
+//   * Non-finite states fail the bound checks (NaN compares let them pass in
+//     the reference, statePropagator.cu:282-293).
+//   * Sub-region novelty (KPAX.cu:394-395, a read-then-exchange race) is
+//     decided as if the expansion threads ran one at a time in slot order:
+//     only the first valid child to reach an uncovered R2 cell counts as new.
+//   * Score total: the sum of all R1 scores in a fixed order (1024 strided
+//     partial sums, then a pairwise tree), reset every iteration (the
+//     reference's reduction is undefined behaviour, Graph.cu:260-267,285,314).
+//   * Goal: the smallest tree index among new nodes inside the goal ball
+//     (the reference keeps whichever thread writes last, KPAX.cu:482-501).
+//   * Every expanded parent leaves the frontier (kernel2 clears arbitrary
+//     indices, KPAX.cu:411); children never overflow the tree cap
+//     (KPAX.cu:327-333 can).
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#include <cstdio>
+#include <cstdlib>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/kpax.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC'11).
+// ---------------------------------------------------------------------------
+struct U4 {
+    uint32_t v[4];
+};
+
+inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
+{
+    for(int r = 0; r < 10; ++r)
+    {
+        if(r)
+        {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.v[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.v[2];
+        U4 n;
+        n.v[0] = (uint32_t)(p1 >> 32) ^ c.v[1] ^ k0;
+        n.v[1] = (uint32_t)p1;
+        n.v[2] = (uint32_t)(p0 >> 32) ^ c.v[3] ^ k1;
+        n.v[3] = (uint32_t)p0;
+        c      = n;
+    }
+    return c;
+}
+
+inline float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+enum : uint32_t { PURPOSE_CONTROL = 0, PURPOSE_STEPS_ADMIT = 1, PURPOSE_READMIT = 2 };
+
+inline U4 draw(uint64_t seed, uint32_t slot, uint32_t iter, uint32_t query, uint32_t purpose)
+{
+    U4 c = {{slot, iter, query, purpose}};
+    return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// ---------------------------------------------------------------------------
+// float sin/cos: reduction by pi/2 with a three-part Cody-Waite split applied
+// through fmaf, then Taylor polynomials of degree 13 (sin) and 12 (cos) on
+// [-pi/4, pi/4], evaluated by Horner's rule with fmaf.  Coefficients are the
+// float roundings of (-1)^k / n!.  The truncation error of both series on
+// [-pi/4, pi/4] is below 1e-11, far under one float ulp.
+// ---------------------------------------------------------------------------
+inline void sincos_f32(float x, float* s, float* c)
+{
+    const float k  = std::rint(x * 0x1.45f306p-1f);
+    const int   q  = (int)k;
+    float       r  = std::fma(-k, 0x1.921fb6p+0f, x);
+    r              = std::fma(-k, -0x1.777a5cp-25f, r);
+    r              = std::fma(-k, -0x1.ee59dap-50f, r);
+    const float r2 = r * r;
+    float ps       = std::fma(r2, 0x1.612462p-33f, -0x1.ae6456p-26f);
+    ps             = std::fma(r2, ps, 0x1.71de3ap-19f);
+    ps             = std::fma(r2, ps, -0x1.a01a02p-13f);
+    ps             = std::fma(r2, ps, 0x1.111112p-7f);
+    ps             = std::fma(r2, ps, -0x1.555556p-3f);
+    const float sp = std::fma(r * r2, ps, r);
+    float pc       = std::fma(r2, 0x1.1eed8ep-29f, -0x1.27e4fcp-22f);
+    pc             = std::fma(r2, pc, 0x1.a01a02p-16f);
+    pc             = std::fma(r2, pc, -0x1.6c16c2p-10f);
+    pc             = std::fma(r2, pc, 0x1.555556p-5f);
+    pc             = std::fma(r2, pc, -0.5f);
+    const float cp = std::fma(r2, pc, 1.0f);
+    switch(q & 3)
+    {
+        case 0: *s = sp; *c = cp; break;
+        case 1: *s = cp; *c = -sp; break;
+        case 2: *s = -sp; *c = -cp; break;
+        default: *s = -cp; *c = sp; break;
+    }
+}
+
+template<class T> struct Math;
+template<> struct Math<float> {
+    static void sincos(float x, float* s, float* c) { sincos_f32(x, s, c); }
+    static float tan_from(float s, float c, float) { return s / c; }
+    static float sqrt(float x) { return std::sqrt(x); }
+};
+template<> struct Math<double> {
+    static void sincos(double x, double* s, double* c)
+    {
+        *s = std::sin(x);
+        *c = std::cos(x);
+    }
+    static double tan_from(double, double, double x) { return std::tan(x); }
+    static double sqrt(double x) { return std::sqrt(x); }
+};
+
+// Float truncation toward zero clamped to [0, len-1]; NaN maps to 0.  Equal
+// to the reference's (int) cast + clamp for every finite value
+// (Graph.cu:107-109, spatialHash.cuh:44-51), defined for all inputs.
+inline int clamp_index(float f, int len)
+{
+    if(!(f >= 0.0f)) return 0;
+    if(f >= (float)len) return len - 1;
+    return (int)f;
+}
+
+// ---------------------------------------------------------------------------
+// Collision: broad phase against one obstacle (collisionCheck.cu:3-10) and the
+// motion check over all obstacles (collisionCheck.cu:49-62).  This is the
+// verdict specification: touching faces count as free.
+// ---------------------------------------------------------------------------
+template<class T>
+bool segment_free(const T bbmin[3], const T bbmax[3], const float* obs, uint32_t n)
+{
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        const float* o = obs + 6 * (size_t)i;
+        bool separated = false;
+        for(int d = 0; d < 3; ++d)
+            if(bbmax[d] <= (T)o[d] || (T)o[3 + d] <= bbmin[d])
+            {
+                separated = true;
+                break;
+            }
+        if(!separated) return false;
+    }
+    return true;
+}
+
+struct Env {
+    const kpax_config* cfg;
+    const float* obs;
+    uint32_t n_obs;
+};
+
+template<class T>
+bool in_box(T v, T lo, T hi)
+{
+    return v >= lo && v <= hi;  // false for NaN
+}
+
+template<class T>
+bool step_checks_ws_and_collision(const Env& e, const T p0[3], const T p1[3])
+{
+    const kpax_config& c = *e.cfg;
+    for(int d = 0; d < 3; ++d)
+        if(!in_box<T>(p1[d], (T)c.ws_min[d], (T)c.ws_max[d])) return false;
+    T bbmin[3], bbmax[3];
+    for(int d = 0; d < 3; ++d)
+    {
+        if(p0[d] > p1[d])
+        {
+            bbmin[d] = p1[d];
+            bbmax[d] = p0[d];
+        }
+        else
+        {
+            bbmin[d] = p0[d];
+            bbmax[d] = p1[d];
+        }
+    }
+    return segment_free<T>(bbmin, bbmax, e.obs, e.n_obs);
+}
+
+// Double integrator, statePropagatorSpatialHash.cu:72-134 (and the MATLAB
+// restatement viz/KGMT_3D_GPU.m:283-288).  Returns the 1-based failing step or 0.
+template<class T>
+int propagate_di(const Env& e, const T* x0, const T* u, int steps, T* x1)
+{
+    const kpax_config& c = *e.cfg;
+    const T dt = (T)c.dt, vmin = (T)c.vel_min, vmax = (T)c.vel_max;
+    T x = x0[0], y = x0[1], z = x0[2], vx = x0[3], vy = x0[4], vz = x0[5];
+    const T ax = u[0], ay = u[1], az = u[2];
+    int fail = 0;
+    for(int i = 0; i < steps; ++i)
+    {
+        const T p0[3] = {x, y, z};
+        if(!c.di_textbook)
+        {
+            x += (vx + (vx + T(2) * (vx + ax * dt / T(2)) + (vx + ax * dt))) * dt / T(6);
+            y += (vy + (vy + T(2) * (vy + ay * dt / T(2)) + (vy + ay * dt))) * dt / T(6);
+            z += (vz + (vz + T(2) * (vz + az * dt / T(2)) + (vz + az * dt))) * dt / T(6);
+            vx += (ax + T(2) * ax + T(2) * ax + ax) * dt / T(6);
+            vy += (ay + T(2) * ay + T(2) * ay + ay) * dt / T(6);
+            vz += (az + T(2) * az + T(2) * az + az) * dt / T(6);
+        }
+        else
+        {
+            x += vx * dt + T(0.5) * ax * dt * dt;
+            y += vy * dt + T(0.5) * ay * dt * dt;
+            z += vz * dt + T(0.5) * az * dt * dt;
+            vx += ax * dt;
+            vy += ay * dt;
+            vz += az * dt;
+        }
+        if(!in_box(vx, vmin, vmax) || !in_box(vy, vmin, vmax) || !in_box(vz, vmin, vmax))
+        {
+            fail = i + 1;
+            break;
+        }
+        const T p1[3] = {x, y, z};
+        if(!step_checks_ws_and_collision<T>(e, p0, p1))
+        {
+            fail = i + 1;
+            break;
+        }
+    }
+    x1[0] = x, x1[1] = y, x1[2] = z, x1[3] = vx, x1[4] = vy, x1[5] = vz;
+    return fail;
+}
+
+// Dubins airplane, statePropagatorSpatialHash.cu:139-229.  The controls are
+// (a, yawRate, pitchRate) in the reference's draw order (:141-143).
+template<class T>
+int propagate_dubins(const Env& e, const T* x0, const T* u, int steps, T* x1)
+{
+    const kpax_config& c = *e.cfg;
+    const T dt = (T)c.dt, h = T(0.5) * dt, d6 = dt / T(6);
+    T x = x0[0], y = x0[1], z = x0[2], yaw = x0[3], pitch = x0[4], v = x0[5];
+    const T a = u[0], yr = u[1], pr = u[2];
+    int fail = 0;
+    for(int i = 0; i < steps; ++i)
+    {
+        const T p0[3] = {x, y, z};
+        T cp, sp, cy, sy, cpm, spm, cym, sym, cpe, spe, cye, sye;
+        Math<T>::sincos(pitch, &sp, &cp);
+        Math<T>::sincos(yaw, &sy, &cy);
+        Math<T>::sincos(pitch + h * pr, &spm, &cpm);
+        Math<T>::sincos(yaw + h * yr, &sym, &cym);
+        Math<T>::sincos(pitch + dt * pr, &spe, &cpe);
+        Math<T>::sincos(yaw + dt * yr, &sye, &cye);
+        const T A  = v + h * a;
+        const T B  = v + dt * a;
+        const T Tx = A * cpm * cym;
+        const T Ty = A * cpm * sym;
+        const T Tz = A * spm;
+        x += d6 * (v * cp * cy + T(2) * (Tx + Tx) + B * cpe * cye);
+        y += d6 * (v * cp * sy + T(2) * (Ty + Ty) + B * cpe * sye);
+        z += d6 * (v * sp + T(2) * (Tz + Tz) + B * spe);
+        yaw += dt * yr;
+        pitch += dt * pr;
+        v += d6 * (a + T(2) * (a + a) + a);
+        if(!in_box(v, (T)c.vel_min, (T)c.vel_max) || !in_box(pitch, (T)c.pitch_min, (T)c.pitch_max))
+        {
+            fail = i + 1;
+            break;
+        }
+        const T p1[3] = {x, y, z};
+        if(!step_checks_ws_and_collision<T>(e, p0, p1))
+        {
+            fail = i + 1;
+            break;
+        }
+    }
+    x1[0] = x, x1[1] = y, x1[2] = z, x1[3] = yaw, x1[4] = pitch, x1[5] = v;
+    return fail;
+}
+
+template<class T> struct QuadK {
+    T negnu, negmu, g, mi, k1x, k1y, k1z, ixi, iyi, izi;
+};
+
+// Coefficients exactly as the reference's macros fold them in float
+// (config.h:300-308: (IY - IZ) / IX, 1 / IX, 1.0f / MASS, -NU, -MU).
+template<class T>
+QuadK<T> quad_constants(const kpax_config& c)
+{
+    const float ix = c.inertia[0], iy = c.inertia[1], iz = c.inertia[2];
+    QuadK<T> k;
+    k.negnu = (T)(-c.nu);
+    k.negmu = (T)(-c.mu);
+    k.g     = (T)c.gravity;
+    k.mi    = (T)(1.0f / c.mass);
+    k.k1x   = (T)((iy - iz) / ix);
+    k.k1y   = (T)((iz - ix) / iy);
+    k.k1z   = (T)((ix - iy) / iz);
+    k.ixi   = (T)(1.0f / ix);
+    k.iyi   = (T)(1.0f / iy);
+    k.izi   = (T)(1.0f / iz);
+    return k;
+}
+
+// ode(), src/statePropagator/statePropagator.cu:325-370.
+template<class T>
+void quad_ode(T* out, const T* x, const T* hh, T delta, const T* u, const QuadK<T>& k)
+{
+    const T phi = x[3] + delta * hh[3], theta = x[4] + delta * hh[4], psi = x[5] + delta * hh[5];
+    const T uu = x[6] + delta * hh[6], v = x[7] + delta * hh[7], w = x[8] + delta * hh[8];
+    const T p = x[9] + delta * hh[9], q = x[10] + delta * hh[10], r = x[11] + delta * hh[11];
+    T sf, cf, st, ct, ss, cs;
+    Math<T>::sincos(phi, &sf, &cf);
+    Math<T>::sincos(theta, &st, &ct);
+    Math<T>::sincos(psi, &ss, &cs);
+    const T tt = Math<T>::tan_from(st, ct, theta);
+    out[0] = ct * cs * uu + (sf * st * cs - cf * ss) * v + (cf * st * cs + sf * ss) * w;
+    out[1] = ct * ss * uu + (sf * st * ss + cf * cs) * v + (cf * st * ss - sf * cs) * w;
+    out[2] = -st * uu + sf * ct * v + cf * ct * w;
+    out[3] = p + (q * sf + r * cf) * tt;
+    out[4] = q * cf - r * sf;
+    out[5] = (q * sf + r * cf) / ct;
+    const T XYZ = k.negnu * Math<T>::sqrt(uu * uu + v * v + w * w);
+    out[6]      = (r * v - q * w) - k.g * st + k.mi * XYZ * uu;
+    out[7]      = (p * w - r * uu) + k.g * ct * sf + k.mi * XYZ * v;
+    out[8]      = (q * uu - p * v) + k.g * ct * cf + k.mi * XYZ * w + k.mi * u[0];
+    const T LMN = k.negmu * Math<T>::sqrt(p * p + q * q + r * r);
+    out[9]      = k.k1x * q * r + k.ixi * LMN * p + k.ixi * u[1];
+    out[10]     = k.k1y * p * r + k.iyi * LMN * q + k.iyi * u[2];
+    out[11]     = k.k1z * p * q + k.izi * LMN * r + k.izi * u[3];
+}
+
+// Quadcopter RK4, statePropagatorSpatialHash.cu:234-308.
+template<class T>
+int propagate_quad(const Env& e, const T* x0, const T* u, int steps, T* x1)
+{
+    const kpax_config& c = *e.cfg;
+    const QuadK<T> k = quad_constants<T>(c);
+    const T dt = (T)c.dt, half = T(0.5) * dt, d6 = dt / T(6);
+    const T vmin = (T)c.vel_min, vmax = (T)c.vel_max;
+    T h0[12], k1[12], k2[12], k3[12], k4[12];
+    const T zero[12] = {};
+    for(int j = 0; j < 12; ++j) h0[j] = x0[j];
+    int fail = 0;
+    for(int i = 0; i < steps; ++i)
+    {
+        const T p0[3] = {h0[0], h0[1], h0[2]};
+        quad_ode<T>(k1, h0, zero, T(1), u, k);
+        quad_ode<T>(k2, h0, k1, half, u, k);
+        quad_ode<T>(k3, h0, k2, half, u, k);
+        quad_ode<T>(k4, h0, k3, dt, u, k);
+        for(int j = 0; j < 12; ++j) h0[j] += d6 * (k1[j] + T(2) * k2[j] + T(2) * k3[j] + k4[j]);
+        if(!in_box(h0[6], vmin, vmax) || !in_box(h0[7], vmin, vmax) || !in_box(h0[8], vmin, vmax))
+        {
+            fail = i + 1;
+            break;
+        }
+        const T p1[3] = {h0[0], h0[1], h0[2]};
+        if(!step_checks_ws_and_collision<T>(e, p0, p1))
+        {
+            fail = i + 1;
+            break;
+        }
+    }
+    for(int j = 0; j < 12; ++j) x1[j] = h0[j];
+    return fail;
+}
+
+// In float the per-step constants fold as the reference's macros do
+// (0.5f * STEP_SIZE, STEP_SIZE / 6.0f); in double they are exact in the
+// float step dt.
+template<class T>
+int propagate(const Env& e, const T* x0, const T* u, int steps, T* x1)
+{
+    switch(e.cfg->model)
+    {
+        case KPAX_MODEL_DOUBLE_INTEGRATOR: return propagate_di<T>(e, x0, u, steps, x1);
+        case KPAX_MODEL_DUBINS_AIRPLANE: return propagate_dubins<T>(e, x0, u, steps, x1);
+        case KPAX_MODEL_QUADCOPTER: return propagate_quad<T>(e, x0, u, steps, x1);
+        default: return -1;
+    }
+}
+
+int state_dim(int model) { return model == KPAX_MODEL_QUADCOPTER ? 12 : 6; }
+int control_dim(int model) { return model == KPAX_MODEL_QUADCOPTER ? 4 : 3; }
+
+// ---------------------------------------------------------------------------
+// Regions: getRegion / getSubRegion (Graph.cu:99-205) with the R1 min corner
+// of initializeRegions_kernel (Graph.cu:58-70), per workspace axis.
+// ---------------------------------------------------------------------------
+struct Regions {
+    int L[3], L2[3];
+    float mn[3], ext[3], size[3];
+    int r1_count, r2_per;
+    explicit Regions(const kpax_config& c)
+    {
+        for(int d = 0; d < 3; ++d)
+        {
+            L[d]    = c.r1_len[d];
+            L2[d]   = c.r2_len[d];
+            mn[d]   = c.ws_min[d];
+            ext[d]  = c.ws_max[d] - c.ws_min[d];
+            size[d] = ext[d] / (float)L[d];
+        }
+        r1_count = L[0] * L[1] * L[2];
+        r2_per   = L2[0] * L2[1] * L2[2];
+    }
+    void locate(const float* p, int* r1, int* r2) const
+    {
+        int idx[3], jdx[3];
+        for(int d = 0; d < 3; ++d) idx[d] = clamp_index((float)L[d] * (p[d] - mn[d]) / ext[d], L[d]);
+        for(int d = 0; d < 3; ++d)
+        {
+            const float lo = mn[d] + (float)idx[d] * size[d];
+            jdx[d]         = clamp_index((float)L2[d] * (p[d] - lo) / size[d], L2[d]);
+        }
+        *r1 = (idx[0] * L[1] + idx[1]) * L[2] + idx[2];
+        *r2 = *r1 * r2_per + (jdx[0] * L2[1] + jdx[1]) * L2[2] + jdx[2];
+    }
+};
+
+// Syclop-style region score (Graph.cu:243-255) and its normalisation
+// (Graph.cu:297-304).  The total is summed in a fixed order: 1024 strided
+// partial sums, then a pairwise tree halving 512, 256, ..., 1.
+void scores(const kpax_config& c, const Regions& R, const int32_t* valid, const int32_t* total,
+            const int32_t* covered, float* out, float* total_out)
+{
+    const float vol = R.size[0] * R.size[1] * R.size[2];
+    std::vector<float> raw(R.r1_count, 0.0f);
+    for(int r = 0; r < R.r1_count; ++r)
+    {
+        if(valid[r] > 0)
+        {
+            const float cov = (float)covered[r] / (float)R.r2_per;
+            const float ev  = c.eps + (float)valid[r];
+            const float fv  = ev / (ev + (float)(total[r] - valid[r])) * vol;
+            const float n   = (float)total[r];
+            raw[r]          = (fv * fv) * (fv * fv) / ((1.0f + cov) * (1.0f + n * n));
+        }
+    }
+    float part[1024];
+    for(int t = 0; t < 1024; ++t)
+    {
+        float s = 0.0f;
+        for(int r = t; r < R.r1_count; r += 1024) s += raw[r];
+        part[t] = s;
+    }
+    for(int off = 512; off >= 1; off >>= 1)
+        for(int t = 0; t < off; ++t) part[t] = part[t] + part[t + off];
+    const float tot = part[0];
+    for(int r = 0; r < R.r1_count; ++r) out[r] = valid[r] == 0 ? 1.0f : c.eps + raw[r] / tot;
+    if(total_out) *total_out = tot;
+}
+
+// fAccept of updateFrontier (KPAX.cu:527-528): the host computes
+// (h_itr_ * EPSILON) in float and pow(treeAddSize, 5) in double.
+float f_accept(const kpax_config& c, uint32_t iter, uint32_t tree_plus_new)
+{
+    const float  tas = 1.0f - ((float)tree_plus_new / (float)c.max_tree);
+    const double t   = (double)tas;
+    const double p5  = t * t * t * t * t;
+    return (float)((double)((float)iter * c.eps) * p5);
+}
+
+inline float goal_distance(const float* p, const float* g)
+{
+    const float dx = p[0] - g[0], dy = p[1] - g[1], dz = p[2] - g[2];
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+struct PlanOut {
+    int32_t status = KPAX_NOT_FOUND;
+    uint32_t iterations = 0, tree_size = 0;
+    int32_t goal = -1;
+    uint64_t expansions = 0;
+};
+
+struct Planner {
+    const kpax_config& c;
+    Env env;
+    Regions R;
+    int S, C;
+    uint32_t N;
+    // tree, AoS per node
+    std::vector<float> st, ct;
+    std::vector<int32_t> steps, parent, reg;
+    std::vector<int32_t> valid, total, covered;
+    std::vector<int32_t> r2_first;
+    std::vector<float> score;
+    float score_total = 0.0f;
+
+    Planner(const kpax_config& cfg, const float* obs, uint32_t n_obs)
+        : c(cfg), env{&cfg, obs, n_obs}, R(cfg), S(state_dim(cfg.model)), C(control_dim(cfg.model)),
+          N(cfg.max_tree)
+    {
+        st.assign((size_t)N * S, 0.0f);
+        ct.assign((size_t)N * C, 0.0f);
+        steps.assign(N, 0);
+        parent.assign(N, -1);
+        reg.assign(N, 0);
+        valid.assign(R.r1_count, 0);
+        total.assign(R.r1_count, 0);
+        covered.assign(R.r1_count, 0);
+        r2_first.assign((size_t)R.r1_count * R.r2_per, std::numeric_limits<int32_t>::max());
+        score.assign(R.r1_count, 0.0f);
+    }
+
+    PlanOut plan(const float* start, const float* goal, uint64_t seed, uint32_t query)
+    {
+        PlanOut out;
+        for(int d = 0; d < S; ++d) st[d] = start[d];
+        parent[0] = -1;
+        int r1, r2;
+        R.locate(start, &r1, &r2);
+        reg[0]      = r1;
+        uint32_t ts = 1;
+        // frontier repeat list: the root, root_repeat times (KPAX.cu:75-77)
+        std::vector<uint32_t> rep(c.root_repeat, 0u);
+        uint64_t W = c.root_repeat;
+        std::vector<float> cs((size_t)N * S), cc((size_t)N * C);
+        std::vector<int32_t> csteps(N), cpar(N), creg(N), cr2(N);
+        std::vector<uint8_t> adm(N), cvalid(N), csadm(N);
+        for(uint32_t t = 1;; ++t)
+        {
+            if(t > c.max_iter)
+            {
+                out.status = KPAX_NOT_FOUND;
+                break;
+            }
+            // fan-out (KPAX.cu:323-350)
+            const uint32_t remaining = N - ts;
+            if(remaining == 0)
+            {
+                out.status = KPAX_BUDGET_EXHAUSTED;
+                break;
+            }
+            out.iterations = t;
+            uint32_t fan = c.fanout;
+            uint64_t M   = W * fan;
+            if(M > remaining)
+            {
+                fan = (uint32_t)(int)((float)remaining / (float)W);
+                if(fan > c.fanout) fan = c.fanout;
+                if(fan == 0) fan = 1;
+                M = std::min<uint64_t>(W * fan, remaining);
+            }
+            out.expansions += M;
+            // expansion (propagateFrontier_kernel1/2, KPAX.cu:358-439)
+#pragma omp parallel for schedule(dynamic, 256)
+            for(int64_t ci = 0; ci < (int64_t)M; ++ci)
+            {
+                const uint32_t cix = (uint32_t)ci;
+                const uint32_t p   = rep[cix / fan];
+                const U4 dc        = draw(seed, cix, t, query, PURPOSE_CONTROL);
+                const U4 ds        = draw(seed, cix, t, query, PURPOSE_STEPS_ADMIT);
+                float u[4];
+                for(int i = 0; i < C; ++i) u[i] = c.ctrl_min[i] + u01(dc.v[i]) * (c.ctrl_max[i] - c.ctrl_min[i]);
+                const int span = c.steps_max - c.steps_min + 1;
+                int k          = (int)(u01(ds.v[0]) * (float)span);
+                if(k > span - 1) k = span - 1;
+                const int nsteps = c.steps_min + k;
+                float* x1        = &cs[(size_t)cix * S];
+                const int fail   = propagate<float>(env, &st[(size_t)p * S], u, nsteps, x1);
+                for(int i = 0; i < C; ++i) cc[(size_t)cix * C + i] = u[i];
+                csteps[cix] = nsteps;
+                cpar[cix]   = (int32_t)p;
+                int a1, a2;
+                R.locate(x1, &a1, &a2);
+                creg[cix] = a1;
+                cr2[cix]  = a2;
+                __atomic_fetch_add(&total[a1], 1, __ATOMIC_RELAXED);
+                if(fail == 0) __atomic_fetch_add(&valid[a1], 1, __ATOMIC_RELAXED);
+                cvalid[cix] = fail == 0 ? 1 : 0;
+                csadm[cix]  = fail == 0 && u01(ds.v[1]) < score[a1] ? 1 : 0;
+            }
+            // Sub-region novelty as the reference's kernel decides it when its
+            // threads run one at a time in slot order (KPAX.cu:394-395): the
+            // first valid child to reach an uncovered R2 cell sees it empty.
+            for(uint32_t ci = 0; ci < M; ++ci)
+            {
+                bool fresh = false;
+                if(cvalid[ci] && r2_first[cr2[ci]] == std::numeric_limits<int32_t>::max())
+                {
+                    r2_first[cr2[ci]] = (int32_t)t;
+                    covered[creg[ci]]++;
+                    fresh = true;
+                }
+                adm[ci] = (csadm[ci] || fresh) ? 1 : 0;
+            }
+            // scores (Graph::updateVertices, Graph.cu:207-227)
+            scores(c, R, valid.data(), total.data(), covered.data(), score.data(), &score_total);
+            // append admitted children in slot order (updateFrontier_kernel, KPAX.cu:458-502)
+            const uint32_t ts_old = ts;
+            int32_t goal_idx      = -1;
+            for(uint32_t ci = 0; ci < M; ++ci)
+            {
+                if(!adm[ci]) continue;
+                const uint32_t idx = ts++;
+                for(int d = 0; d < S; ++d) st[(size_t)idx * S + d] = cs[(size_t)ci * S + d];
+                for(int d = 0; d < C; ++d) ct[(size_t)idx * C + d] = cc[(size_t)ci * C + d];
+                steps[idx]  = csteps[ci];
+                parent[idx] = cpar[ci];
+                reg[idx]    = creg[ci];
+                if(goal_idx < 0 && goal_distance(&cs[(size_t)ci * S], goal) < c.goal_radius) goal_idx = (int32_t)idx;
+            }
+            const float fa = f_accept(c, t, ts);
+            if(std::getenv("KPAX_ORACLE_TRACE"))
+            {
+                size_t nv = 0;
+                for(uint32_t ci = 0; ci < M; ++ci) nv += adm[ci];
+                std::fprintf(stderr, "t=%u W=%llu fan=%u M=%llu admitted=%zu ts=%u fa=%g\n", t, (unsigned long long)W,
+                             fan, (unsigned long long)M, nv, ts, fa);
+            }
+            if(goal_idx >= 0)
+            {
+                out.status = KPAX_OK;
+                out.goal   = goal_idx;
+                break;
+            }
+            // next frontier: re-admitted old nodes and every new node (KPAX.cu:471-517)
+            // The list holds at most N entries (no launch reads past M <= N);
+            // W counts every entry.
+            rep.clear();
+            W = 0;
+            for(uint32_t j = 0; j < ts; ++j)
+            {
+                uint32_t w;
+                if(j < ts_old)
+                {
+                    const U4 dr = draw(seed, j, t, query, PURPOSE_READMIT);
+                    w           = u01(dr.v[0]) <= score[reg[j]] + fa ? 1u : 0u;
+                }
+                else
+                    w = valid[reg[j]] < c.sparse_valid_below ? c.sparse_repeat : c.dense_repeat;
+                for(uint32_t k = 0; k < w && rep.size() < N; ++k) rep.push_back(j);
+                W += w;
+            }
+        }
+        out.tree_size = ts;
+        return out;
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C interface for the Python test harness (ctypes).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out)
+{
+    U4 c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
+    U4 r = philox4x32_10(c, key[0], key[1]);
+    for(int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+
+void oracle_sincos_f32(const float* x, uint32_t n, float* s, float* c)
+{
+    for(uint32_t i = 0; i < n; ++i) sincos_f32(x[i], &s[i], &c[i]);
+}
+
+// Propagate n edges with injected controls/steps (SoA in and out).
+// use_double = 0: float path (parity with the GPU); 1: double path (truth).
+// out_x1 is float* or double* accordingly.
+int oracle_propagate(const kpax_config* cfg, const float* obs, uint32_t n_obs, const float* x0_soa,
+                     const float* u_soa, const int32_t* steps, uint32_t n, int32_t use_double, void* out_x1,
+                     uint8_t* valid, int32_t* fail_step)
+{
+    const Env e{cfg, obs, n_obs};
+    const int S = state_dim(cfg->model), C = control_dim(cfg->model);
+#pragma omp parallel for schedule(dynamic, 64)
+    for(int64_t ii = 0; ii < (int64_t)n; ++ii)
+    {
+        const uint32_t i = (uint32_t)ii;
+        int fail;
+        if(use_double)
+        {
+            double x0[12], u[4], x1[12];
+            for(int d = 0; d < S; ++d) x0[d] = x0_soa[(size_t)d * n + i];
+            for(int d = 0; d < C; ++d) u[d] = u_soa[(size_t)d * n + i];
+            fail = propagate<double>(e, x0, u, steps[i], x1);
+            for(int d = 0; d < S; ++d) ((double*)out_x1)[(size_t)d * n + i] = x1[d];
+        }
+        else
+        {
+            float x0[12], u[4], x1[12];
+            for(int d = 0; d < S; ++d) x0[d] = x0_soa[(size_t)d * n + i];
+            for(int d = 0; d < C; ++d) u[d] = u_soa[(size_t)d * n + i];
+            fail = propagate<float>(e, x0, u, steps[i], x1);
+            for(int d = 0; d < S; ++d) ((float*)out_x1)[(size_t)d * n + i] = x1[d];
+        }
+        valid[i]     = fail == 0 ? 1 : 0;
+        fail_step[i] = fail;
+    }
+    return 0;
+}
+
+// Brute-force verdicts for n segment boxes (SoA 3 x n).
+void oracle_collide(const float* obs, uint32_t n_obs, const float* bbmin_soa, const float* bbmax_soa, uint32_t n,
+                    uint8_t* free_out)
+{
+#pragma omp parallel for schedule(static)
+    for(int64_t ii = 0; ii < (int64_t)n; ++ii)
+    {
+        const uint32_t i = (uint32_t)ii;
+        float lo[3], hi[3];
+        for(int d = 0; d < 3; ++d)
+        {
+            lo[d] = bbmin_soa[(size_t)d * n + i];
+            hi[d] = bbmax_soa[(size_t)d * n + i];
+        }
+        free_out[i] = segment_free<float>(lo, hi, obs, n_obs) ? 1 : 0;
+    }
+}
+
+// Uniform grid restated from spatialHash.cu:9-54 (count per cell; the CSR
+// start offsets are the exclusive scan of the counts).  Cells are indexed
+// x fastest (gridToIndex, spatialHash.cuh:57-60).
+void oracle_grid_counts(const kpax_config* cfg, const float* obs, uint32_t n_obs, int32_t* counts)
+{
+    const int* D = cfg->grid_dims;
+    float cell[3];
+    for(int d = 0; d < 3; ++d) cell[d] = (cfg->ws_max[d] - cfg->ws_min[d]) / (float)D[d];
+    std::fill(counts, counts + (size_t)D[0] * D[1] * D[2], 0);
+    for(uint32_t i = 0; i < n_obs; ++i)
+    {
+        int lo[3], hi[3];
+        for(int d = 0; d < 3; ++d)
+        {
+            lo[d] = clamp_index((obs[6 * i + d] - cfg->ws_min[d]) / cell[d], D[d]);
+            hi[d] = clamp_index((obs[6 * i + 3 + d] - cfg->ws_min[d]) / cell[d], D[d]);
+        }
+        for(int z = lo[2]; z <= hi[2]; ++z)
+            for(int y = lo[1]; y <= hi[1]; ++y)
+                for(int x = lo[0]; x <= hi[0]; ++x) counts[x + D[0] * (y + D[1] * z)]++;
+    }
+}
+
+void oracle_regions(const kpax_config* cfg, const float* states_soa, uint32_t n, int32_t* r1, int32_t* r2)
+{
+    const Regions R(*cfg);
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        const float p[3] = {states_soa[i], states_soa[n + i], states_soa[2 * (size_t)n + i]};
+        R.locate(p, &r1[i], &r2[i]);
+    }
+}
+
+// Counters of n children in one iteration (the atomics of KPAX.cu:386-396):
+// accumulates into total/valid_count/covered; r2_first (R1*R2per entries,
+// INT32_MAX = never reached) carries coverage between calls.  is_new[i] = 1
+// for the valid child that first reaches an uncovered R2 cell in slot order.
+void oracle_histogram(const kpax_config* cfg, const float* states_soa, const uint8_t* valid_in, uint32_t n,
+                      int32_t iter, int32_t* total, int32_t* valid_count, int32_t* covered, int32_t* r2_first,
+                      uint8_t* is_new)
+{
+    const Regions R(*cfg);
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        const float p[3] = {states_soa[i], states_soa[n + i], states_soa[2 * (size_t)n + i]};
+        int a1, a2;
+        R.locate(p, &a1, &a2);
+        total[a1]++;
+        uint8_t nw = 0;
+        if(valid_in[i])
+        {
+            valid_count[a1]++;
+            if(r2_first[a2] == std::numeric_limits<int32_t>::max())
+            {
+                r2_first[a2] = iter;
+                covered[a1]++;
+                nw = 1;
+            }
+        }
+        if(is_new) is_new[i] = nw;
+    }
+}
+
+void oracle_scores(const kpax_config* cfg, const int32_t* valid, const int32_t* total, const int32_t* covered,
+                   float* out, float* total_out)
+{
+    const Regions R(*cfg);
+    scores(*cfg, R, valid, total, covered, out, total_out);
+}
+
+float oracle_f_accept(const kpax_config* cfg, uint32_t iter, uint32_t tree_plus_new)
+{
+    return f_accept(*cfg, iter, tree_plus_new);
+}
+
+// findInd + repeatInd (Planner.cu:52-80): index i repeated weights[i] times.
+uint64_t oracle_repeat_expand(const uint32_t* weights, uint32_t n, uint32_t* out, uint64_t cap)
+{
+    uint64_t k = 0;
+    for(uint32_t i = 0; i < n; ++i)
+        for(uint32_t j = 0; j < weights[i]; ++j, ++k)
+            if(k < cap) out[k] = i;
+    return k;
+}
+
+// Whole planner.  Outputs are optional (NULL to skip); SoA outputs use
+// max_tree as column stride.  Returns the status; *info receives
+// {iterations, tree_size, goal_index, expansions_lo, expansions_hi}.
+int oracle_plan(const kpax_config* cfg, const float* obs, uint32_t n_obs, const float* start, const float* goal,
+                uint64_t seed, int32_t threads, uint32_t* info, float* states_soa, float* controls_soa,
+                int32_t* steps_out, int32_t* parents_out, int32_t* regions_out, int32_t* valid_out,
+                int32_t* total_out, int32_t* covered_out, float* scores_out)
+{
+#ifdef _OPENMP
+    if(threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    Planner P(*cfg, obs, n_obs);
+    const PlanOut o = P.plan(start, goal, seed, 0);
+    const uint32_t N = cfg->max_tree;
+    if(info)
+    {
+        info[0] = o.iterations;
+        info[1] = o.tree_size;
+        info[2] = (uint32_t)o.goal;
+        info[3] = (uint32_t)o.expansions;
+        info[4] = (uint32_t)(o.expansions >> 32);
+    }
+    for(uint32_t i = 0; i < o.tree_size; ++i)
+    {
+        if(states_soa)
+            for(int d = 0; d < P.S; ++d) states_soa[(size_t)d * N + i] = P.st[(size_t)i * P.S + d];
+        if(controls_soa)
+            for(int d = 0; d < P.C; ++d) controls_soa[(size_t)d * N + i] = P.ct[(size_t)i * P.C + d];
+        if(steps_out) steps_out[i] = P.steps[i];
+        if(parents_out) parents_out[i] = P.parent[i];
+        if(regions_out) regions_out[i] = P.reg[i];
+    }
+    const int r1 = P.R.r1_count;
+    if(valid_out) std::memcpy(valid_out, P.valid.data(), sizeof(int32_t) * r1);
+    if(total_out) std::memcpy(total_out, P.total.data(), sizeof(int32_t) * r1);
+    if(covered_out) std::memcpy(covered_out, P.covered.data(), sizeof(int32_t) * r1);
+    if(scores_out) std::memcpy(scores_out, P.score.data(), sizeof(float) * r1);
+    return o.status;
+}
+
+}  // extern "C"

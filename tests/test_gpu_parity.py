@@ -1,0 +1,201 @@
+"""GPU parity: every hot-path operator of libkpax against the CPU oracle.
+
+Integer, index and verdict outputs must be bit-exact.  Propagated FP32 states
+must equal the oracle's float path bit for bit (same operation order, no
+contraction, the same sin/cos routine) and lie within 1e-5 relative of the
+oracle's FP64 path (|x32 - x64| <= 1e-5 * max(1, |x64|) per component).
+"""
+import numpy as np
+import pytest
+
+from workloads import random_states, workload
+
+pytestmark = pytest.mark.gpu
+
+MODELS = [1, 2, 3]  # workload ids: double integrator, quadcopter, Dubins
+
+
+def _planner(kpax, k, **kw):
+    cfg, obs, start, goal = workload(k, **kw)
+    p = kpax.KPAX(cfg)
+    p.set_obstacles(obs)
+    return p, cfg, obs, start, goal
+
+
+@pytest.mark.parametrize("k", MODELS)
+def test_propagate_bit_exact_vs_oracle_float(kpax, oracle, rng, k):
+    p, cfg, obs, _, _ = _planner(kpax, k, max_tree=4096)
+    print(f"propagate parity workload {k}", flush=True)
+    x0, u, steps = random_states(cfg, 20000, rng)
+    x1, valid, fail = p.propagate_batch(x0, u, steps)
+    o1, ovalid, ofail = oracle.propagate(cfg, obs, x0, u, steps)
+    assert np.array_equal(valid, ovalid)
+    assert np.array_equal(fail, ofail)
+    assert np.array_equal(x1.view(np.uint32), o1.view(np.uint32)), \
+        f"max |diff| {np.max(np.abs(x1 - o1))}"
+    assert valid.any() and (~valid).any()
+
+
+@pytest.mark.parametrize("k", MODELS)
+def test_propagate_within_1e5_of_fp64(kpax, oracle, rng, k):
+    p, cfg, _, _, _ = _planner(kpax, k, max_tree=4096, n_obs=0)
+    x0, u, steps = random_states(cfg, 4000, rng)
+    x1, valid, fail = p.propagate_batch(x0, u, steps)
+    d1, dvalid, dfail = oracle.propagate(cfg, None, x0, u, steps, double=True)
+    both = valid & dvalid & (fail == dfail)
+    assert both.mean() > 0.5
+    err = np.abs(x1[:, both].astype(np.float64) - d1[:, both]) / np.maximum(1.0, np.abs(d1[:, both]))
+    assert err.max() <= 1e-5, err.max()
+
+
+def test_collide_matches_brute_force(kpax, oracle, rng):
+    cfg = kpax.config_workload(2)
+    # 2,000 small boxes: many cells hold more than 16 obstacles
+    c = rng.uniform(0, 1, (2000, 3)) * np.array([10, 10, 5])
+    e = rng.uniform(0.05, 0.6, (2000, 3))
+    obs = np.concatenate([c - e / 2, c + e / 2], 1).astype(np.float32)
+    obs[:, :3] = np.maximum(obs[:, :3], 0)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        start, items = p.grid()
+        assert np.diff(start).max() > 16
+        n = 100000
+        a = (rng.uniform(-0.5, 1.5, (3, n)) * np.array([[10], [10], [5]])).astype(np.float32)
+        b = a + rng.uniform(-0.3, 0.3, (3, n)).astype(np.float32)
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        # exact face contact and zero-length segments
+        lo[:, :500] = obs[:500, 3:].T
+        hi[:, :500] = obs[:500, 3:].T + 0.01
+        hi[:, 500:1000] = lo[:, 500:1000]
+        got = p.collide_batch(lo, hi)
+        want = oracle.collide(obs, lo, hi)
+        assert np.array_equal(got, want)
+        assert got[:500].all()  # touching counts as free, as in collisionCheck.cu:7
+        assert (~got).sum() > 1000
+
+
+@pytest.mark.parametrize("k", MODELS)
+def test_grid_csr_matches_oracle_counts(kpax, oracle, k):
+    p, cfg, obs, _, _ = _planner(kpax, k, max_tree=4096)
+    start, items = p.grid()
+    counts = oracle.grid_counts(cfg, obs)
+    assert np.array_equal(np.diff(start), counts)
+    assert start[0] == 0 and start[-1] == items.size
+    for cidx in np.flatnonzero(counts)[:2000]:
+        seg = items[start[cidx]:start[cidx + 1]]
+        assert np.all(np.diff(seg) > 0)
+
+
+@pytest.mark.parametrize("k", MODELS)
+def test_regions_match_oracle(kpax, oracle, rng, k):
+    p, cfg, _, _, _ = _planner(kpax, k, max_tree=4096)
+    lo = np.array(cfg.ws_min, np.float32)[:, None]
+    ext = (np.array(cfg.ws_max, np.float32) - np.array(cfg.ws_min, np.float32))[:, None]
+    n = 50000
+    st = (lo + ext * rng.uniform(-0.1, 1.1, (3, n))).astype(np.float32)
+    # exact region and sub-region boundaries
+    L = np.array(cfg.r1_len)[:, None]
+    st[:, :1000] = (lo + ext * rng.integers(0, 4 * L.max() + 1, (3, 1000)) / (4 * L)).astype(np.float32)
+    st[0, 1000] = np.nan
+    r1, r2 = p.region_batch(st)
+    o1, o2 = oracle.regions(cfg, st)
+    assert np.array_equal(r1, o1) and np.array_equal(r2, o2)
+
+
+@pytest.mark.parametrize("k", MODELS)
+def test_histogram_matches_oracle(kpax, oracle, rng, k):
+    p, cfg, _, _, _ = _planner(kpax, k, max_tree=4096)
+    lo = np.array(cfg.ws_min, np.float32)[:, None]
+    ext = (np.array(cfg.ws_max, np.float32) - np.array(cfg.ws_min, np.float32))[:, None]
+    n = 60000
+    # clustered states: heavy atomic contention on a few regions
+    centre = lo + ext * rng.uniform(0.2, 0.8, (3, 1))
+    st = (centre + 0.05 * ext * rng.standard_normal((3, n))).astype(np.float32)
+    valid = rng.uniform(0, 1, n) < 0.7
+    t0 = rng.integers(0, 5, cfg.r1_count).astype(np.int32)
+    got = p.region_histogram(st, valid, total=t0 + 3, valid_count=t0, covered=np.zeros_like(t0))
+    want = oracle.histogram(cfg, st, valid, 1, total=t0 + 3, valid_count=t0, covered=np.zeros_like(t0))
+    for g, w in zip(got, want[:3]):
+        assert np.array_equal(g, w)
+
+
+@pytest.mark.parametrize("k", MODELS)
+def test_scores_bit_exact(kpax, oracle, rng, k):
+    p, cfg, _, _, _ = _planner(kpax, k, max_tree=4096)
+    R1 = cfg.r1_count
+    valid = rng.integers(0, 50, R1).astype(np.int32)
+    valid[rng.uniform(0, 1, R1) < 0.5] = 0
+    total = valid + rng.integers(0, 30, R1).astype(np.int32)
+    covered = np.minimum(valid, rng.integers(0, cfg.r2_per + 1, R1)).astype(np.int32)
+    got = p.scores(valid, total, covered)
+    want, _ = oracle.scores(cfg, valid, total, covered)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 1023, 1024, 1025, 300000, 2_100_000])
+def test_repeat_expand(kpax, rng, n):
+    with kpax.KPAX(kpax.config_workload(1)) as p:
+        w = rng.integers(0, 4, n).astype(np.uint32)
+        w[rng.uniform(0, 1, n) < 0.3] = 15
+        out, total = p.repeat_expand(w)
+        assert total == int(w.sum())
+        assert np.array_equal(out, np.repeat(np.arange(n, dtype=np.uint32), w))
+
+
+def test_repeat_expand_zeros_and_cap(kpax):
+    with kpax.KPAX(kpax.config_workload(1)) as p:
+        out, total = p.repeat_expand(np.zeros(5000, np.uint32))
+        assert total == 0 and out.size == 0
+        w = np.full(5000, 3, np.uint32)
+        out, total = p.repeat_expand(w, cap=100)
+        assert total == 15000 and np.array_equal(out, np.repeat(np.arange(34, dtype=np.uint32), 3)[:100])
+
+
+def _compare_plans(kpax, oracle, k, seed, **kw):
+    cfg, obs, start, goal = workload(k, seed=seed, **kw)
+    with kpax.KPAX(cfg) as p:
+        r = p.plan(start, goal, obs, seed=seed + 77)
+        tree = p.tree()
+        reg = p.regions()
+    o = oracle.plan(cfg, obs, start, goal, seed=seed + 77, threads=8)
+    print(f"plan workload {k} seed {seed}: gpu status {r.status} it {r.iterations} ts {r.tree_size}; "
+          f"oracle status {o['status']} it {o['iterations']} ts {o['tree_size']}", flush=True)
+    assert r.status == o["status"]
+    assert r.iterations == o["iterations"]
+    assert r.tree_size == o["tree_size"]
+    assert r.expansions == o["expansions"]
+    assert np.array_equal(tree["parents"], o["parents"])
+    assert np.array_equal(tree["steps"], o["steps"])
+    assert np.array_equal(tree["regions"], o["regions"])
+    assert np.array_equal(tree["states"].view(np.uint32), o["states"].view(np.uint32))
+    assert np.array_equal(tree["controls"].view(np.uint32), o["controls"].view(np.uint32))
+    for key in ("valid", "total", "covered"):
+        assert np.array_equal(reg[key], o[key]), key
+    assert np.array_equal(reg["scores"].view(np.uint32), o["scores"].view(np.uint32))
+    if r.status == kpax.OK:
+        assert r.goal_index == o["goal_index"]
+        assert r.verified == 1
+    return r
+
+
+@pytest.mark.parametrize("k,seed,kw", [
+    (1, 0, dict(max_tree=20000, max_iter=12)),
+    (1, 1, dict(max_tree=65536, max_iter=8)),
+    (2, 0, dict(max_tree=1 << 17)),
+    (2, 3, dict(max_tree=1 << 16, max_iter=10)),
+    (3, 0, dict(max_tree=1 << 17, max_iter=15)),
+])
+def test_plan_tree_bit_exact_vs_oracle(kpax, oracle, k, seed, kw):
+    _compare_plans(kpax, oracle, k, seed, **kw)
+
+
+def test_plan_quadcopter_full_size_finds_verified_path(kpax):
+    cfg, obs, start, goal = workload(2, seed=0)
+    with kpax.KPAX(cfg) as p:
+        r = p.plan(start, goal, obs, seed=1)
+        assert r.status == kpax.OK, r
+        assert r.verified == 1
+        assert np.linalg.norm(r.rows[-1, :3] - goal) < cfg.goal_radius
+        assert np.array_equal(r.rows[0, :12], start)
+        ok, dev = kpax.verify_path(cfg, obs, r.rows, r.steps, list(goal))
+        assert ok and dev < 1e-3
