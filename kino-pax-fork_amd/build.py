@@ -46,6 +46,17 @@ def build(verbose=False):
         if verbose and err:
             sys.stderr.write(err + "\n")
     os.replace(tmp, LIB)
+    # command-line planner over the C++ mirror (host/kpax.hpp)
+    exe = os.path.join(HERE, "plan_main")
+    cmd = ["g++", "-O2", "-std=c++17", os.path.join(HERE, "host", "plan_main.cpp"), "-o", exe + ".tmp",
+           "-L" + HERE, "-lkpax", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("plan_main build failed")
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -511,7 +511,7 @@ __device__ __forceinline__ Iter make_iter(uint32_t t, uint32_t ts, uint32_t W, c
     n.t                      = t;
     n.ts                     = ts;
     n.W                      = W;
-    const uint32_t remaining = PP.N - ts;
+    const uint32_t remaining = ts < PP.N ? PP.N - ts : 0u;
     uint32_t fan             = PP.fanout;
     uint64_t M               = (uint64_t)W * fan;
     if(M > remaining)
@@ -524,7 +524,7 @@ __device__ __forceinline__ Iter make_iter(uint32_t t, uint32_t ts, uint32_t W, c
     }
     n.M      = (uint32_t)M;
     n.fan    = fan;
-    n.status = t > PP.max_iter ? KPAX_NOT_FOUND : (remaining == 0 ? KPAX_BUDGET_EXHAUSTED : 0);
+    n.status = t > PP.max_iter ? kDoneNotFound : (remaining == 0 ? kDoneBudget : kRunning);
     return n;
 }
 

@@ -21,6 +21,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(256) void k_init(Ctl* ctl, TreeView tree, uint32_t*
         memset(&c, 0, sizeof(c));
         c.nxt        = make_iter(1u, 1u, PP.root_repeat, PP);
         c.cur        = c.nxt;
-        c.cur.status = -1;  // nothing runs until k_advance publishes nxt
+        c.cur.status = kNotStarted;  // nothing runs until k_advance publishes nxt
         c.goal       = INT_MAX;
         *ctl         = c;
     }
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(256) void k_advance(Ctl* ctl, uint64_t* st_append, 
     {
         const Iter n = ctl->nxt;
         ctl->cur     = n;
-        if(n.status == 0)
+        if(n.status == kRunning)
         {
             ctl->expansions += n.M;
             ctl->Fn       = 0;
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(256) void k_advance(Ctl* ctl, uint64_t* st_append, 
         ctl->tile_frontier = 0;
         // tiles the scans of this iteration can touch
         const uint64_t items = (uint64_t)n.ts + n.M;
-        s_clear              = n.status == 0 ? (uint32_t)((items + kTile - 1) / kTile) : 0u;
+        s_clear              = n.status == kRunning ? (uint32_t)((items + kTile - 1) / kTile) : 0u;
     }
     __syncthreads();
     for(uint32_t i = threadIdx.x; i < s_clear; i += blockDim.x)
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(256) void k_expand(Ctl* ctl, TreeView tree, ChildVi
 {
     __shared__ uint32_t s_tile;
     const Iter it = ctl->cur;
-    if(it.status != 0) return;
+    if(it.status != kRunning) return;
     const uint32_t ntiles = (it.M + 255) / 256;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     uint32_t steps_taken = 0;
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(1024) void k_score(Ctl* ctl, RegionView rv, RegionP
                                                 int32_t check_status)
 {
     __shared__ float part[1024];
-    if(check_status && ctl->cur.status != 0) return;
+    if(check_status && ctl->cur.status != kRunning) return;
     const int t = threadIdx.x;
     float s     = 0.0f;
     for(int r = t; r < RP.r1_count; r += 1024)
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, ChildView ch, TreeView
     __shared__ uint32_t lds[4];
     __shared__ uint32_t s_tile, s_pre;
     const Iter it = ctl->cur;
-    if(it.status != 0) return;
+    if(it.status != kRunning) return;
     const uint32_t N = PP.N, M = it.M, ts = it.ts;
     const uint32_t ntiles = (M + kTile - 1) / kTile;
     for(;;)
@@ -283,6 +284,11 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, ChildView ch, TreeView
         {
             if(!f[j]) continue;
             const uint32_t c = base + j;
+            if(dst >= N)
+            {
+                ctl->overflow = 1;  // cannot happen while M <= N - ts; never write past the tree
+                break;
+            }
             for(int d = 0; d < S; ++d) tree.st[(size_t)d * N + dst] = ch.st[(size_t)d * N + c];
             for(int d = 0; d < C; ++d) tree.ct[(size_t)d * N + dst] = ch.ct[(size_t)d * N + c];
             tree.steps[dst]  = ch.steps[c];
@@ -313,7 +319,7 @@ __global__ __launch_bounds__(256) void k_frontier(Ctl* ctl, TreeView tree, uint3
     __shared__ uint32_t lds[4];
     __shared__ uint32_t s_tile, s_pre;
     const Iter it = ctl->cur;
-    if(it.status != 0) return;
+    if(it.status != kRunning) return;
     const int32_t goal = ctl->goal;
     const uint32_t Fn  = ctl->Fn;
     if(goal != INT_MAX)
@@ -322,7 +328,7 @@ __global__ __launch_bounds__(256) void k_frontier(Ctl* ctl, TreeView tree, uint3
         {
             Iter n          = it;
             n.ts            = it.ts + Fn;
-            n.status        = KPAX_OK;
+            n.status        = kDoneFound;
             ctl->nxt        = n;
             ctl->iterations = it.t;
         }
@@ -618,8 +624,9 @@ struct kpax_handle {
     // last plan
     uint32_t last_tree    = 0;
     bool planned          = false;
-    // profiling
+    // profiling; KPAX_TRACE=1 prints every iteration record to stderr
     bool profiling        = false;
+    bool trace            = false;
     std::vector<hipEvent_t> ev;
     kpax_stats stats{};
     std::vector<void*> allocs;
@@ -685,15 +692,15 @@ int launch_expand(kpax_handle* h, uint64_t seed, uint32_t query, uint32_t blocks
     switch(h->cfg.model)
     {
         case KPAX_MODEL_DOUBLE_INTEGRATOR:
-            k_expand<KPAX_MODEL_DOUBLE_INTEGRATOR><<<blocks, 256, 0, h->stream>>>(
+            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DOUBLE_INTEGRATOR>, dim3(blocks), dim3(256), 0, h->stream, 
               h->ctl, h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
             break;
         case KPAX_MODEL_DUBINS_AIRPLANE:
-            k_expand<KPAX_MODEL_DUBINS_AIRPLANE><<<blocks, 256, 0, h->stream>>>(
+            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DUBINS_AIRPLANE>, dim3(blocks), dim3(256), 0, h->stream, 
               h->ctl, h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
             break;
         default:
-            k_expand<KPAX_MODEL_QUADCOPTER><<<blocks, 256, 0, h->stream>>>(h->ctl, h->tree, h->ch, h->R, h->rv,
+            hipLaunchKernelGGL(k_expand<KPAX_MODEL_QUADCOPTER>, dim3(blocks), dim3(256), 0, h->stream, h->ctl, h->tree, h->ch, h->R, h->rv,
                                                                            h->P, h->RP, G, h->PP, seed, query);
             break;
     }
@@ -741,8 +748,9 @@ int kpax_create(const kpax_config* cfg, kpax_handle** out)
         delete h;
         return KPAX_NO_DEVICE;
     }
-    h->cfg = *cfg;
-    h->S   = state_dim(cfg->model);
+    h->cfg   = *cfg;
+    h->trace = std::getenv("KPAX_TRACE") != nullptr;
+    h->S     = state_dim(cfg->model);
     h->C   = control_dim(cfg->model);
     h->P   = make_model_params(*cfg);
     h->RP  = make_region_params(*cfg);
@@ -875,8 +883,8 @@ int kpax_set_obstacles(kpax_handle* h, const float* aabb6, uint32_t n)
     HIPC(hipMemsetAsync(cursor, 0, (size_t)G * 4, h->stream));
     HIPC(hipMemsetAsync(status, 0, (G / kTile + 2) * 8, h->stream));
     HIPC(hipMemsetAsync(ticket, 0, 4, h->stream));
-    if(n) k_grid_count<<<blocks_for(n, 256), 256, 0, h->stream>>>(h->GPm, h->obs_lo, h->obs_hi, n, counts);
-    k_scan_offsets<<<std::min<uint32_t>(blocks_for(G, kTile), 4 * h->cu_count), 256, 0, h->stream>>>(
+    if(n) hipLaunchKernelGGL(k_grid_count, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->GPm, h->obs_lo, h->obs_hi, n, counts);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(std::min<uint32_t>(blocks_for(G, kTile), 4 * h->cu_count)), dim3(256), 0, h->stream, 
       counts, G, h->grid_start, status, ticket);
     HIPC(hipGetLastError());
     int32_t K = 0;
@@ -884,9 +892,9 @@ int kpax_set_obstacles(kpax_handle* h, const float* aabb6, uint32_t n)
     HIPC(hipStreamSynchronize(h->stream));
     h->grid_k = (uint32_t)K;
     HIPC(hipMalloc(&h->grid_items, std::max<size_t>(K, 1) * sizeof(int32_t)));
-    if(n) k_grid_fill<<<blocks_for(n, 256), 256, 0, h->stream>>>(h->GPm, h->obs_lo, h->obs_hi, n, h->grid_start,
+    if(n) hipLaunchKernelGGL(k_grid_fill, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->GPm, h->obs_lo, h->obs_hi, n, h->grid_start,
                                                                   cursor, h->grid_items);
-    k_grid_sort<<<blocks_for(G, 256), 256, 0, h->stream>>>(h->grid_start, h->grid_items, G);
+    hipLaunchKernelGGL(k_grid_sort, dim3(blocks_for(G, 256)), dim3(256), 0, h->stream, h->grid_start, h->grid_items, G);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     return KPAX_OK;
@@ -909,12 +917,12 @@ int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t se
     HIPC(hipMemcpyAsync(h->d_start, sg, sizeof(sg), hipMemcpyHostToDevice, h->stream));
     const uint32_t fb = std::min<uint32_t>(2048, 8 * h->cu_count);
     rec(h, 0);
-    k_fill_i32<<<fb, 256, 0, h->stream>>>(h->rv.valid, 0, h->RP.r1_count);
-    k_fill_i32<<<fb, 256, 0, h->stream>>>(h->rv.total, 0, h->RP.r1_count);
-    k_fill_i32<<<fb, 256, 0, h->stream>>>(h->rv.covered, 0, h->RP.r1_count);
-    k_fill_f32<<<fb, 256, 0, h->stream>>>(h->rv.score, 0.0f, h->RP.r1_count);
-    k_fill_u64<<<fb, 256, 0, h->stream>>>(h->rv.r2_state, ~0ull, (uint64_t)h->RP.r1_count * h->RP.r2_per);
-    k_init<<<1, 256, 0, h->stream>>>(h->ctl, h->tree, h->R, h->d_start, h->S, h->C, h->RP, h->PP);
+    hipLaunchKernelGGL(k_fill_i32, dim3(fb), dim3(256), 0, h->stream, h->rv.valid, 0, h->RP.r1_count);
+    hipLaunchKernelGGL(k_fill_i32, dim3(fb), dim3(256), 0, h->stream, h->rv.total, 0, h->RP.r1_count);
+    hipLaunchKernelGGL(k_fill_i32, dim3(fb), dim3(256), 0, h->stream, h->rv.covered, 0, h->RP.r1_count);
+    hipLaunchKernelGGL(k_fill_f32, dim3(fb), dim3(256), 0, h->stream, h->rv.score, 0.0f, h->RP.r1_count);
+    hipLaunchKernelGGL(k_fill_u64, dim3(fb), dim3(256), 0, h->stream, h->rv.r2_state, ~0ull, (uint64_t)h->RP.r1_count * h->RP.r2_per);
+    hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, h->stream, h->ctl, h->tree, h->R, h->d_start, h->S, h->C, h->RP, h->PP);
     rec(h, 1);
     HIPC(hipGetLastError());
     const uint32_t eb = std::min<uint32_t>(blocks_for(N, 256), 8 * h->cu_count);
@@ -923,16 +931,16 @@ int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t se
     for(uint32_t it = 1; it <= h->cfg.max_iter; ++it)
     {
         const size_t e = 10 * (size_t)it;
-        k_advance<<<1, 256, 0, h->stream>>>(h->ctl, h->st_append, h->st_frontier, h->PP);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(256), 0, h->stream, h->ctl, h->st_append, h->st_frontier, h->PP);
         rec(h, e + 0);
         launch_expand(h, seed, 0, eb);
         rec(h, e + 1);
-        k_score<<<1, 1024, 0, h->stream>>>(h->ctl, h->rv, h->RP, h->PP.eps, 1);
+        hipLaunchKernelGGL(k_score, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps, 1);
         rec(h, e + 2);
-        k_append<<<sb, 256, 0, h->stream>>>(h->ctl, h->ch, h->tree, h->rv.r2_state, h->st_append, h->PP, h->S, h->C,
+        hipLaunchKernelGGL(k_append, dim3(sb), dim3(256), 0, h->stream, h->ctl, h->ch, h->tree, h->rv.r2_state, h->st_append, h->PP, h->S, h->C,
                                             goal[0], goal[1], goal[2]);
         rec(h, e + 3);
-        k_frontier<<<sb, 256, 0, h->stream>>>(h->ctl, h->tree, h->R, h->rv, h->st_frontier, h->PP, seed, 0);
+        hipLaunchKernelGGL(k_frontier, dim3(sb), dim3(256), 0, h->stream, h->ctl, h->tree, h->R, h->rv, h->st_frontier, h->PP, seed, 0);
         rec(h, e + 4);
         HIPC(hipGetLastError());
         HIPC(hipMemcpyAsync(&h->h_iter[it & 1], &h->ctl->nxt, sizeof(Iter), hipMemcpyDeviceToHost, h->stream));
@@ -944,14 +952,26 @@ int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t se
         if(it >= 2)
         {
             HIPC(hipEventSynchronize(h->it_ev[(it - 1) & 1]));
-            if(h->h_iter[(it - 1) & 1].status != 0) break;
+            if(h->trace)
+            {
+                const Iter& n = h->h_iter[(it - 1) & 1];
+                std::fprintf(stderr, "kpax: after iteration %u: next t=%u ts=%u W=%u M=%u fan=%u status=%d\n",
+                             it - 1, n.t, n.ts, n.W, n.M, n.fan, n.status);
+            }
+            if(h->h_iter[(it - 1) & 1].status != kRunning) break;
         }
     }
     HIPC(hipStreamSynchronize(h->stream));
     HIPC(hipMemcpy(h->h_ctl, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     const Ctl c          = *h->h_ctl;
-    int status           = c.nxt.status;
-    if(status == 0) status = KPAX_NOT_FOUND;  // ran out of launched iterations
+    if(c.overflow) return fail(h, KPAX_DEVICE_ERROR, "tree index past the configured capacity");
+    int status;
+    switch(c.nxt.status)
+    {
+        case kDoneFound: status = KPAX_OK; break;
+        case kDoneBudget: status = KPAX_BUDGET_EXHAUSTED; break;
+        default: status = KPAX_NOT_FOUND; break;  // max_iter reached
+    }
     out->status          = status;
     out->iterations      = c.iterations;
     out->tree_size       = c.nxt.ts;
@@ -964,7 +984,7 @@ int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t se
     if(status == KPAX_OK && c.goal != INT_MAX)
     {
         out->goal_index = c.goal;
-        k_path<<<1, 64, 0, h->stream>>>(h->ctl, h->tree, h->S, h->C, N, h->cfg.dt, h->path_rows, h->path_steps,
+        hipLaunchKernelGGL(k_path, dim3(1), dim3(64), 0, h->stream, h->ctl, h->tree, h->S, h->C, N, h->cfg.dt, h->path_rows, h->path_steps,
                                          h->path_idx, h->path_cap);
         HIPC(hipGetLastError());
         uint32_t len = 0;
@@ -1136,13 +1156,13 @@ int kpax_propagate_batch(kpax_handle* h, const float* x0_soa, const float* u_soa
     switch(h->cfg.model)
     {
         case KPAX_MODEL_DOUBLE_INTEGRATOR:
-            k_propagate<KPAX_MODEL_DOUBLE_INTEGRATOR><<<b, 256, 0, h->stream>>>(h->P, G, dx0, du, dst, n, dx1, dv, df);
+            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_DOUBLE_INTEGRATOR>, dim3(b), dim3(256), 0, h->stream, h->P, G, dx0, du, dst, n, dx1, dv, df);
             break;
         case KPAX_MODEL_DUBINS_AIRPLANE:
-            k_propagate<KPAX_MODEL_DUBINS_AIRPLANE><<<b, 256, 0, h->stream>>>(h->P, G, dx0, du, dst, n, dx1, dv, df);
+            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_DUBINS_AIRPLANE>, dim3(b), dim3(256), 0, h->stream, h->P, G, dx0, du, dst, n, dx1, dv, df);
             break;
         default:
-            k_propagate<KPAX_MODEL_QUADCOPTER><<<b, 256, 0, h->stream>>>(h->P, G, dx0, du, dst, n, dx1, dv, df);
+            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_QUADCOPTER>, dim3(b), dim3(256), 0, h->stream, h->P, G, dx0, du, dst, n, dx1, dv, df);
             break;
     }
     HIPC(hipGetLastError());
@@ -1165,7 +1185,7 @@ int kpax_collide_batch(kpax_handle* h, const float* bbmin_soa, const float* bbma
     if(!lo || !hi || !o) return fail(h, KPAX_NO_MEMORY, "scratch allocation failed");
     HIPC(hipMemcpy(lo, bbmin_soa, 12 * (size_t)n, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(hi, bbmax_soa, 12 * (size_t)n, hipMemcpyHostToDevice));
-    k_collide<<<blocks_for(n, 256), 256, 0, h->stream>>>(grid_view(h), lo, hi, n, o);
+    hipLaunchKernelGGL(k_collide, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, grid_view(h), lo, hi, n, o);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     HIPC(hipMemcpy(free_out, o, n, hipMemcpyDeviceToHost));
@@ -1183,7 +1203,7 @@ int kpax_region_batch(kpax_handle* h, const float* states_soa, uint32_t n, int32
     int32_t* b = sc.get<int32_t>(n);
     if(!st || !a || !b) return fail(h, KPAX_NO_MEMORY, "scratch allocation failed");
     HIPC(hipMemcpy(st, states_soa, 12 * (size_t)n, hipMemcpyHostToDevice));
-    k_regions<<<blocks_for(n, 256), 256, 0, h->stream>>>(h->RP, st, n, a, b);
+    hipLaunchKernelGGL(k_regions, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->RP, st, n, a, b);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     HIPC(hipMemcpy(r1, a, 4 * (size_t)n, hipMemcpyDeviceToHost));
@@ -1217,8 +1237,8 @@ int kpax_region_histogram(kpax_handle* h, const float* states_soa, const uint8_t
     HIPC(hipMemcpy(rv.valid, valid_count, r1 * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(rv.covered, covered, r1 * 4, hipMemcpyHostToDevice));
     // the batch is one iteration on an empty coverage map
-    k_fill_u64<<<blocks_for(r2, 256), 256, 0, h->stream>>>(rv.r2_state, ~0ull, r2);
-    if(n) k_histogram<<<blocks_for(n, 256), 256, 0, h->stream>>>(h->RP, st, v, n, 1u, rv);
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(r2, 256)), dim3(256), 0, h->stream, rv.r2_state, ~0ull, r2);
+    if(n) hipLaunchKernelGGL(k_histogram, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->RP, st, v, n, 1u, rv);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     HIPC(hipMemcpy(total, rv.total, r1 * 4, hipMemcpyDeviceToHost));
@@ -1242,7 +1262,7 @@ int kpax_scores(kpax_handle* h, const int32_t* valid_count, const int32_t* total
     HIPC(hipMemcpy(rv.valid, valid_count, r1 * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(rv.total, total, r1 * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(rv.covered, covered, r1 * 4, hipMemcpyHostToDevice));
-    k_score<<<1, 1024, 0, h->stream>>>(nullptr, rv, h->RP, h->PP.eps, 0);
+    hipLaunchKernelGGL(k_score, dim3(1), dim3(1024), 0, h->stream, nullptr, rv, h->RP, h->PP.eps, 0);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     HIPC(hipMemcpy(scores, rv.score, r1 * 4, hipMemcpyDeviceToHost));
@@ -1269,7 +1289,7 @@ int kpax_repeat_expand(kpax_handle* h, const uint32_t* weights, uint32_t n, uint
     HIPC(hipMemcpy(dw, weights, (size_t)n * 4, hipMemcpyHostToDevice));
     HIPC(hipMemsetAsync(status, 0, (ntiles + 1) * 8, h->stream));
     HIPC(hipMemsetAsync(ticket, 0, 8, h->stream));
-    k_repeat_expand<<<std::min<uint32_t>(ntiles, 4 * h->cu_count), 256, 0, h->stream>>>(dw, n, dout, out_cap, status,
+    hipLaunchKernelGGL(k_repeat_expand, dim3(std::min<uint32_t>(ntiles, 4 * h->cu_count)), dim3(256), 0, h->stream, dw, n, dout, out_cap, status,
                                                                                          ticket, ticket + 1);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));

@@ -13,6 +13,10 @@ namespace kpax {
 constexpr int kMaxS = 12;
 constexpr int kMaxC = 4;
 
+// Iteration status codes on the device.  Terminal codes are kept apart from
+// "running" (0): KPAX_OK is also 0, so the host maps them to kpax_status.
+enum : int32_t { kRunning = 0, kDoneFound = 1, kDoneNotFound = 2, kDoneBudget = 3, kNotStarted = 4 };
+
 // Per-iteration control record.  The frontier kernel of iteration t writes
 // `nxt`; the one-block advance kernel copies it to `cur` at the start of
 // iteration t+1, so no kernel ever changes a field its own blocks read.
@@ -22,7 +26,7 @@ struct Iter {
     uint32_t W;       // length of the frontier repeat list (may exceed its capacity)
     uint32_t M;       // children expanded in this iteration
     uint32_t fan;     // children per repeat entry
-    int32_t status;   // 0 = running, else the final kpax_status
+    int32_t status;   // kRunning, or a terminal code (kDone*)
 };
 
 struct Ctl {
@@ -39,7 +43,8 @@ struct Ctl {
     uint32_t tile_append;
     uint32_t tile_frontier;
     uint32_t path_len;     // rows written by the path kernel
-    uint32_t pad[3];
+    uint32_t overflow;     // set if any kernel found an index past its buffer
+    uint32_t pad[2];
 };
 
 // Dynamics and checks, with every constant folded on the host exactly as the

@@ -15,7 +15,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libkpax.so")
+# KPAX_LIB selects another build of the same library (for instance the CPU
+# build of the kernels that the test suite runs under AddressSanitizer).
+LIB_PATH = os.environ.get("KPAX_LIB", os.path.join(HERE, "libkpax.so"))
 
 STATUS = {0: "OK", 1: "NOT_FOUND", 2: "BUDGET_EXHAUSTED", -1: "INVALID_ARG", -2: "DEVICE_ERROR",
           -3: "NO_MEMORY", -4: "NO_DEVICE", -5: "IO_ERROR"}

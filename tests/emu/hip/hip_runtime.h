@@ -1,0 +1,285 @@
+// hip/hip_runtime.h for a CPU build of the kpax kernels (test harness).
+//
+// The kernel source is compiled unchanged by the host compiler: each
+// work-group runs as blockDim.x host threads (work-groups one after another,
+// in launch order), __syncthreads is a barrier over the group, and the wave
+// intrinsics (__shfl_up, __shfl_xor, __ballot) exchange values through a
+// per-wave buffer between barriers over the 64 lanes of that wave.  Device
+// memory is host memory from malloc, so AddressSanitizer sees every access a
+// kernel makes.  Runtime calls are synchronous.
+#pragma once
+
+#include <atomic>
+#include <barrier>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct emu_idx {
+    unsigned x, y, z;
+};
+struct uint4 {
+    uint32_t x, y, z, w;
+};
+struct float4 {
+    float x, y, z, w;
+};
+inline uint4 make_uint4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return {a, b, c, d}; }
+inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
+
+inline thread_local emu_idx threadIdx{0, 0, 0};
+inline thread_local emu_idx blockIdx{0, 0, 0};
+inline emu_idx blockDim{1, 1, 1};
+inline emu_idx gridDim{1, 1, 1};
+
+struct emu_group {
+    std::unique_ptr<std::barrier<>> bar;
+    std::vector<std::unique_ptr<std::barrier<>>> wave_bar;
+    std::vector<uint64_t> xbuf;  // 64 slots per wave
+};
+inline emu_group* emu_g = nullptr;
+
+inline void __syncthreads() { emu_g->bar->arrive_and_wait(); }
+
+inline uint64_t emu_exchange(uint64_t v, unsigned src_lane, bool* ok = nullptr)
+{
+    const unsigned t = threadIdx.x, w = t / 64, lane = t % 64;
+    auto& b          = *emu_g->wave_bar[w];
+    b.arrive_and_wait();
+    emu_g->xbuf[w * 64 + lane] = v;
+    b.arrive_and_wait();
+    const uint64_t r = emu_g->xbuf[w * 64 + (src_lane % 64)];
+    b.arrive_and_wait();
+    (void)ok;
+    return r;
+}
+
+template<class T> inline T __shfl_up(T x, unsigned off, int = 64)
+{
+    const unsigned lane = threadIdx.x % 64;
+    uint64_t bits       = 0;
+    std::memcpy(&bits, &x, sizeof(T));
+    const uint64_t r = emu_exchange(bits, lane >= off ? lane - off : lane);
+    T y;
+    std::memcpy(&y, &r, sizeof(T));
+    return y;
+}
+
+template<class T> inline T __shfl_xor(T x, unsigned m, int = 64)
+{
+    const unsigned lane = threadIdx.x % 64;
+    uint64_t bits       = 0;
+    std::memcpy(&bits, &x, sizeof(T));
+    const uint64_t r = emu_exchange(bits, lane ^ m);
+    T y;
+    std::memcpy(&y, &r, sizeof(T));
+    return y;
+}
+
+inline uint64_t __ballot(int pred)
+{
+    const unsigned t = threadIdx.x, w = t / 64, lane = t % 64;
+    auto& b          = *emu_g->wave_bar[w];
+    b.arrive_and_wait();
+    emu_g->xbuf[w * 64 + lane] = pred ? 1 : 0;
+    b.arrive_and_wait();
+    uint64_t m = 0;
+    for(unsigned i = 0; i < 64; ++i)
+        if(emu_g->xbuf[w * 64 + i]) m |= 1ull << i;
+    b.arrive_and_wait();
+    return m;
+}
+
+inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+
+template<class T> inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+template<class T> inline T atomicMin(T* p, T v)
+{
+    T cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while(v < cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+    return cur;
+}
+template<class T> inline T atomicCAS(T* p, T cmp, T v)
+{
+    __atomic_compare_exchange_n(p, &cmp, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+    return cmp;
+}
+
+#define __HIP_MEMORY_SCOPE_AGENT 0
+#define __hip_atomic_load(p, order, scope) __atomic_load_n((p), (order))
+#define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), (order))
+
+// ---- runtime ----------------------------------------------------------------
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorUnknown = 999 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyDefault };
+typedef struct emu_stream_t* hipStream_t;
+typedef struct emu_event_t* hipEvent_t;
+struct hipDeviceProp_t {
+    int multiProcessorCount;
+};
+#define hipStreamNonBlocking 1
+#define hipEventDisableTiming 2
+#define hipHostMallocDefault 0
+
+inline hipError_t hipGetDeviceCount(int* n)
+{
+    *n = 1;
+    return hipSuccess;
+}
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int)
+{
+    p->multiProcessorCount = 1;
+    return hipSuccess;
+}
+inline hipError_t hipMalloc(void** p, size_t n)
+{
+    *p = std::malloc(n ? n : 1);
+    std::memset(*p, 0xA5, n ? n : 1);  // device memory starts as garbage
+    return *p ? hipSuccess : hipErrorUnknown;
+}
+template<class T> inline hipError_t hipMalloc(T** p, size_t n) { return hipMalloc((void**)p, n); }
+inline hipError_t hipFree(void* p)
+{
+    std::free(p);
+    return hipSuccess;
+}
+inline hipError_t hipHostMalloc(void** p, size_t n, int) { return hipMalloc(p, n); }
+inline hipError_t hipHostFree(void* p) { return hipFree(p); }
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind)
+{
+    std::memcpy(d, s, n);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t)
+{
+    return hipMemcpy(d, s, n, k);
+}
+inline hipError_t hipMemset(void* d, int v, size_t n)
+{
+    std::memset(d, v, n);
+    return hipSuccess;
+}
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { return hipMemset(d, v, n); }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned)
+{
+    *s = (hipStream_t)0x1;
+    return hipSuccess;
+}
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t* e)
+{
+    *e = (hipEvent_t)0x1;
+    return hipSuccess;
+}
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t)
+{
+    *ms = 0.0f;
+    return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline const char* hipGetErrorString(hipError_t) { return "emulated"; }
+
+// Persistent workers, one per work-item slot of the largest work-group.
+struct emu_pool {
+    std::vector<std::thread> th;
+    std::mutex m;
+    std::condition_variable cv_start, cv_done;
+    uint64_t gen       = 0;
+    unsigned active    = 0;
+    unsigned remaining = 0;
+    std::function<void(unsigned)> job;
+    explicit emu_pool(unsigned n)
+    {
+        for(unsigned i = 0; i < n; ++i) th.emplace_back([this, i] { worker(i); });
+    }
+    void worker(unsigned i)
+    {
+        uint64_t seen = 0;
+        for(;;)
+        {
+            std::function<void(unsigned)> j;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv_start.wait(lk, [&] { return gen != seen; });
+                seen = gen;
+                if(i >= active) continue;
+                j = job;
+            }
+            j(i);
+            std::lock_guard<std::mutex> lk(m);
+            if(--remaining == 0) cv_done.notify_one();
+        }
+    }
+    void run(unsigned n, std::function<void(unsigned)> f)
+    {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            job       = std::move(f);
+            active    = n;
+            remaining = n;
+            ++gen;
+        }
+        cv_start.notify_all();
+        std::unique_lock<std::mutex> lk(m);
+        cv_done.wait(lk, [&] { return remaining == 0; });
+    }
+};
+
+inline emu_pool& emu_workers()
+{
+    static emu_pool* p = new emu_pool(1024);  // never joined: lives until exit
+    return *p;
+}
+
+template<class... KArgs, class... Args>
+void emu_launch(void (*k)(KArgs...), dim3 grid, dim3 block, Args... args)
+{
+    gridDim  = {grid.x, 1, 1};
+    blockDim = {block.x, 1, 1};
+    const unsigned nw = (block.x + 63) / 64;
+    for(unsigned b = 0; b < grid.x; ++b)
+    {
+        emu_group g;
+        g.bar = std::make_unique<std::barrier<>>(block.x);
+        for(unsigned w = 0; w < nw; ++w)
+            g.wave_bar.push_back(std::make_unique<std::barrier<>>(std::min(64u, block.x - 64 * w)));
+        g.xbuf.assign(64 * nw, 0);
+        emu_g = &g;
+        emu_workers().run(block.x, [&, b](unsigned t) {
+            threadIdx = {t, 0, 0};
+            blockIdx  = {b, 0, 0};
+            k(args...);
+            // a work-item that returns early no longer takes part in barriers
+            g.wave_bar[t / 64]->arrive_and_drop();
+            g.bar->arrive_and_drop();
+        });
+        emu_g = nullptr;
+    }
+}
+
+#define hipLaunchKernelGGL(k, grid, block, shmem, stream, ...) emu_launch(k, grid, block, __VA_ARGS__)
