@@ -171,94 +171,72 @@ template<> struct Dims<KPAX_MODEL_QUADCOPTER> {
     static constexpr int S = 12, C = 4;
 };
 
-// Double integrator: statePropagatorSpatialHash.cu:88-129.
-__device__ __forceinline__ int prop_di(const ModelParams& P, const GridView& G, const float* x0, const float* u,
-                                       int steps, float* x1)
+// One integration step of each model on the state x (in place) with the
+// controls u, followed by the reference's checks; returns false when the step
+// is invalid (the state then holds the failing step's result, as the
+// reference leaves it).  The expression text is the reference's, so the
+// float result does not depend on how steps are scheduled.
+
+// Double integrator: statePropagatorSpatialHash.cu:90-128.
+__device__ __forceinline__ bool step_di(const ModelParams& P, const GridView& G, float* s, const float* u)
 {
     const float dt = P.dt;
-    float x = x0[0], y = x0[1], z = x0[2], vx = x0[3], vy = x0[4], vz = x0[5];
+    float x = s[0], y = s[1], z = s[2], vx = s[3], vy = s[4], vz = s[5];
     const float ax = u[0], ay = u[1], az = u[2];
-    int fail = 0;
-    for(int i = 0; i < steps; ++i)
+    const float p0[3] = {x, y, z};
+    if(!P.di_textbook)
     {
-        const float p0[3] = {x, y, z};
-        if(!P.di_textbook)
-        {
-            x += (vx + (vx + 2.0f * (vx + ax * dt / 2.0f) + (vx + ax * dt))) * dt / 6.0f;
-            y += (vy + (vy + 2.0f * (vy + ay * dt / 2.0f) + (vy + ay * dt))) * dt / 6.0f;
-            z += (vz + (vz + 2.0f * (vz + az * dt / 2.0f) + (vz + az * dt))) * dt / 6.0f;
-            vx += (ax + 2.0f * ax + 2.0f * ax + ax) * dt / 6.0f;
-            vy += (ay + 2.0f * ay + 2.0f * ay + ay) * dt / 6.0f;
-            vz += (az + 2.0f * az + 2.0f * az + az) * dt / 6.0f;
-        }
-        else
-        {
-            x += vx * dt + 0.5f * ax * dt * dt;
-            y += vy * dt + 0.5f * ay * dt * dt;
-            z += vz * dt + 0.5f * az * dt * dt;
-            vx += ax * dt;
-            vy += ay * dt;
-            vz += az * dt;
-        }
-        if(!inb(vx, P.vmin, P.vmax) || !inb(vy, P.vmin, P.vmax) || !inb(vz, P.vmin, P.vmax))
-        {
-            fail = i + 1;
-            break;
-        }
-        const float p1[3] = {x, y, z};
-        if(!ws_and_collision(P, G, p0, p1))
-        {
-            fail = i + 1;
-            break;
-        }
+        x += (vx + (vx + 2.0f * (vx + ax * dt / 2.0f) + (vx + ax * dt))) * dt / 6.0f;
+        y += (vy + (vy + 2.0f * (vy + ay * dt / 2.0f) + (vy + ay * dt))) * dt / 6.0f;
+        z += (vz + (vz + 2.0f * (vz + az * dt / 2.0f) + (vz + az * dt))) * dt / 6.0f;
+        vx += (ax + 2.0f * ax + 2.0f * ax + ax) * dt / 6.0f;
+        vy += (ay + 2.0f * ay + 2.0f * ay + ay) * dt / 6.0f;
+        vz += (az + 2.0f * az + 2.0f * az + az) * dt / 6.0f;
     }
-    x1[0] = x, x1[1] = y, x1[2] = z, x1[3] = vx, x1[4] = vy, x1[5] = vz;
-    return fail;
+    else
+    {
+        x += vx * dt + 0.5f * ax * dt * dt;
+        y += vy * dt + 0.5f * ay * dt * dt;
+        z += vz * dt + 0.5f * az * dt * dt;
+        vx += ax * dt;
+        vy += ay * dt;
+        vz += az * dt;
+    }
+    s[0] = x, s[1] = y, s[2] = z, s[3] = vx, s[4] = vy, s[5] = vz;
+    if(!inb(vx, P.vmin, P.vmax) || !inb(vy, P.vmin, P.vmax) || !inb(vz, P.vmin, P.vmax)) return false;
+    const float p1[3] = {x, y, z};
+    return ws_and_collision(P, G, p0, p1);
 }
 
-// Dubins airplane: statePropagatorSpatialHash.cu:156-215.
-__device__ __forceinline__ int prop_dubins(const ModelParams& P, const GridView& G, const float* x0, const float* u,
-                                           int steps, float* x1)
+// Dubins airplane: statePropagatorSpatialHash.cu:158-214.
+__device__ __forceinline__ bool step_dubins(const ModelParams& P, const GridView& G, float* s, const float* u)
 {
     const float dt = P.dt, h = P.half, d6 = P.d6;
-    float x = x0[0], y = x0[1], z = x0[2], yaw = x0[3], pitch = x0[4], v = x0[5];
+    float x = s[0], y = s[1], z = s[2], yaw = s[3], pitch = s[4], v = s[5];
     const float a = u[0], yr = u[1], pr = u[2];
-    int fail = 0;
-    for(int i = 0; i < steps; ++i)
-    {
-        const float p0[3] = {x, y, z};
-        float cp, sp, cy, sy, cpm, spm, cym, sym, cpe, spe, cye, sye;
-        sincos_f32(pitch, &sp, &cp);
-        sincos_f32(yaw, &sy, &cy);
-        sincos_f32(pitch + h * pr, &spm, &cpm);
-        sincos_f32(yaw + h * yr, &sym, &cym);
-        sincos_f32(pitch + dt * pr, &spe, &cpe);
-        sincos_f32(yaw + dt * yr, &sye, &cye);
-        const float A  = v + h * a;
-        const float B  = v + dt * a;
-        const float Tx = A * cpm * cym;
-        const float Ty = A * cpm * sym;
-        const float Tz = A * spm;
-        x += d6 * (v * cp * cy + 2.0f * (Tx + Tx) + B * cpe * cye);
-        y += d6 * (v * cp * sy + 2.0f * (Ty + Ty) + B * cpe * sye);
-        z += d6 * (v * sp + 2.0f * (Tz + Tz) + B * spe);
-        yaw += dt * yr;
-        pitch += dt * pr;
-        v += d6 * (a + 2.0f * (a + a) + a);
-        if(!inb(v, P.vmin, P.vmax) || !inb(pitch, P.pmin, P.pmax))
-        {
-            fail = i + 1;
-            break;
-        }
-        const float p1[3] = {x, y, z};
-        if(!ws_and_collision(P, G, p0, p1))
-        {
-            fail = i + 1;
-            break;
-        }
-    }
-    x1[0] = x, x1[1] = y, x1[2] = z, x1[3] = yaw, x1[4] = pitch, x1[5] = v;
-    return fail;
+    const float p0[3] = {x, y, z};
+    float cp, sp, cy, sy, cpm, spm, cym, sym, cpe, spe, cye, sye;
+    sincos_f32(pitch, &sp, &cp);
+    sincos_f32(yaw, &sy, &cy);
+    sincos_f32(pitch + h * pr, &spm, &cpm);
+    sincos_f32(yaw + h * yr, &sym, &cym);
+    sincos_f32(pitch + dt * pr, &spe, &cpe);
+    sincos_f32(yaw + dt * yr, &sye, &cye);
+    const float A  = v + h * a;
+    const float B  = v + dt * a;
+    const float Tx = A * cpm * cym;
+    const float Ty = A * cpm * sym;
+    const float Tz = A * spm;
+    x += d6 * (v * cp * cy + 2.0f * (Tx + Tx) + B * cpe * cye);
+    y += d6 * (v * cp * sy + 2.0f * (Ty + Ty) + B * cpe * sye);
+    z += d6 * (v * sp + 2.0f * (Tz + Tz) + B * spe);
+    yaw += dt * yr;
+    pitch += dt * pr;
+    v += d6 * (a + 2.0f * (a + a) + a);
+    s[0] = x, s[1] = y, s[2] = z, s[3] = yaw, s[4] = pitch, s[5] = v;
+    if(!inb(v, P.vmin, P.vmax) || !inb(pitch, P.pmin, P.pmax)) return false;
+    const float p1[3] = {x, y, z};
+    return ws_and_collision(P, G, p0, p1);
 }
 
 // 12D quadcopter ode: statePropagator.cu:325-370.
@@ -289,52 +267,47 @@ __device__ __forceinline__ void quad_ode(float* out, const float* x, const float
     out[11]         = k.k1z * p * q + k.izi * LMN * r + k.izi * u[3];
 }
 
-// Quadcopter RK4: statePropagatorSpatialHash.cu:246-307.
-__device__ __forceinline__ int prop_quad(const ModelParams& P, const GridView& G, const float* x0, const float* u,
-                                         int steps, float* x1)
+// Quadcopter RK4: statePropagatorSpatialHash.cu:256-296.
+__device__ __forceinline__ bool step_quad(const ModelParams& P, const GridView& G, float* h0, const float* u)
 {
     const float dt = P.dt, half = P.half, d6 = P.d6;
-    float h0[12], k1[12], k2[12], k3[12], k4[12];
+    float k1[12], k2[12], k3[12], k4[12];
     const float zero[12] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    const float p0[3] = {h0[0], h0[1], h0[2]};
+    quad_ode(k1, h0, zero, 1.0f, u, P);
+    quad_ode(k2, h0, k1, half, u, P);
+    quad_ode(k3, h0, k2, half, u, P);
+    quad_ode(k4, h0, k3, dt, u, P);
 #pragma unroll
-    for(int j = 0; j < 12; ++j) h0[j] = x0[j];
-    int fail = 0;
-    for(int i = 0; i < steps; ++i)
-    {
-        const float p0[3] = {h0[0], h0[1], h0[2]};
-        quad_ode(k1, h0, zero, 1.0f, u, P);
-        quad_ode(k2, h0, k1, half, u, P);
-        quad_ode(k3, h0, k2, half, u, P);
-        quad_ode(k4, h0, k3, dt, u, P);
-#pragma unroll
-        for(int j = 0; j < 12; ++j) h0[j] += d6 * (k1[j] + 2.0f * k2[j] + 2.0f * k3[j] + k4[j]);
-        if(!inb(h0[6], P.vmin, P.vmax) || !inb(h0[7], P.vmin, P.vmax) || !inb(h0[8], P.vmin, P.vmax))
-        {
-            fail = i + 1;
-            break;
-        }
-        const float p1[3] = {h0[0], h0[1], h0[2]};
-        if(!ws_and_collision(P, G, p0, p1))
-        {
-            fail = i + 1;
-            break;
-        }
-    }
-#pragma unroll
-    for(int j = 0; j < 12; ++j) x1[j] = h0[j];
-    return fail;
+    for(int j = 0; j < 12; ++j) h0[j] += d6 * (k1[j] + 2.0f * k2[j] + 2.0f * k3[j] + k4[j]);
+    if(!inb(h0[6], P.vmin, P.vmax) || !inb(h0[7], P.vmin, P.vmax) || !inb(h0[8], P.vmin, P.vmax)) return false;
+    const float p1[3] = {h0[0], h0[1], h0[2]};
+    return ws_and_collision(P, G, p0, p1);
 }
 
+template<int MODEL>
+__device__ __forceinline__ bool step(const ModelParams& P, const GridView& G, float* s, const float* u)
+{
+    if constexpr(MODEL == KPAX_MODEL_DOUBLE_INTEGRATOR)
+        return step_di(P, G, s, u);
+    else if constexpr(MODEL == KPAX_MODEL_DUBINS_AIRPLANE)
+        return step_dubins(P, G, s, u);
+    else
+        return step_quad(P, G, s, u);
+}
+
+// A whole edge: `steps` steps, stopping at the first invalid one.  Returns the
+// 1-based failing step or 0; x1 holds the last state computed.
 template<int MODEL>
 __device__ __forceinline__ int propagate(const ModelParams& P, const GridView& G, const float* x0, const float* u,
                                          int steps, float* x1)
 {
-    if constexpr(MODEL == KPAX_MODEL_DOUBLE_INTEGRATOR)
-        return prop_di(P, G, x0, u, steps, x1);
-    else if constexpr(MODEL == KPAX_MODEL_DUBINS_AIRPLANE)
-        return prop_dubins(P, G, x0, u, steps, x1);
-    else
-        return prop_quad(P, G, x0, u, steps, x1);
+    constexpr int S = Dims<MODEL>::S;
+#pragma unroll
+    for(int d = 0; d < S; ++d) x1[d] = x0[d];
+    for(int i = 0; i < steps; ++i)
+        if(!step<MODEL>(P, G, x1, u)) return i + 1;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -140,67 +140,105 @@ __global__ __launch_bounds__(256) void k_advance(Ctl* ctl, uint64_t* st_append, 
 }
 
 // ---------------------------------------------------------------------------
-// Expansion: one lane per child, 256-child tiles handed out dynamically.
-// propagateFrontier_kernel1/2 (KPAX.cu:358-439) with counter-based draws.
+// Expansion: propagateFrontier_kernel1/2 (KPAX.cu:358-439) with counter-based
+// draws.  Waves are persistent and advance their 64 children one integration
+// step per trip through the loop; a lane whose child has finished (all its
+// steps, or an invalid step) writes the child out and takes the next one
+// from the wave's 64-child chunk, and the wave takes a new chunk from a
+// global counter when its chunk runs out.  Children draw 5..20 steps and many
+// stop early, so refilling keeps the lanes of a wave busy instead of waiting
+// for the longest edge.  Each child's arithmetic is that of propagate().
 // ---------------------------------------------------------------------------
-template<int MODEL>
-__device__ __forceinline__ uint32_t expand_child(uint32_t c, const Iter& it, const TreeView& tree, const ChildView& ch,
-                                             const uint32_t* R, const RegionView& rv, const ModelParams& P,
-                                             const RegionParams& RP, const GridView& G, uint32_t N, uint32_t k0,
-                                             uint32_t k1, uint32_t query)
-{
-    constexpr int S = Dims<MODEL>::S, C = Dims<MODEL>::C;
-    const uint32_t p = R[c / it.fan];
-    float x0[S];
-#pragma unroll
-    for(int d = 0; d < S; ++d) x0[d] = tree.st[(size_t)d * N + p];
-    const uint4 dc = philox(c, it.t, query, PURPOSE_CONTROL, k0, k1);
-    const uint4 ds = philox(c, it.t, query, PURPOSE_STEPS_ADMIT, k0, k1);
-    const uint32_t dcw[4] = {dc.x, dc.y, dc.z, dc.w};
-    float u[C];
-#pragma unroll
-    for(int i = 0; i < C; ++i) u[i] = P.ctrl_min[i] + u01(dcw[i]) * P.ctrl_rng[i];
-    int kk = (int)(u01(ds.x) * (float)P.span);
-    if(kk > P.span - 1) kk = P.span - 1;
-    const int steps = P.steps_min + kk;
-    float x1[S];
-    const int fail = propagate<MODEL>(P, G, x0, u, steps, x1);
-    int r1, r2;
-    locate(RP, x1, &r1, &r2);
-    uint32_t flags = count_child(rv.total, rv.valid, rv.covered, rv.r2_state, r1, r2, fail == 0, it.t, c);
-    if(fail == 0 && u01(ds.y) < rv.score[r1]) flags |= kScoreAdmit;
-#pragma unroll
-    for(int d = 0; d < S; ++d) ch.st[(size_t)d * N + c] = x1[d];
-#pragma unroll
-    for(int d = 0; d < C; ++d) ch.ct[(size_t)d * N + c] = u[d];
-    ch.steps[c]  = steps;
-    ch.parent[c] = (int32_t)p;
-    ch.reg[c]    = r1;
-    ch.r2[c]     = r2;
-    ch.adm[c]    = (uint8_t)flags;
-    return fail ? (uint32_t)fail : (uint32_t)steps;
-}
-
 template<int MODEL>
 __global__ __launch_bounds__(256) void k_expand(Ctl* ctl, TreeView tree, ChildView ch, const uint32_t* R,
                                                 RegionView rv, ModelParams P, RegionParams RP, GridView G,
                                                 PlanParams PP, uint64_t seed, uint32_t query)
 {
-    __shared__ uint32_t s_tile;
+    constexpr int S = Dims<MODEL>::S, C = Dims<MODEL>::C;
+    constexpr uint32_t kNone = 0xffffffffu;
     const Iter it = ctl->cur;
     if(it.status != kRunning) return;
-    const uint32_t ntiles = (it.M + 255) / 256;
+    const uint32_t M = it.M, N = PP.N, t = it.t;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-    uint32_t steps_taken = 0;
+    const int lane      = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t c = kNone, parent = 0, admit_draw = 0;
+    int i = 0, n = 0;
+    float x[S], u[C];
+    uint32_t q_next = 0, q_end = 0;  // the wave's current chunk [q_next, q_end)
+    bool drained        = false;
+    uint32_t steps_done = 0;
     for(;;)
     {
-        const uint32_t tile = next_tile(&ctl->tile_expand, &s_tile);
-        if(tile >= ntiles) break;
-        const uint32_t c = tile * 256 + threadIdx.x;
-        if(c < it.M) steps_taken += expand_child<MODEL>(c, it, tree, ch, R, rv, P, RP, G, PP.N, k0, k1, query);
+        if(!drained)
+        {
+            uint64_t need = __ballot(c == kNone);
+            while(need)
+            {
+                if(q_next >= q_end)
+                {
+                    uint32_t base = 0;
+                    if(lane == 0) base = atomicAdd(&ctl->tile_expand, 64u);
+                    base = __shfl(base, 0, 64);
+                    if(base >= M)
+                    {
+                        drained = true;
+                        break;
+                    }
+                    q_next = base;
+                    q_end  = base + 64u < M ? base + 64u : M;
+                }
+                const uint32_t avail = q_end - q_next;
+                const uint32_t rank  = (uint32_t)__popcll(need & below);
+                if(c == kNone && rank < avail)
+                {
+                    c      = q_next + rank;
+                    parent = R[c / it.fan];
+#pragma unroll
+                    for(int d = 0; d < S; ++d) x[d] = tree.st[(size_t)d * N + parent];
+                    const uint4 dc        = philox(c, t, query, PURPOSE_CONTROL, k0, k1);
+                    const uint4 ds        = philox(c, t, query, PURPOSE_STEPS_ADMIT, k0, k1);
+                    const uint32_t dcw[4] = {dc.x, dc.y, dc.z, dc.w};
+#pragma unroll
+                    for(int j = 0; j < C; ++j) u[j] = P.ctrl_min[j] + u01(dcw[j]) * P.ctrl_rng[j];
+                    int kk = (int)(u01(ds.x) * (float)P.span);
+                    if(kk > P.span - 1) kk = P.span - 1;
+                    n          = P.steps_min + kk;
+                    i          = 0;
+                    admit_draw = ds.y;
+                }
+                const uint32_t wanted = (uint32_t)__popcll(need);
+                q_next += wanted < avail ? wanted : avail;
+                need = __ballot(c == kNone);
+            }
+        }
+        if(__ballot(c != kNone) == 0) break;
+        if(c != kNone)
+        {
+            const bool ok = step<MODEL>(P, G, x, u);
+            ++i;
+            if(!ok || i == n)
+            {
+                steps_done += (uint32_t)i;
+                int r1, r2;
+                locate(RP, x, &r1, &r2);
+                uint32_t flags = count_child(rv.total, rv.valid, rv.covered, rv.r2_state, r1, r2, ok, t, c);
+                if(ok && u01(admit_draw) < rv.score[r1]) flags |= kScoreAdmit;
+#pragma unroll
+                for(int d = 0; d < S; ++d) ch.st[(size_t)d * N + c] = x[d];
+#pragma unroll
+                for(int d = 0; d < C; ++d) ch.ct[(size_t)d * N + c] = u[d];
+                ch.steps[c]  = n;
+                ch.parent[c] = (int32_t)parent;
+                ch.reg[c]    = r1;
+                ch.r2[c]     = r2;
+                ch.adm[c]    = (uint8_t)flags;
+                c            = kNone;
+            }
+        }
     }
-    const uint32_t ws = wave_sum(steps_taken);
-    if((threadIdx.x & 63) == 0 && ws) atomicAdd((unsigned long long*)&ctl->steps_done, (unsigned long long)ws);
+    const uint32_t ws = wave_sum(steps_done);
+    if(lane == 0 && ws) atomicAdd((unsigned long long*)&ctl->steps_done, (unsigned long long)ws);
 }
 
 // ---------------------------------------------------------------------------

@@ -94,6 +94,18 @@ template<class T> inline T __shfl_xor(T x, unsigned m, int = 64)
     return y;
 }
 
+template<class T> inline T __shfl(T x, int src, int = 64)
+{
+    uint64_t bits = 0;
+    std::memcpy(&bits, &x, sizeof(T));
+    const uint64_t r = emu_exchange(bits, (unsigned)src);
+    T y;
+    std::memcpy(&y, &r, sizeof(T));
+    return y;
+}
+
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+
 inline uint64_t __ballot(int pred)
 {
     const unsigned t = threadIdx.x, w = t / 64, lane = t % 64;

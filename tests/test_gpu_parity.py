@@ -50,8 +50,10 @@ def test_propagate_within_1e5_of_fp64(kpax, oracle, rng, k):
 
 def test_collide_matches_brute_force(kpax, oracle, rng):
     cfg = kpax.config_workload(2)
-    # 2,000 small boxes: many cells hold more than 16 obstacles
+    # 2,000 boxes, half packed into a 1 x 1 x 0.5 corner: many cells hold more
+    # than 16 obstacles (the reference's grid keeps only 16, spatialHash.cu:44)
     c = rng.uniform(0, 1, (2000, 3)) * np.array([10, 10, 5])
+    c[:1000] = 4.0 + rng.uniform(0, 1, (1000, 3)) * np.array([1, 1, 0.5])
     e = rng.uniform(0.05, 0.6, (2000, 3))
     obs = np.concatenate([c - e / 2, c + e / 2], 1).astype(np.float32)
     obs[:, :3] = np.maximum(obs[:, :3], 0)
