@@ -67,7 +67,19 @@ def main():
     ap.add_argument("--env-seed", type=int, default=0)
     ap.add_argument("--cpu-plans", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (calibrates the byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch  # device memory plumbing for the calibration copy only
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (each GPU box here has one MI355X); got --gpus {args.gpus}",
               file=sys.stderr)

@@ -109,8 +109,8 @@ typedef struct kpax_result {
 } kpax_result;
 
 /* Per-phase device time, accumulated with HIP events on the handle's stream
- * while profiling is on (kpax_set_profiling).  Phases: 0 expand, 1 score,
- * 2 append, 3 frontier, 4 init. */
+ * while profiling is on (kpax_set_profiling).  Phases: 0 expand, 1 append,
+ * 2 score, 3 frontier, 4 init. */
 #define KPAX_N_PHASES 5
 typedef struct kpax_stats {
     double   phase_ms[KPAX_N_PHASES];

@@ -330,30 +330,24 @@ __device__ __forceinline__ void locate(const RegionParams& R, const float* p, in
     *r2 = *r1 * R.r2_per + (jdx[0] * R.L2[1] + jdx[1]) * R.L2[2] + jdx[2];
 }
 
-// Region counters of one child (KPAX.cu:386-396).  Sub-region novelty is the
-// outcome of the reference's read-then-exchange (KPAX.cu:394-395) when the
-// children run one at a time in slot order: only the smallest slot among the
-// valid children that reach an uncovered R2 cell sees it empty.  Each cell
-// keeps (first iteration << 32 | smallest slot), lowered with atomicMin; a
-// cell stamped with an earlier iteration is settled, so most children decide
-// with one plain read.  Returns kValid | kCandidate; a candidate is confirmed
-// after the launch by comparing the cell's word with its own key.
+// Sub-region novelty (KPAX.cu:394-395, a read-then-exchange race) decided as
+// the reference's kernel decides it when its children run one at a time in
+// slot order: only the smallest slot among the valid children reaching an
+// uncovered R2 cell sees it empty.  Each cell keeps
+// (first iteration << 32 | smallest slot), lowered with a fire-and-forget
+// atomicMin; a cell stamped with an earlier iteration is settled, so most
+// children decide with one plain read.  A candidate wins when, after the
+// launch, the cell's word equals its own key.
 constexpr uint32_t kValid = 1u, kCandidate = 2u, kScoreAdmit = 4u;
 
 __device__ __forceinline__ uint64_t cell_key(uint32_t t, uint32_t slot) { return ((uint64_t)t << 32) | slot; }
 
-__device__ __forceinline__ uint32_t count_child(int32_t* total, int32_t* valid, int32_t* covered,
-                                                uint64_t* r2_state, int r1, int r2, bool ok, uint32_t t,
-                                                uint32_t slot)
+__device__ __forceinline__ uint32_t claim_cell(uint64_t* r2_state, int r2, uint32_t t, uint32_t slot)
 {
-    atomicAdd(&total[r1], 1);
-    if(!ok) return 0u;
-    atomicAdd(&valid[r1], 1);
     const uint64_t seen = __hip_atomic_load(&r2_state[r2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if((uint32_t)(seen >> 32) < t) return kValid;
-    const uint64_t old = atomicMin((unsigned long long*)&r2_state[r2], (unsigned long long)cell_key(t, slot));
-    if(old == ~0ull) atomicAdd(&covered[r1], 1);
-    return kValid | kCandidate;
+    if((uint32_t)(seen >> 32) < t) return 0u;
+    (void)atomicMin((unsigned long long*)&r2_state[r2], (unsigned long long)cell_key(t, slot));
+    return kCandidate;
 }
 
 // ---------------------------------------------------------------------------

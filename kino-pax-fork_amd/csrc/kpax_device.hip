@@ -61,7 +61,8 @@ struct RegionView {
     uint64_t* r2_state;  // per R2 cell: first iteration << 32 | smallest slot, ~0 = never reached
 };
 
-constexpr int kTile = 1024;  // items per scan tile (256 threads x 4)
+constexpr int kTile       = 1024;  // items per scan tile (256 threads x 4)
+constexpr int kLdsRegions = 4096;  // R1 counts up to this size are summed in LDS
 
 // ---------------------------------------------------------------------------
 // Fills (used instead of hipMemsetAsync so every per-plan launch can live in
@@ -156,9 +157,20 @@ __global__ __launch_bounds__(256) void k_expand(Ctl* ctl, TreeView tree, ChildVi
 {
     constexpr int S = Dims<MODEL>::S, C = Dims<MODEL>::C;
     constexpr uint32_t kNone = 0xffffffffu;
+    // Region counters are summed in LDS (total in [0, R1), valid in
+    // [R1, 2 R1)) and added to HBM once per work-group: siblings land in the
+    // same few regions, so per-child HBM atomics would serialise on them.
+    HIP_DYNAMIC_SHARED(int32_t, s_hist)
     const Iter it = ctl->cur;
     if(it.status != kRunning) return;
     const uint32_t M = it.M, N = PP.N, t = it.t;
+    const int R1      = RP.r1_count;
+    const bool local  = R1 <= kLdsRegions;
+    if(local)
+    {
+        for(int r = threadIdx.x; r < 2 * R1; r += blockDim.x) s_hist[r] = 0;
+        __syncthreads();
+    }
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const int lane      = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -222,8 +234,22 @@ __global__ __launch_bounds__(256) void k_expand(Ctl* ctl, TreeView tree, ChildVi
                 steps_done += (uint32_t)i;
                 int r1, r2;
                 locate(RP, x, &r1, &r2);
-                uint32_t flags = count_child(rv.total, rv.valid, rv.covered, rv.r2_state, r1, r2, ok, t, c);
-                if(ok && u01(admit_draw) < rv.score[r1]) flags |= kScoreAdmit;
+                if(local)
+                {
+                    atomicAdd(&s_hist[r1], 1);
+                    if(ok) atomicAdd(&s_hist[R1 + r1], 1);
+                }
+                else
+                {
+                    atomicAdd(&rv.total[r1], 1);
+                    if(ok) atomicAdd(&rv.valid[r1], 1);
+                }
+                uint32_t flags = 0;
+                if(ok)
+                {
+                    flags = kValid | claim_cell(rv.r2_state, r2, t, c);
+                    if(u01(admit_draw) < rv.score[r1]) flags |= kScoreAdmit;
+                }
 #pragma unroll
                 for(int d = 0; d < S; ++d) ch.st[(size_t)d * N + c] = x[d];
 #pragma unroll
@@ -239,6 +265,16 @@ __global__ __launch_bounds__(256) void k_expand(Ctl* ctl, TreeView tree, ChildVi
     }
     const uint32_t ws = wave_sum(steps_done);
     if(lane == 0 && ws) atomicAdd((unsigned long long*)&ctl->steps_done, (unsigned long long)ws);
+    if(local)
+    {
+        __syncthreads();
+        for(int r = threadIdx.x; r < R1; r += blockDim.x)
+        {
+            const int32_t a = s_hist[r], b = s_hist[R1 + r];
+            if(a) atomicAdd(&rv.total[r], a);
+            if(b) atomicAdd(&rv.valid[r], b);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -287,8 +323,8 @@ __global__ __launch_bounds__(1024) void k_score(Ctl* ctl, RegionView rv, RegionP
 // and the goal test (updateFrontier_kernel, KPAX.cu:458-502).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_append(Ctl* ctl, ChildView ch, TreeView tree, const uint64_t* r2_state,
-                                                uint64_t* status, PlanParams PP, int S, int C, float gx, float gy,
-                                                float gz)
+                                                int32_t* covered, uint64_t* status, PlanParams PP, int S, int C,
+                                                float gx, float gy, float gz)
 {
     __shared__ uint32_t lds[4];
     __shared__ uint32_t s_tile, s_pre;
@@ -306,11 +342,16 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, ChildView ch, TreeView
         for(int j = 0; j < 4; ++j)
         {
             const uint32_t c = base + j;
-            uint32_t a       = c < M ? ch.adm[c] : 0u;
-            // admitted: valid and (score draw passed, or first to reach a new R2 cell)
-            if((a & kValid) && !(a & kScoreAdmit) && (a & kCandidate))
-                a = r2_state[ch.r2[c]] == cell_key(it.t, c) ? kValid | kScoreAdmit : 0u;
-            f[j] = (a & kValid) && (a & kScoreAdmit) ? 1u : 0u;
+            const uint32_t a = c < M ? ch.adm[c] : 0u;
+            // a candidate that holds its cell's key was the first to reach it:
+            // it covers the cell and is admitted (KPAX.cu:394-395)
+            bool first = false;
+            if(a & kCandidate)
+            {
+                first = r2_state[ch.r2[c]] == cell_key(it.t, c);
+                if(first) atomicAdd(&covered[ch.reg[c]], 1);
+            }
+            f[j] = (a & kValid) && ((a & kScoreAdmit) || first) ? 1u : 0u;
             cnt += f[j];
         }
         uint32_t agg;
@@ -491,7 +532,24 @@ __global__ __launch_bounds__(256) void k_histogram(RegionParams RP, const float*
     const float p[3] = {st[i], st[(size_t)n + i], st[2 * (size_t)n + i]};
     int r1, r2;
     locate(RP, p, &r1, &r2);
-    count_child(rv.total, rv.valid, rv.covered, rv.r2_state, r1, r2, valid[i] != 0, t, i);
+    atomicAdd(&rv.total[r1], 1);
+    if(valid[i])
+    {
+        atomicAdd(&rv.valid[r1], 1);
+        claim_cell(rv.r2_state, r2, t, i);
+    }
+}
+
+// Coverage after k_histogram: the winner of each newly reached cell counts it.
+__global__ __launch_bounds__(256) void k_histogram_cover(RegionParams RP, const float* st, const uint8_t* valid,
+                                                         uint32_t n, uint32_t t, RegionView rv)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n || !valid[i]) return;
+    const float p[3] = {st[i], st[(size_t)n + i], st[2 * (size_t)n + i]};
+    int r1, r2;
+    locate(RP, p, &r1, &r2);
+    if(rv.r2_state[r2] == cell_key(t, i)) atomicAdd(&rv.covered[r1], 1);
 }
 
 // Run-length expansion of weights (repeatInd after findInd).
@@ -632,6 +690,7 @@ struct kpax_handle {
     PlanParams PP{};
     std::string err;
     int cu_count          = 256;
+    uint32_t expand_blocks = 1;
     hipStream_t stream    = nullptr;
     // tree and children
     TreeView tree{};
@@ -724,22 +783,50 @@ struct Scratch {
     }
 };
 
+size_t expand_lds_bytes(const kpax_handle* h)
+{
+    return h->RP.r1_count <= kLdsRegions ? 2 * (size_t)h->RP.r1_count * sizeof(int32_t) : 0;
+}
+
+// Work-groups of k_expand that fit on the device at once: its waves are
+// persistent, so more would only wait for the first ones to drain the queue.
+template<int MODEL>
+uint32_t expand_blocks_for(const kpax_handle* h)
+{
+    int per_cu = 0;
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand<MODEL>, 256, expand_lds_bytes(h)) != hipSuccess ||
+       per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)per_cu * (uint32_t)h->cu_count;
+}
+
+uint32_t expand_blocks(const kpax_handle* h)
+{
+    switch(h->cfg.model)
+    {
+        case KPAX_MODEL_DOUBLE_INTEGRATOR: return expand_blocks_for<KPAX_MODEL_DOUBLE_INTEGRATOR>(h);
+        case KPAX_MODEL_DUBINS_AIRPLANE: return expand_blocks_for<KPAX_MODEL_DUBINS_AIRPLANE>(h);
+        default: return expand_blocks_for<KPAX_MODEL_QUADCOPTER>(h);
+    }
+}
+
 int launch_expand(kpax_handle* h, uint64_t seed, uint32_t query, uint32_t blocks)
 {
-    const GridView G = grid_view(h);
+    const GridView G   = grid_view(h);
+    const size_t lds   = expand_lds_bytes(h);
     switch(h->cfg.model)
     {
         case KPAX_MODEL_DOUBLE_INTEGRATOR:
-            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DOUBLE_INTEGRATOR>, dim3(blocks), dim3(256), 0, h->stream, 
-              h->ctl, h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
+            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DOUBLE_INTEGRATOR>, dim3(blocks), dim3(256), lds, h->stream,
+                               h->ctl, h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
             break;
         case KPAX_MODEL_DUBINS_AIRPLANE:
-            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DUBINS_AIRPLANE>, dim3(blocks), dim3(256), 0, h->stream, 
-              h->ctl, h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
+            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DUBINS_AIRPLANE>, dim3(blocks), dim3(256), lds, h->stream,
+                               h->ctl, h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
             break;
         default:
-            hipLaunchKernelGGL(k_expand<KPAX_MODEL_QUADCOPTER>, dim3(blocks), dim3(256), 0, h->stream, h->ctl, h->tree, h->ch, h->R, h->rv,
-                                                                           h->P, h->RP, G, h->PP, seed, query);
+            hipLaunchKernelGGL(k_expand<KPAX_MODEL_QUADCOPTER>, dim3(blocks), dim3(256), lds, h->stream, h->ctl,
+                               h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
             break;
     }
     return KPAX_OK;
@@ -810,6 +897,7 @@ int kpax_create(const kpax_config* cfg, kpax_handle** out)
     if(hipGetDeviceProperties(&prop, cfg->device) == hipSuccess && prop.multiProcessorCount > 0)
         h->cu_count = prop.multiProcessorCount;
     if(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    h->expand_blocks = expand_blocks(h);
     const size_t N = cfg->max_tree;
     TRY(dalloc(h, &h->tree.st, N * h->S));
     TRY(dalloc(h, &h->tree.ct, N * h->C));
@@ -963,7 +1051,7 @@ int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t se
     hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, h->stream, h->ctl, h->tree, h->R, h->d_start, h->S, h->C, h->RP, h->PP);
     rec(h, 1);
     HIPC(hipGetLastError());
-    const uint32_t eb = std::min<uint32_t>(blocks_for(N, 256), 8 * h->cu_count);
+    const uint32_t eb = std::min<uint32_t>(blocks_for(N, 256), h->expand_blocks);
     const uint32_t sb = std::min<uint32_t>(blocks_for(N, kTile), 4 * h->cu_count);
     uint32_t launched = 0;
     for(uint32_t it = 1; it <= h->cfg.max_iter; ++it)
@@ -973,10 +1061,10 @@ int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t se
         rec(h, e + 0);
         launch_expand(h, seed, 0, eb);
         rec(h, e + 1);
-        hipLaunchKernelGGL(k_score, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps, 1);
+        hipLaunchKernelGGL(k_append, dim3(sb), dim3(256), 0, h->stream, h->ctl, h->ch, h->tree, h->rv.r2_state,
+                           h->rv.covered, h->st_append, h->PP, h->S, h->C, goal[0], goal[1], goal[2]);
         rec(h, e + 2);
-        hipLaunchKernelGGL(k_append, dim3(sb), dim3(256), 0, h->stream, h->ctl, h->ch, h->tree, h->rv.r2_state, h->st_append, h->PP, h->S, h->C,
-                                            goal[0], goal[1], goal[2]);
+        hipLaunchKernelGGL(k_score, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps, 1);
         rec(h, e + 3);
         hipLaunchKernelGGL(k_frontier, dim3(sb), dim3(256), 0, h->stream, h->ctl, h->tree, h->R, h->rv, h->st_frontier, h->PP, seed, 0);
         rec(h, e + 4);
@@ -1276,7 +1364,12 @@ int kpax_region_histogram(kpax_handle* h, const float* states_soa, const uint8_t
     HIPC(hipMemcpy(rv.covered, covered, r1 * 4, hipMemcpyHostToDevice));
     // the batch is one iteration on an empty coverage map
     hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(r2, 256)), dim3(256), 0, h->stream, rv.r2_state, ~0ull, r2);
-    if(n) hipLaunchKernelGGL(k_histogram, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->RP, st, v, n, 1u, rv);
+    if(n)
+    {
+        hipLaunchKernelGGL(k_histogram, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->RP, st, v, n, 1u, rv);
+        hipLaunchKernelGGL(k_histogram_cover, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->RP, st, v, n, 1u,
+                           rv);
+    }
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     HIPC(hipMemcpy(total, rv.total, r1 * 4, hipMemcpyDeviceToHost));

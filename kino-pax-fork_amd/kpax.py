@@ -24,7 +24,7 @@ STATUS = {0: "OK", 1: "NOT_FOUND", 2: "BUDGET_EXHAUSTED", -1: "INVALID_ARG", -2:
 OK, NOT_FOUND, BUDGET_EXHAUSTED = 0, 1, 2
 MODEL_DOUBLE_INTEGRATOR, MODEL_DUBINS_AIRPLANE, MODEL_QUADCOPTER = 1, 2, 3
 N_PHASES = 5
-PHASES = ("expand", "score", "append", "frontier", "init")
+PHASES = ("expand", "append", "score", "frontier", "init")
 
 
 class Config(ctypes.Structure):

@@ -28,6 +28,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+inline char* emu_dyn_shared = nullptr;  // dynamic LDS of the running work-group
+#define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(emu_dyn_shared);
 
 struct dim3 {
     unsigned x, y, z;
@@ -268,9 +270,17 @@ inline emu_pool& emu_workers()
     return *p;
 }
 
-template<class... KArgs, class... Args>
-void emu_launch(void (*k)(KArgs...), dim3 grid, dim3 block, Args... args)
+template<class F> inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, F, int, size_t)
 {
+    *n = 1;
+    return hipSuccess;
+}
+
+template<class... KArgs, class... Args>
+void emu_launch(void (*k)(KArgs...), dim3 grid, dim3 block, size_t shmem, Args... args)
+{
+    std::vector<char> dyn(shmem ? shmem : 1);
+    emu_dyn_shared = dyn.data();
     gridDim  = {grid.x, 1, 1};
     blockDim = {block.x, 1, 1};
     const unsigned nw = (block.x + 63) / 64;
@@ -294,4 +304,4 @@ void emu_launch(void (*k)(KArgs...), dim3 grid, dim3 block, Args... args)
     }
 }
 
-#define hipLaunchKernelGGL(k, grid, block, shmem, stream, ...) emu_launch(k, grid, block, __VA_ARGS__)
+#define hipLaunchKernelGGL(k, grid, block, shmem, stream, ...) emu_launch(k, grid, block, (size_t)(shmem), __VA_ARGS__)
