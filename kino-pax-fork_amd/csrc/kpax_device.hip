@@ -150,11 +150,37 @@ __global__ __launch_bounds__(256) void k_advance(Ctl* ctl, uint64_t* st_append, 
 // stop early, so refilling keeps the lanes of a wave busy instead of waiting
 // for the longest edge.  Each child's arithmetic is that of propagate().
 // ---------------------------------------------------------------------------
+// Everything k_expand reads that is fixed for a handle lives in one
+// device-side block behind a single pointer: as by-value kernel arguments the
+// structures exceed the scalar register budget and get spilled into vector
+// lanes, costing a readlane at every use inside the step loop.
+struct ExpandArgs {
+    Ctl* ctl;
+    TreeView tree;
+    ChildView ch;
+    const uint32_t* R;
+    RegionView rv;
+    ModelParams P;
+    RegionParams RP;
+    GridView G;
+    PlanParams PP;
+};
+
+// Five resident waves per SIMD (96 VGPRs for the quadcopter) hide more of
+// the grid lookups' latency than four.
 template<int MODEL>
-__global__ __launch_bounds__(256) void k_expand(Ctl* ctl, TreeView tree, ChildView ch, const uint32_t* R,
-                                                RegionView rv, ModelParams P, RegionParams RP, GridView G,
-                                                PlanParams PP, uint64_t seed, uint32_t query)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_expand(
+  const ExpandArgs* __restrict__ A, uint64_t seed, uint32_t query)
 {
+    Ctl* ctl              = A->ctl;
+    const TreeView& tree  = A->tree;
+    const ChildView& ch   = A->ch;
+    const uint32_t* R     = A->R;
+    const RegionView& rv  = A->rv;
+    const ModelParams& P  = A->P;
+    const RegionParams& RP = A->RP;
+    const GridView& G     = A->G;
+    const PlanParams& PP  = A->PP;
     constexpr int S = Dims<MODEL>::S, C = Dims<MODEL>::C;
     constexpr uint32_t kNone = 0xffffffffu;
     // Region counters are summed in LDS (total in [0, R1), valid in
@@ -691,6 +717,7 @@ struct kpax_handle {
     std::string err;
     int cu_count          = 256;
     uint32_t expand_blocks = 1;
+    void* expand_args      = nullptr;  // device copy of ExpandArgs
     hipStream_t stream    = nullptr;
     // tree and children
     TreeView tree{};
@@ -810,23 +837,41 @@ uint32_t expand_blocks(const kpax_handle* h)
     }
 }
 
+// Writes the device-side argument block of k_expand (after creation and
+// whenever the collision grid moves).
+int upload_expand_args(kpax_handle* h)
+{
+    ExpandArgs a;
+    a.ctl  = h->ctl;
+    a.tree = h->tree;
+    a.ch   = h->ch;
+    a.R    = h->R;
+    a.rv   = h->rv;
+    a.P    = h->P;
+    a.RP   = h->RP;
+    a.G    = grid_view(h);
+    a.PP   = h->PP;
+    HIPC(hipMemcpy(h->expand_args, &a, sizeof(a), hipMemcpyHostToDevice));
+    return KPAX_OK;
+}
+
 int launch_expand(kpax_handle* h, uint64_t seed, uint32_t query, uint32_t blocks)
 {
-    const GridView G   = grid_view(h);
-    const size_t lds   = expand_lds_bytes(h);
+    const size_t lds = expand_lds_bytes(h);
+    const ExpandArgs* A = (const ExpandArgs*)h->expand_args;
     switch(h->cfg.model)
     {
         case KPAX_MODEL_DOUBLE_INTEGRATOR:
-            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DOUBLE_INTEGRATOR>, dim3(blocks), dim3(256), lds, h->stream,
-                               h->ctl, h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
+            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DOUBLE_INTEGRATOR>, dim3(blocks), dim3(256), lds, h->stream, A, seed,
+                               query);
             break;
         case KPAX_MODEL_DUBINS_AIRPLANE:
-            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DUBINS_AIRPLANE>, dim3(blocks), dim3(256), lds, h->stream,
-                               h->ctl, h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
+            hipLaunchKernelGGL(k_expand<KPAX_MODEL_DUBINS_AIRPLANE>, dim3(blocks), dim3(256), lds, h->stream, A, seed,
+                               query);
             break;
         default:
-            hipLaunchKernelGGL(k_expand<KPAX_MODEL_QUADCOPTER>, dim3(blocks), dim3(256), lds, h->stream, h->ctl,
-                               h->tree, h->ch, h->R, h->rv, h->P, h->RP, G, h->PP, seed, query);
+            hipLaunchKernelGGL(k_expand<KPAX_MODEL_QUADCOPTER>, dim3(blocks), dim3(256), lds, h->stream, A, seed,
+                               query);
             break;
     }
     return KPAX_OK;
@@ -938,6 +983,8 @@ int kpax_create(const kpax_config* cfg, kpax_handle** out)
         TRY(KPAX_DEVICE_ERROR);
     if(hipMemset(h->st_append, 0, h->n_status * sizeof(uint64_t)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
     if(hipMemset(h->st_frontier, 0, h->n_status * sizeof(uint64_t)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    TRY(dalloc(h, (char**)&h->expand_args, sizeof(ExpandArgs)));
+    TRY(upload_expand_args(h));
 #undef TRY
     *out = h;
     return KPAX_OK;
@@ -1023,7 +1070,7 @@ int kpax_set_obstacles(kpax_handle* h, const float* aabb6, uint32_t n)
     hipLaunchKernelGGL(k_grid_sort, dim3(blocks_for(G, 256)), dim3(256), 0, h->stream, h->grid_start, h->grid_items, G);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
-    return KPAX_OK;
+    return upload_expand_args(h);
 }
 
 int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t seed, kpax_result* out)

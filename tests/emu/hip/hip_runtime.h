@@ -27,6 +27,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define amdgpu_waves_per_eu(...) unused
 #define __shared__ static
 inline char* emu_dyn_shared = nullptr;  // dynamic LDS of the running work-group
 #define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(emu_dyn_shared);
