@@ -117,7 +117,9 @@ typedef struct kpax_stats {
     uint64_t phase_launches[KPAX_N_PHASES];
     uint64_t expansions;       /* children propagated by the timed expand launches       */
     uint64_t integrated_steps; /* integration steps taken by those children               */
-    uint64_t plans;
+    uint64_t plans;            /* queries finished (single or batched)                    */
+    uint64_t iterations;       /* planner iterations launched                             */
+    uint64_t graph_launches;   /* iterations launched as one captured HIP graph           */
 } kpax_stats;
 
 /* ---- configuration ---------------------------------------------------- */
@@ -141,6 +143,15 @@ typedef struct kpax_handle kpax_handle;
  * configured capacity on cfg->device. */
 int kpax_create(const kpax_config* cfg, kpax_handle** out);
 
+/* Batched form (no counterpart in the reference, which plans one query per
+ * process: Planner.cuh:19-27).  A handle with `max_queries` query slots and
+ * `node_capacity` tree nodes shared by all slots: each admitted query takes a
+ * contiguous range of `budget` nodes (its tree cap) and one slot, and every
+ * planner iteration advances all admitted queries together.  kpax_create is
+ * kpax_create_batch(cfg, 1, cfg->max_tree).  max_queries <= 1024. */
+int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t node_capacity,
+                      kpax_handle** out);
+
 /* Replaces KPAX::~KPAX() (KPAX.cu:46-49); frees everything. */
 int kpax_destroy(kpax_handle* h);
 
@@ -154,6 +165,11 @@ const char* kpax_last_error(const kpax_handle* h);
  * which the reference re-runs every iteration (KPAX.cu:306). */
 int kpax_set_obstacles(kpax_handle* h, const float* aabb6, uint32_t n);
 
+/* Obstacles of environment `env` (< 64) for batched queries; environment 0 is
+ * the one kpax_set_obstacles sets.  Every environment shares the configured
+ * workspace and grid dimensions. */
+int kpax_set_environment(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n);
+
 /* Plan from start (S floats) to the goal ball goal[0..2] with the configured
  * radius.  Replaces KPAX::plan(h_initial, h_goal, d_obstacles, n, saveTree)
  * (include/planners/KPAX.cuh:14, src/planners/KPAX.cu:51-115), with an
@@ -162,7 +178,27 @@ int kpax_set_obstacles(kpax_handle* h, const float* aabb6, uint32_t n);
 int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t seed,
               kpax_result* out);
 
-/* Free the arrays of a result filled by kpax_plan. */
+/* One query of a batch. */
+typedef struct kpax_query {
+    const float* start;        /* S floats                                                */
+    const float* goal;         /* goal ball centre, 3 floats (radius from the config)     */
+    uint64_t     seed;         /* a query planned alone with this seed gives the same tree */
+    uint32_t     budget;       /* tree cap of this query in nodes; 0 = cfg.max_tree       */
+    uint32_t     env;          /* environment index (kpax_set_environment)                */
+} kpax_query;
+
+/* Plan n queries on a batched handle.  Queries are admitted in order while a
+ * slot and a contiguous node range of their budget are free (first fit);
+ * the rest wait until running queries finish and release theirs.  out[i]
+ * receives query i's result exactly as kpax_plan would return it for the same
+ * start, goal, seed, environment and a tree cap of its budget; plan_ms is the
+ * time from the call to the query's completion (queueing included).  A query
+ * that can never run (budget above the node capacity, unknown environment,
+ * non-finite start) gets status KPAX_INVALID_ARG and the others still run.
+ * Every out[i] must be released with kpax_result_free. */
+int kpax_plan_batch(kpax_handle* h, const kpax_query* queries, uint32_t n, kpax_result* out);
+
+/* Free the arrays of a result filled by kpax_plan or kpax_plan_batch. */
 void kpax_result_free(kpax_result* r);
 
 /* Copy the tree of the last plan: tree_size nodes, SoA states (S x cap),
@@ -184,6 +220,31 @@ int kpax_get_regions(kpax_handle* h, int32_t* valid, int32_t* total, int32_t* co
 int kpax_set_profiling(kpax_handle* h, int32_t on);
 int kpax_get_stats(kpax_handle* h, kpax_stats* out);
 int kpax_reset_stats(kpax_handle* h);
+
+/* ---- launch shape and its autotuner (no counterpart in the reference, whose
+ * launch sizes are compile-time constants: KPAX.cu:20-25, config.h:206) ---- */
+
+typedef struct kpax_launch {
+    uint32_t expand_blocks;    /* work-groups of the expansion kernel (its waves are persistent) */
+    uint32_t expand_chunk;     /* children a wave takes per work ticket (multiple of 64, <= 4096) */
+    uint32_t scan_blocks;      /* work-groups of the append and frontier scans                   */
+    int32_t  use_graph;        /* 1: replay each iteration as one captured HIP graph             */
+} kpax_launch;
+
+/* Current launch shape of the handle. */
+int kpax_get_launch(kpax_handle* h, kpax_launch* out);
+
+/* Set the launch shape.  Results do not depend on it (every plan is the same
+ * bit for bit); only the time does. */
+int kpax_set_launch(kpax_handle* h, const kpax_launch* launch);
+
+/* Time the n sample queries (as kpax_plan_batch) under candidate launch
+ * shapes, one knob at a time, keeping the fastest of `repeats` runs of each;
+ * the best shape is applied to the handle and written to *best.  Every
+ * candidate must reproduce the first run's results, or the call fails with
+ * KPAX_DEVICE_ERROR. */
+int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_t repeats,
+                  kpax_launch* best);
 
 /* ---- hot-path operators (parity entry points, host buffers) ------------- */
 
@@ -247,7 +308,8 @@ int kpax_read_obstacles_csv(const char* path, float* out, uint32_t cap, uint32_t
  *   1  random boxes: n AABBs, centres U[ws], edges U[0.05,0.15], clipped to the
  *      workspace, rejected if they touch the start point or the goal ball
  *   2  forest: n vertical columns of footprint U[0.1,0.3] over the full height
- *   3  maze: a dividing wall of tiles with one 0.1-wide passage plus slabs
+ *   3  maze: a dividing wall of tiles with one passage a tenth of the workspace
+ *      wide (0.1 in the unit cube) plus slabs; lengths scale with the workspace
  * `start` and `goal` (xyz + radius) are kept clear.  Call with out = NULL to
  * get the count. */
 int kpax_generate_environment(int32_t kind, uint64_t seed, uint32_t n, const float* ws_min,

@@ -353,12 +353,16 @@ __device__ __forceinline__ uint32_t claim_cell(uint64_t* r2_state, int r2, uint3
 // ---------------------------------------------------------------------------
 // Scans.  256-thread blocks (4 waves of 64) scan 1024-item tiles; tiles are
 // handed out in order by an atomic ticket and chained with a decoupled
-// look-back whose status words carry the aggregate (flag 1) or inclusive
-// prefix (flag 2) in their low 32 bits.  A tile only waits on tiles with
-// smaller tickets, which are already resident, so every wait ends.
+// look-back.  A status word holds the flag in bits 62-63 (1 = aggregate,
+// 2 = inclusive prefix), the launch's epoch in bits 32-61 and the value in
+// bits 0-31; a word whose epoch differs from the launch's is treated as not
+// yet written, so the status arrays never need clearing between launches.  A
+// tile only waits on tiles with smaller tickets, which are already resident,
+// so every wait ends.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kFlagAgg    = 1ull << 62;
 constexpr uint64_t kFlagPrefix = 2ull << 62;
+constexpr uint32_t kEpochMask  = (1u << 30) - 1u;
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
 {
@@ -379,21 +383,31 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
     return x;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block; *agg gets
-// the block total.  lds must hold 4 words; contains two barriers.
-__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* agg, uint32_t* lds)
+// Exclusive scan of one value per thread over a block of NW waves; *agg gets
+// the block total.  lds must hold NW words; contains two barriers.
+template<int NW>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* agg, uint32_t* lds)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t inc = wave_incl_scan(v);
     if(lane == 63) lds[wid] = inc;
     __syncthreads();
-    uint32_t before = 0;
+    uint32_t before = 0, total = 0;
 #pragma unroll
-    for(int w = 0; w < 4; ++w)
-        if(w < wid) before += lds[w];
-    *agg = lds[0] + lds[1] + lds[2] + lds[3];
+    for(int w = 0; w < NW; ++w)
+    {
+        const uint32_t s = lds[w];
+        if(w < wid) before += s;
+        total += s;
+    }
+    *agg = total;
     __syncthreads();
     return before + inc - v;
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* agg, uint32_t* lds)
+{
+    return block_excl_scan<4>(v, agg, lds);
 }
 
 __device__ __forceinline__ void status_store(uint64_t* p, uint64_t v)
@@ -405,26 +419,36 @@ __device__ __forceinline__ uint64_t status_load(const uint64_t* p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Called by all 64 lanes of wave 0; returns the exclusive prefix of `tile`.
-__device__ __forceinline__ uint32_t lookback(uint64_t* status, uint32_t tile, uint32_t agg)
+__device__ __forceinline__ uint64_t status_word(uint64_t flag, uint32_t epoch, uint32_t value)
+{
+    return flag | ((uint64_t)(epoch & kEpochMask) << 32) | value;
+}
+
+// Called by all 64 lanes of wave 0; returns the exclusive prefix of `tile`
+// within its segment, whose first tile is `first` (the tiles of one query
+// form a segment; tiles before `first` belong to other queries and are not
+// read).
+__device__ __forceinline__ uint32_t lookback(uint64_t* status, uint32_t tile, uint32_t agg, uint32_t first,
+                                             uint32_t epoch)
 {
     const int lane = threadIdx.x & 63;
-    if(tile == 0)
+    if(tile == first)
     {
-        if(lane == 0) status_store(&status[0], kFlagPrefix | agg);
+        if(lane == 0) status_store(&status[tile], status_word(kFlagPrefix, epoch, agg));
         return 0;
     }
-    if(lane == 0) status_store(&status[tile], kFlagAgg | agg);
-    uint32_t excl = 0;
-    int64_t base  = (int64_t)tile - 1;
+    if(lane == 0) status_store(&status[tile], status_word(kFlagAgg, epoch, agg));
+    const uint32_t ep = epoch & kEpochMask;
+    uint32_t excl     = 0;
+    int64_t base      = (int64_t)tile - 1;
     while(true)
     {
         const int64_t j = base - lane;
-        uint64_t s      = kFlagPrefix;  // before tile 0: an empty prefix
-        if(j >= 0)
+        uint64_t s      = kFlagPrefix;  // before the segment: an empty prefix
+        if(j >= (int64_t)first)
         {
             s = status_load(&status[j]);
-            while((s >> 62) == 0) s = status_load(&status[j]);
+            while((s >> 62) == 0 || (uint32_t)((s >> 32) & kEpochMask) != ep) s = status_load(&status[j]);
         }
         const uint64_t pm = __ballot((s >> 62) == 2);
         const int stop    = pm ? __builtin_ctzll(pm) : 64;
@@ -432,16 +456,17 @@ __device__ __forceinline__ uint32_t lookback(uint64_t* status, uint32_t tile, ui
         if(pm) break;
         base -= 64;
     }
-    if(lane == 0) status_store(&status[tile], kFlagPrefix | (excl + agg));
+    if(lane == 0) status_store(&status[tile], status_word(kFlagPrefix, epoch, excl + agg));
     return excl;
 }
 
 // Block-wide: exclusive prefix of this tile given its aggregate.
-__device__ __forceinline__ uint32_t tile_prefix(uint64_t* status, uint32_t tile, uint32_t agg, uint32_t* lds_word)
+__device__ __forceinline__ uint32_t tile_prefix(uint64_t* status, uint32_t tile, uint32_t agg, uint32_t first,
+                                                uint32_t epoch, uint32_t* lds_word)
 {
     if(threadIdx.x < 64)
     {
-        const uint32_t e = lookback(status, tile, agg);
+        const uint32_t e = lookback(status, tile, agg, first, epoch);
         if(threadIdx.x == 0) *lds_word = e;
     }
     __syncthreads();
@@ -461,24 +486,24 @@ __device__ __forceinline__ uint32_t next_tile(uint32_t* counter, uint32_t* lds_w
 }
 
 // fAccept of updateFrontier (KPAX.cu:527-528): float iteration * eps times the
-// double fifth power of the remaining tree fraction.
-__device__ __forceinline__ float f_accept(uint32_t t, uint32_t tree_plus_new, const PlanParams& PP)
+// double fifth power of the remaining tree fraction (cap = the query's tree cap).
+__device__ __forceinline__ float f_accept(uint32_t t, uint32_t tree_plus_new, uint32_t cap, float eps)
 {
-    const float tas = 1.0f - ((float)tree_plus_new / (float)PP.N);
+    const float tas = 1.0f - ((float)tree_plus_new / (float)cap);
     const double x  = (double)tas;
     const double p5 = x * x * x * x * x;
-    return (float)((double)((float)t * PP.eps) * p5);
+    return (float)((double)((float)t * eps) * p5);
 }
 
 // Fan-out of the next iteration (KPAX.cu:323-333), capped so the children
 // never exceed the remaining tree capacity.
-__device__ __forceinline__ Iter make_iter(uint32_t t, uint32_t ts, uint32_t W, const PlanParams& PP)
+__device__ __forceinline__ Iter make_iter(uint32_t t, uint32_t ts, uint32_t W, uint32_t cap, const PlanParams& PP)
 {
     Iter n;
     n.t                      = t;
     n.ts                     = ts;
     n.W                      = W;
-    const uint32_t remaining = ts < PP.N ? PP.N - ts : 0u;
+    const uint32_t remaining = ts < cap ? cap - ts : 0u;
     uint32_t fan             = PP.fanout;
     uint64_t M               = (uint64_t)W * fan;
     if(M > remaining)
@@ -493,6 +518,22 @@ __device__ __forceinline__ Iter make_iter(uint32_t t, uint32_t ts, uint32_t W, c
     n.fan    = fan;
     n.status = t > PP.max_iter ? kDoneNotFound : (remaining == 0 ? kDoneBudget : kRunning);
     return n;
+}
+
+// First index i in [0, n) with off[i + 1] > k (off holds n + 1 ascending
+// offsets): the segment that owns item k.
+__device__ __forceinline__ uint32_t segment_of(const uint32_t* off, uint32_t n, uint32_t k)
+{
+    uint32_t lo = 0, hi = n;
+    while(hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if(off[mid] <= k)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
 }  // namespace dev

@@ -1,0 +1,198 @@
+// kpax_handle.h — the planner handle and the helpers shared by the planner
+// (kpax_planner.hip) and the hot-path operators (kpax_ops.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kpax_device.cuh"
+#include "kpax_internal.h"
+
+namespace kpax {
+
+// Node arrays are SoA with column stride T (the handle's node capacity);
+// query slot q owns columns [base_q, base_q + cap_q).
+struct TreeView {
+    float* st;
+    float* ct;
+    int32_t* steps;
+    int32_t* parent;
+    int32_t* reg;
+};
+
+struct ChildView {
+    float* st;
+    float* ct;
+    int32_t* steps;
+    int32_t* parent;
+    int32_t* reg;
+    int32_t* r2;
+    uint8_t* adm;  // kValid | kCandidate | kScoreAdmit
+};
+
+// Region state of slot q: counters and scores at q * R1, sub-region cells at
+// q * R1 * R2_per_R1.
+struct RegionView {
+    int32_t* valid;
+    int32_t* total;
+    int32_t* covered;
+    float* score;
+    uint64_t* r2_state;  // first iteration << 32 | smallest slot, ~0 = never reached
+};
+
+// One environment: obstacles and their CSR grid (device), plus the host copy
+// the FP64 re-verifier reads.
+struct Env {
+    int32_t* start  = nullptr;
+    int32_t* items  = nullptr;
+    float4* lo      = nullptr;
+    float4* hi      = nullptr;
+    uint32_t n_obs  = 0;
+    uint32_t k      = 0;
+    std::vector<float> host;
+};
+
+constexpr int kTile       = 1024;  // items per scan tile (256 threads x 4)
+constexpr int kLdsRegions = 4096;  // R1 counts up to this size are summed in LDS
+constexpr int kMaxEnvs    = 64;    // environments of one handle (LDS table in batched expansion)
+
+inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)std::max<uint64_t>(1, (n + bs - 1) / bs); }
+
+
+}  // namespace kpax
+
+struct kpax_handle {
+    kpax_config cfg{};
+    int S = 0, C = 0;
+    kpax::ModelParams P{};
+    kpax::RegionParams RP{};
+    kpax::GridParams GPm{};
+    kpax::PlanParams PP{};
+    std::string err;
+    int cu_count       = 256;
+    hipStream_t stream = nullptr;
+    // capacity
+    uint32_t T    = 0;  // tree nodes over all slots (column stride of the node arrays)
+    uint32_t Qmax = 1;  // query slots
+    uint32_t G    = 0;  // grid cells
+    uint32_t R2   = 0;  // R2 cells per slot
+    // planner state (device)
+    kpax::TreeView tree{};
+    kpax::ChildView ch{};
+    uint32_t* R = nullptr;
+    kpax::RegionView rv{};
+    kpax::Ctl* ctl          = nullptr;  // Qmax records
+    kpax::QueryDev* qd      = nullptr;  // Qmax records
+    kpax::BatchCtl* bc      = nullptr;
+    uint32_t* child_off     = nullptr;  // Qmax + 1 offsets, written by the advance kernel
+    uint32_t* atile_off     = nullptr;
+    uint32_t* ftile_off     = nullptr;
+    int32_t* slot_status    = nullptr;  // Qmax statuses + the running count
+    uint64_t* st_append     = nullptr;
+    uint64_t* st_frontier   = nullptr;
+    uint32_t n_status       = 0;
+    uint32_t* d_init_slots  = nullptr;  // slots being initialised
+    float* d_init_starts    = nullptr;  // 16 floats per slot being initialised
+    kpax::dev::GridView* envs_dev = nullptr;  // kMaxEnvs grids
+    void* expand_args       = nullptr;
+    uint32_t expand_blocks  = 1;
+    uint32_t expand_chunk   = 64;
+    uint32_t scan_blocks    = 1;
+    uint32_t path_cap       = 0;        // rows per slot
+    float* path_rows        = nullptr;  // Qmax x path_cap x (S + C + 1), goal -> root
+    int32_t* path_steps     = nullptr;
+    int32_t* path_idx       = nullptr;
+    uint32_t* d_path_slots  = nullptr;
+    // pinned host mirrors
+    int32_t* h_status       = nullptr;  // 2 x (Qmax + 1): statuses after iterations of each parity
+    kpax::Ctl* h_ctl        = nullptr;
+    kpax::QueryDev* h_qd    = nullptr;
+    uint32_t* h_init_slots  = nullptr;
+    float* h_init_starts    = nullptr;
+    uint32_t* h_path_slots  = nullptr;
+    float* h_path_rows      = nullptr;
+    int32_t* h_path_steps   = nullptr;
+    int32_t* h_path_idx     = nullptr;
+    hipEvent_t it_ev[2]     = {nullptr, nullptr};
+    hipEvent_t ph_ev[2][6]  = {};       // phase boundaries of iterations of each parity (profiling)
+    hipEvent_t init_ev[2]   = {nullptr, nullptr};
+    // one captured iteration per status-buffer parity, replayed while profiling is off
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    bool graph_tried        = false;
+    bool use_graph          = true;
+    // environments
+    std::vector<kpax::Env> envs;
+    // last single-query plan (for kpax_get_tree / kpax_get_regions)
+    uint32_t last_tree = 0;
+    bool planned       = false;
+    // profiling; KPAX_TRACE=1 prints every iteration record to stderr
+    bool profiling = false;
+    bool trace     = false;
+    kpax_stats stats{};
+    std::vector<void*> allocs;
+};
+
+namespace kpax {
+
+inline int fail(kpax_handle* h, int code, const std::string& msg)
+{
+    if(h) h->err = msg;
+    return code;
+}
+
+#define HIPC(expr)                                                                                       \
+    do                                                                                                   \
+    {                                                                                                    \
+        hipError_t e_ = (expr);                                                                          \
+        if(e_ != hipSuccess)                                                                             \
+            return ::kpax::fail(h, KPAX_DEVICE_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while(0)
+
+template<class T>
+int dalloc(kpax_handle* h, T** p, size_t count)
+{
+    void* q      = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if(e != hipSuccess) return fail(h, KPAX_NO_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    h->allocs.push_back(q);
+    *p = (T*)q;
+    return KPAX_OK;
+}
+
+inline dev::GridView grid_view(const kpax_handle* h, uint32_t env)
+{
+    dev::GridView g;
+    g.gp    = h->GPm;
+    g.start = h->envs[env].start;
+    g.items = h->envs[env].items;
+    g.lo    = h->envs[env].lo;
+    g.hi    = h->envs[env].hi;
+    return g;
+}
+
+// Scratch device buffers for the parity operators (allocated per call).
+struct Scratch {
+    std::vector<void*> ptrs;
+    ~Scratch()
+    {
+        for(void* p : ptrs) (void)hipFree(p);
+    }
+    template<class T> T* get(size_t count)
+    {
+        void* q = nullptr;
+        if(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(q);
+        return (T*)q;
+    }
+};
+
+// Builds environment `env`'s CSR grid on the device (kpax_ops.hip).
+int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n);
+void free_env(Env& e);
+
+}  // namespace kpax

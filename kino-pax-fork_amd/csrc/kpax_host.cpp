@@ -127,7 +127,6 @@ GridParams make_grid_params(const kpax_config& c)
 PlanParams make_plan_params(const kpax_config& c)
 {
     PlanParams p{};
-    p.N                  = c.max_tree;
     p.max_iter           = c.max_iter;
     p.fanout             = c.fanout;
     p.root_repeat        = c.root_repeat;
@@ -528,20 +527,24 @@ int kpax_generate_environment(int32_t kind, uint64_t seed, uint32_t n, const flo
     else if(kind == 3)
     {
         // A wall across x at the middle of the workspace, tiled 40 x 40 in
-        // (y, z), with one square 0.1-wide passage; the rest of the budget
-        // goes to thin slabs scattered on both sides.
+        // (y, z), with one square passage a tenth of the workspace wide; the
+        // rest of the budget goes to thin slabs scattered on both sides.
+        // Lengths scale with the workspace extent (unit cube: a 0.1 passage).
         const int T     = 40;
-        const double xm = 0.5 * (ws_min[0] + ws_max[0]), half = 0.01 * (ws_max[0] - ws_min[0]);
-        const double ty = (ws_max[1] - ws_min[1]) / T, tz = (ws_max[2] - ws_min[2]) / T;
-        const double py = rng.uni(ws_min[1] + 0.25 * (ws_max[1] - ws_min[1]), ws_max[1] - 0.25 * (ws_max[1] - ws_min[1]));
-        const double pz = rng.uni(ws_min[2] + 0.25 * (ws_max[2] - ws_min[2]), ws_max[2] - 0.25 * (ws_max[2] - ws_min[2]));
-        const double pw = 0.1;
+        const double ex = ws_max[0] - ws_min[0], ey = ws_max[1] - ws_min[1], ez = ws_max[2] - ws_min[2];
+        const double ext[3] = {ex, ey, ez};
+        const double sc     = std::min(ex, std::min(ey, ez));
+        const double xm = 0.5 * (ws_min[0] + ws_max[0]), half = 0.01 * ex;
+        const double ty = ey / T, tz = ez / T;
+        const double py = rng.uni(ws_min[1] + 0.25 * ey, ws_max[1] - 0.25 * ey);
+        const double pz = rng.uni(ws_min[2] + 0.25 * ez, ws_max[2] - 0.25 * ez);
+        const double pwy = 0.1 * ey, pwz = 0.1 * ez;
         for(int iy = 0; iy < T; ++iy)
             for(int iz = 0; iz < T; ++iz)
             {
                 const double y0 = ws_min[1] + iy * ty, y1 = y0 + ty, z0 = ws_min[2] + iz * tz, z1 = z0 + tz;
                 // tiles overlapping the passage window are cut back to its edges
-                const double wy0 = py - 0.5 * pw, wy1 = py + 0.5 * pw, wz0 = pz - 0.5 * pw, wz1 = pz + 0.5 * pw;
+                const double wy0 = py - 0.5 * pwy, wy1 = py + 0.5 * pwy, wz0 = pz - 0.5 * pwz, wz1 = pz + 0.5 * pwz;
                 const bool hit = y1 > wy0 && y0 < wy1 && z1 > wz0 && z0 < wz1;
                 if(!hit)
                 {
@@ -570,15 +573,15 @@ int kpax_generate_environment(int32_t kind, uint64_t seed, uint32_t n, const flo
                 const int thin = (int)(rng.next() % 3);
                 for(int d = 0; d < 3; ++d)
                 {
-                    const double e = d == thin ? 0.01 : rng.uni(0.05, 0.12);
+                    const double e = (d == thin ? 0.01 : rng.uni(0.05, 0.12)) * ext[d];
                     const double c = rng.uni(ws_min[d] + 0.5 * e, ws_max[d] - 0.5 * e);
                     b[d]           = (float)(c - 0.5 * e);
                     b[3 + d]       = (float)(c + 0.5 * e);
                 }
                 // away from the dividing wall's passage and the endpoints
                 const float pc[3] = {(float)xm, (float)py, (float)pz};
-                if(box_point_dist(b, pc) > 0.15 && clear_of(b, start, goal, 0.05) &&
-                   (b[3] < xm - half - 0.02 || b[0] > xm + half + 0.02))
+                if(box_point_dist(b, pc) > 0.15 * sc && clear_of(b, start, goal, 0.05 * sc) &&
+                   (b[3] < xm - half - 0.02 * ex || b[0] > xm + half + 0.02 * ex))
                 {
                     push(b);
                     break;

@@ -1,5 +1,5 @@
-// kpax_internal.h — structures shared by the HIP planner (kpax_device.hip)
-// and the host-only parts of libkpax (kpax_host.cpp).  Not part of the ABI.
+// kpax_internal.h — structures shared by the HIP planner (kpax_planner.hip,
+// kpax_ops.hip) and the host-only parts of libkpax (kpax_host.cpp).  Not part of the ABI.
 #pragma once
 
 #include <cstdint>
@@ -29,6 +29,7 @@ struct Iter {
     int32_t status;   // kRunning, or a terminal code (kDone*)
 };
 
+// Per-query planner record.
 struct Ctl {
     Iter cur;
     Iter nxt;
@@ -39,13 +40,35 @@ struct Ctl {
     uint64_t expansions;   // children propagated since the plan started
     uint64_t steps_done;   // integration steps taken by those children
     uint32_t iterations;   // last iteration that ran
-    uint32_t tile_expand;  // dynamic tile counters, cleared by the advance kernel
-    uint32_t tile_append;
-    uint32_t tile_frontier;
     uint32_t path_len;     // rows written by the path kernel
     uint32_t overflow;     // set if any kernel found an index past its buffer
+    uint32_t pad[5];
+};
+
+// A query slot: where its tree lives and what it plans for.
+struct QueryDev {
+    uint32_t base;   // first node of the slot's tree, children and repeat-list segments
+    uint32_t cap;    // tree cap of the query
+    uint32_t env;    // environment (collision grid) index
+    uint32_t query;  // Philox query word
+    uint64_t seed;
+    float goal[4];   // goal ball centre (xyz)
     uint32_t pad[2];
 };
+
+// Shared record of one iteration over all query slots (written by the
+// advance kernel, read by the others).
+struct BatchCtl {
+    uint32_t children;        // children of all running slots
+    uint32_t atiles, ftiles;  // append and frontier scan tiles (frontier: an upper bound)
+    uint32_t running;         // slots running in this iteration
+    uint32_t tile_expand, tile_append, tile_frontier;  // work tickets
+    uint32_t epoch;           // tag of this iteration's scan status words
+};
+
+// Largest number of query slots of one handle (the advance kernel gives each
+// slot one thread of a 1024-thread block).
+constexpr uint32_t kMaxSlots = 1024;
 
 // Dynamics and checks, with every constant folded on the host exactly as the
 // reference's macros fold in float.
@@ -72,9 +95,8 @@ struct GridParams {
     float mn[3], cell[3];
 };
 
-// Planner parameters that are not per-model.
+// Planner parameters that are not per-model or per-query.
 struct PlanParams {
-    uint32_t N;             // tree cap
     uint32_t max_iter;
     uint32_t fanout, root_repeat, sparse_repeat, dense_repeat;
     int32_t sparse_valid_below;

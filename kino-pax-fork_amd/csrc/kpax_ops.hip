@@ -1,0 +1,455 @@
+// kpax_ops.hip — the hot path's operators one at a time (parity entry points
+// of include/kpax.h) and the collision-grid build of every environment.
+//
+// The operators run the same device functions as the planner kernels
+// (kpax_device.cuh), one item per lane, on host buffers copied in and out.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kpax_handle.h"
+
+using namespace kpax;
+using namespace kpax::dev;
+
+namespace {
+
+template<int MODEL>
+__global__ __launch_bounds__(256) void k_propagate(ModelParams P, GridView G, const float* x0s, const float* us,
+                                                   const int32_t* steps, uint32_t n, float* x1s, uint8_t* valid,
+                                                   int32_t* fail)
+{
+    constexpr int S = Dims<MODEL>::S, C = Dims<MODEL>::C;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    float x0[S], u[C], x1[S];
+#pragma unroll
+    for(int d = 0; d < S; ++d) x0[d] = x0s[(size_t)d * n + i];
+#pragma unroll
+    for(int d = 0; d < C; ++d) u[d] = us[(size_t)d * n + i];
+    const int f = propagate<MODEL>(P, G, x0, u, steps[i], x1);
+#pragma unroll
+    for(int d = 0; d < S; ++d) x1s[(size_t)d * n + i] = x1[d];
+    valid[i] = f == 0 ? 1 : 0;
+    fail[i]  = f;
+}
+
+__global__ __launch_bounds__(256) void k_collide(GridView G, const float* bmin, const float* bmax, uint32_t n,
+                                                 uint8_t* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const float lo[3] = {bmin[i], bmin[(size_t)n + i], bmin[2 * (size_t)n + i]};
+    const float hi[3] = {bmax[i], bmax[(size_t)n + i], bmax[2 * (size_t)n + i]};
+    out[i]            = seg_free(G, lo, hi) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_regions(RegionParams RP, const float* st, uint32_t n, int32_t* r1,
+                                                 int32_t* r2)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const float p[3] = {st[i], st[(size_t)n + i], st[2 * (size_t)n + i]};
+    locate(RP, p, &r1[i], &r2[i]);
+}
+
+__global__ __launch_bounds__(256) void k_histogram(RegionParams RP, const float* st, const uint8_t* valid, uint32_t n,
+                                                   uint32_t t, RegionView rv)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const float p[3] = {st[i], st[(size_t)n + i], st[2 * (size_t)n + i]};
+    int r1, r2;
+    locate(RP, p, &r1, &r2);
+    atomicAdd(&rv.total[r1], 1);
+    if(valid[i])
+    {
+        atomicAdd(&rv.valid[r1], 1);
+        claim_cell(rv.r2_state, r2, t, i);
+    }
+}
+
+// Coverage after k_histogram: the winner of each newly reached cell counts it.
+__global__ __launch_bounds__(256) void k_histogram_cover(RegionParams RP, const float* st, const uint8_t* valid,
+                                                         uint32_t n, uint32_t t, RegionView rv)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n || !valid[i]) return;
+    const float p[3] = {st[i], st[(size_t)n + i], st[2 * (size_t)n + i]};
+    int r1, r2;
+    locate(RP, p, &r1, &r2);
+    if(rv.r2_state[r2] == cell_key(t, i)) atomicAdd(&rv.covered[r1], 1);
+}
+
+// Run-length expansion of weights (repeatInd after findInd).
+__global__ __launch_bounds__(256) void k_repeat_expand(const uint32_t* w, uint32_t n, uint32_t* out, uint32_t cap,
+                                                       uint64_t* status, uint32_t* ticket, uint32_t* total)
+{
+    __shared__ uint32_t lds[4];
+    __shared__ uint32_t s_tile, s_pre;
+    const uint32_t ntiles = (n + kTile - 1) / kTile;
+    for(;;)
+    {
+        const uint32_t tile = next_tile(ticket, &s_tile);
+        if(tile >= ntiles) break;
+        const uint32_t base = tile * kTile + threadIdx.x * 4;
+        uint32_t v[4], cnt = 0;
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            v[j] = base + j < n ? w[base + j] : 0u;
+            cnt += v[j];
+        }
+        uint32_t agg;
+        const uint32_t loc = block_excl_scan_256(cnt, &agg, lds);
+        const uint32_t pre = tile_prefix(status, tile, agg, 0u, 0u, &s_pre);
+        uint32_t pos       = pre + loc;
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+            for(uint32_t k = 0; k < v[j]; ++k, ++pos)
+                if(pos < cap) out[pos] = base + j;
+        if(tile == ntiles - 1 && threadIdx.x == 0) *total = pre + agg;
+    }
+}
+
+// Exclusive scan of counts into CSR offsets out[0..n] (out[n] = total).
+__global__ __launch_bounds__(256) void k_scan_offsets(const int32_t* in, uint32_t n, int32_t* out, uint64_t* status,
+                                                      uint32_t* ticket)
+{
+    __shared__ uint32_t lds[4];
+    __shared__ uint32_t s_tile, s_pre;
+    const uint32_t ntiles = (n + kTile - 1) / kTile;
+    for(;;)
+    {
+        const uint32_t tile = next_tile(ticket, &s_tile);
+        if(tile >= ntiles) break;
+        const uint32_t base = tile * kTile + threadIdx.x * 4;
+        uint32_t v[4], cnt = 0;
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            v[j] = base + j < n ? (uint32_t)in[base + j] : 0u;
+            cnt += v[j];
+        }
+        uint32_t agg;
+        const uint32_t loc = block_excl_scan_256(cnt, &agg, lds);
+        const uint32_t pre = tile_prefix(status, tile, agg, 0u, 0u, &s_pre);
+        uint32_t pos       = pre + loc;
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            if(base + j < n) out[base + j] = (int32_t)pos;
+            pos += v[j];
+        }
+        if(tile == ntiles - 1 && threadIdx.x == 0) out[n] = (int32_t)(pre + agg);
+    }
+}
+
+// Grid build (spatialHash.cu:9-54 without the 16-slot cap): count, scan,
+// fill, then sort each cell's ids so the layout is deterministic.
+__device__ __forceinline__ void obstacle_cells(const GridParams& gp, float4 a, float4 b, int lo[3], int hi[3])
+{
+    const float av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z};
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        lo[d] = cell_of(av[d], gp.mn[d], gp.cell[d], gp.D[d]);
+        hi[d] = cell_of(bv[d], gp.mn[d], gp.cell[d], gp.D[d]);
+    }
+}
+
+__global__ void k_grid_count(GridParams gp, const float4* lo, const float4* hi, uint32_t n, int32_t* counts)
+{
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if(o >= n) return;
+    int a[3], b[3];
+    obstacle_cells(gp, lo[o], hi[o], a, b);
+    for(int z = a[2]; z <= b[2]; ++z)
+        for(int y = a[1]; y <= b[1]; ++y)
+            for(int x = a[0]; x <= b[0]; ++x) atomicAdd(&counts[x + gp.D[0] * (y + gp.D[1] * z)], 1);
+}
+
+__global__ void k_grid_fill(GridParams gp, const float4* lo, const float4* hi, uint32_t n, const int32_t* start,
+                            int32_t* cursor, int32_t* items)
+{
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if(o >= n) return;
+    int a[3], b[3];
+    obstacle_cells(gp, lo[o], hi[o], a, b);
+    for(int z = a[2]; z <= b[2]; ++z)
+        for(int y = a[1]; y <= b[1]; ++y)
+            for(int x = a[0]; x <= b[0]; ++x)
+            {
+                const int c = x + gp.D[0] * (y + gp.D[1] * z);
+                items[start[c] + atomicAdd(&cursor[c], 1)] = (int32_t)o;
+            }
+}
+
+__global__ void k_grid_sort(const int32_t* start, int32_t* items, uint32_t G)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= G) return;
+    const int b = start[c], e = start[c + 1];
+    for(int i = b + 1; i < e; ++i)
+    {
+        const int32_t v = items[i];
+        int j           = i - 1;
+        while(j >= b && items[j] > v)
+        {
+            items[j + 1] = items[j];
+            --j;
+        }
+        items[j + 1] = v;
+    }
+}
+
+}  // namespace
+
+namespace kpax {
+
+void free_env(Env& e)
+{
+    if(e.start) (void)hipFree(e.start);
+    if(e.items) (void)hipFree(e.items);
+    if(e.lo) (void)hipFree(e.lo);
+    if(e.hi) (void)hipFree(e.hi);
+    e = Env{};
+}
+
+// Replaces environment `env` by n boxes and builds its CSR grid; the planner
+// reads the grid through envs_dev[env].  Runs on the handle's stream after
+// everything queued before it.
+int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n)
+{
+    HIPC(hipStreamSynchronize(h->stream));
+    Env& e = h->envs[env];
+    free_env(e);
+    e.host.assign(aabb6, aabb6 + 6 * (size_t)n);
+    e.n_obs = n;
+    std::vector<float4> lo(std::max<uint32_t>(n, 1)), hi(std::max<uint32_t>(n, 1));
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        const float* o = aabb6 + 6 * (size_t)i;
+        lo[i]          = make_float4(o[0], o[1], o[2], 0.0f);
+        hi[i]          = make_float4(o[3], o[4], o[5], 0.0f);
+    }
+    const uint32_t G = h->G;
+    HIPC(hipMalloc(&e.lo, lo.size() * sizeof(float4)));
+    HIPC(hipMalloc(&e.hi, hi.size() * sizeof(float4)));
+    HIPC(hipMalloc(&e.start, ((size_t)G + 1) * sizeof(int32_t)));
+    HIPC(hipMemcpy(e.lo, lo.data(), lo.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(e.hi, hi.data(), hi.size() * sizeof(float4), hipMemcpyHostToDevice));
+    Scratch sc;
+    int32_t* counts  = sc.get<int32_t>(G);
+    int32_t* cursor  = sc.get<int32_t>(G);
+    uint64_t* status = sc.get<uint64_t>(G / kTile + 2);
+    uint32_t* ticket = sc.get<uint32_t>(1);
+    if(!counts || !cursor || !status || !ticket) return fail(h, KPAX_NO_MEMORY, "grid scratch allocation failed");
+    HIPC(hipMemsetAsync(counts, 0, (size_t)G * 4, h->stream));
+    HIPC(hipMemsetAsync(cursor, 0, (size_t)G * 4, h->stream));
+    HIPC(hipMemsetAsync(status, 0, (G / kTile + 2) * 8, h->stream));
+    HIPC(hipMemsetAsync(ticket, 0, 4, h->stream));
+    if(n)
+        hipLaunchKernelGGL(k_grid_count, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->GPm, e.lo, e.hi, n,
+                           counts);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(std::min<uint32_t>(blocks_for(G, kTile), 4 * h->cu_count)), dim3(256), 0,
+                       h->stream, counts, G, e.start, status, ticket);
+    HIPC(hipGetLastError());
+    int32_t K = 0;
+    HIPC(hipMemcpyAsync(&K, e.start + G, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    e.k = (uint32_t)K;
+    HIPC(hipMalloc(&e.items, std::max<size_t>(K, 1) * sizeof(int32_t)));
+    if(n)
+        hipLaunchKernelGGL(k_grid_fill, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->GPm, e.lo, e.hi, n,
+                           e.start, cursor, e.items);
+    hipLaunchKernelGGL(k_grid_sort, dim3(blocks_for(G, 256)), dim3(256), 0, h->stream, e.start, e.items, G);
+    HIPC(hipGetLastError());
+    const GridView g = grid_view(h, env);
+    HIPC(hipMemcpyAsync(h->envs_dev + env, &g, sizeof(g), hipMemcpyHostToDevice, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    return KPAX_OK;
+}
+
+}  // namespace kpax
+
+extern "C" {
+
+int kpax_get_grid(kpax_handle* h, int32_t* cell_start, int32_t* items, uint32_t items_cap, uint32_t* n_cells,
+                  uint32_t* n_items)
+{
+    if(!h) return KPAX_INVALID_ARG;
+    const Env& e = h->envs[0];
+    if(n_cells) *n_cells = h->G;
+    if(n_items) *n_items = e.k;
+    HIPC(hipSetDevice(h->cfg.device));
+    if(cell_start) HIPC(hipMemcpy(cell_start, e.start, ((size_t)h->G + 1) * 4, hipMemcpyDeviceToHost));
+    if(items && e.k)
+    {
+        if(items_cap < e.k) return fail(h, KPAX_INVALID_ARG, "items_cap is smaller than the grid");
+        HIPC(hipMemcpy(items, e.items, (size_t)e.k * 4, hipMemcpyDeviceToHost));
+    }
+    return KPAX_OK;
+}
+
+int kpax_propagate_batch(kpax_handle* h, const float* x0_soa, const float* u_soa, const int32_t* steps, uint32_t n,
+                         float* x1_soa, uint8_t* valid, int32_t* fail_step)
+{
+    if(!h || (n && (!x0_soa || !u_soa || !steps || !x1_soa || !valid || !fail_step)))
+        return fail(h, KPAX_INVALID_ARG, "null argument");
+    if(n == 0) return KPAX_OK;
+    for(uint32_t i = 0; i < n; ++i)
+        if(steps[i] < 0 || steps[i] > 1000000) return fail(h, KPAX_INVALID_ARG, "step count out of range");
+    HIPC(hipSetDevice(h->cfg.device));
+    Scratch sc;
+    const int S = h->S, C = h->C;
+    float* dx0   = sc.get<float>((size_t)S * n);
+    float* du    = sc.get<float>((size_t)C * n);
+    int32_t* dst = sc.get<int32_t>(n);
+    float* dx1   = sc.get<float>((size_t)S * n);
+    uint8_t* dv  = sc.get<uint8_t>(n);
+    int32_t* df  = sc.get<int32_t>(n);
+    if(!dx0 || !du || !dst || !dx1 || !dv || !df) return fail(h, KPAX_NO_MEMORY, "scratch allocation failed");
+    HIPC(hipMemcpy(dx0, x0_soa, (size_t)S * n * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(du, u_soa, (size_t)C * n * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(dst, steps, (size_t)n * 4, hipMemcpyHostToDevice));
+    const GridView G = grid_view(h, 0);
+    const uint32_t b = blocks_for(n, 256);
+    switch(h->cfg.model)
+    {
+        case KPAX_MODEL_DOUBLE_INTEGRATOR:
+            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_DOUBLE_INTEGRATOR>, dim3(b), dim3(256), 0, h->stream, h->P, G,
+                               dx0, du, dst, n, dx1, dv, df);
+            break;
+        case KPAX_MODEL_DUBINS_AIRPLANE:
+            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_DUBINS_AIRPLANE>, dim3(b), dim3(256), 0, h->stream, h->P, G, dx0,
+                               du, dst, n, dx1, dv, df);
+            break;
+        default:
+            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_QUADCOPTER>, dim3(b), dim3(256), 0, h->stream, h->P, G, dx0, du,
+                               dst, n, dx1, dv, df);
+            break;
+    }
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+    HIPC(hipMemcpy(x1_soa, dx1, (size_t)S * n * 4, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(valid, dv, n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(fail_step, df, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
+int kpax_collide_batch(kpax_handle* h, const float* bbmin_soa, const float* bbmax_soa, uint32_t n, uint8_t* free_out)
+{
+    if(!h || (n && (!bbmin_soa || !bbmax_soa || !free_out))) return fail(h, KPAX_INVALID_ARG, "null argument");
+    if(n == 0) return KPAX_OK;
+    HIPC(hipSetDevice(h->cfg.device));
+    Scratch sc;
+    float* lo  = sc.get<float>(3 * (size_t)n);
+    float* hi  = sc.get<float>(3 * (size_t)n);
+    uint8_t* o = sc.get<uint8_t>(n);
+    if(!lo || !hi || !o) return fail(h, KPAX_NO_MEMORY, "scratch allocation failed");
+    HIPC(hipMemcpy(lo, bbmin_soa, 12 * (size_t)n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(hi, bbmax_soa, 12 * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_collide, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, grid_view(h, 0), lo, hi, n, o);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+    HIPC(hipMemcpy(free_out, o, n, hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
+int kpax_region_batch(kpax_handle* h, const float* states_soa, uint32_t n, int32_t* r1, int32_t* r2)
+{
+    if(!h || (n && (!states_soa || !r1 || !r2))) return fail(h, KPAX_INVALID_ARG, "null argument");
+    if(n == 0) return KPAX_OK;
+    HIPC(hipSetDevice(h->cfg.device));
+    Scratch sc;
+    float* st  = sc.get<float>(3 * (size_t)n);
+    int32_t* a = sc.get<int32_t>(n);
+    int32_t* b = sc.get<int32_t>(n);
+    if(!st || !a || !b) return fail(h, KPAX_NO_MEMORY, "scratch allocation failed");
+    HIPC(hipMemcpy(st, states_soa, 12 * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_regions, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->RP, st, n, a, b);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+    HIPC(hipMemcpy(r1, a, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(r2, b, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
+int kpax_region_histogram(kpax_handle* h, const float* states_soa, const uint8_t* valid, uint32_t n, int32_t* total,
+                          int32_t* valid_count, int32_t* covered)
+{
+    if(!h || !total || !valid_count || !covered || (n && (!states_soa || !valid)))
+        return fail(h, KPAX_INVALID_ARG, "null argument");
+    HIPC(hipSetDevice(h->cfg.device));
+    const size_t r1 = (size_t)h->RP.r1_count, r2 = r1 * h->RP.r2_per;
+    Scratch sc;
+    float* st  = sc.get<float>(3 * (size_t)n);
+    uint8_t* v = sc.get<uint8_t>(n);
+    RegionView rv{};
+    rv.total    = sc.get<int32_t>(r1);
+    rv.valid    = sc.get<int32_t>(r1);
+    rv.covered  = sc.get<int32_t>(r1);
+    rv.r2_state = sc.get<uint64_t>(r2);
+    if(!st || !v || !rv.total || !rv.valid || !rv.covered || !rv.r2_state)
+        return fail(h, KPAX_NO_MEMORY, "scratch allocation failed");
+    if(n)
+    {
+        HIPC(hipMemcpy(st, states_soa, 12 * (size_t)n, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(v, valid, n, hipMemcpyHostToDevice));
+    }
+    HIPC(hipMemcpy(rv.total, total, r1 * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(rv.valid, valid_count, r1 * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(rv.covered, covered, r1 * 4, hipMemcpyHostToDevice));
+    // the batch is one iteration on an empty coverage map
+    HIPC(hipMemsetAsync(rv.r2_state, 0xff, r2 * sizeof(uint64_t), h->stream));
+    if(n)
+    {
+        hipLaunchKernelGGL(k_histogram, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->RP, st, v, n, 1u, rv);
+        hipLaunchKernelGGL(k_histogram_cover, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->RP, st, v, n, 1u,
+                           rv);
+    }
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+    HIPC(hipMemcpy(total, rv.total, r1 * 4, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(valid_count, rv.valid, r1 * 4, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(covered, rv.covered, r1 * 4, hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
+int kpax_repeat_expand(kpax_handle* h, const uint32_t* weights, uint32_t n, uint32_t* out, uint32_t out_cap,
+                       uint32_t* total)
+{
+    if(!h || !total || (n && !weights) || (out_cap && !out)) return fail(h, KPAX_INVALID_ARG, "null argument");
+    uint64_t sum = 0;
+    for(uint32_t i = 0; i < n; ++i) sum += weights[i];
+    if(sum > 0xffffffffull) return fail(h, KPAX_INVALID_ARG, "total weight exceeds 2^32-1");
+    *total = 0;
+    if(n == 0) return KPAX_OK;
+    HIPC(hipSetDevice(h->cfg.device));
+    Scratch sc;
+    const uint32_t ntiles = (n + kTile - 1) / kTile;
+    uint32_t* dw          = sc.get<uint32_t>(n);
+    uint32_t* dout        = sc.get<uint32_t>(std::max<uint32_t>(out_cap, 1));
+    uint64_t* status      = sc.get<uint64_t>(ntiles + 1);
+    uint32_t* ticket      = sc.get<uint32_t>(2);
+    if(!dw || !dout || !status || !ticket) return fail(h, KPAX_NO_MEMORY, "scratch allocation failed");
+    HIPC(hipMemcpy(dw, weights, (size_t)n * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemsetAsync(status, 0, (ntiles + 1) * 8, h->stream));
+    HIPC(hipMemsetAsync(ticket, 0, 8, h->stream));
+    hipLaunchKernelGGL(k_repeat_expand, dim3(std::min<uint32_t>(ntiles, 4 * h->cu_count)), dim3(256), 0, h->stream,
+                       dw, n, dout, out_cap, status, ticket, ticket + 1);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+    HIPC(hipMemcpy(total, ticket + 1, 4, hipMemcpyDeviceToHost));
+    const uint32_t k = std::min<uint32_t>(*total, out_cap);
+    if(k) HIPC(hipMemcpy(out, dout, (size_t)k * 4, hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1591 @@
+// kpax_planner.hip — MI355X kernels of the Kino-PAX planner and the planner
+// half of the C ABI (include/kpax.h): single plans and batches of queries.
+//
+// A handle holds Qmax query slots over one node pool of T nodes.  Each
+// admitted query owns a slot and a contiguous node range [base, base + cap)
+// of the tree, children and repeat-list arrays (SoA, column d of node i at
+// d * T + i), plus its own region counters.  One planner iteration advances
+// every running query with five launches on the handle's stream and no host
+// round trip between them (the reference makes at least eight blocking
+// device->host transfers per iteration of its single query, KPAX.cu:303-543):
+//   k_advance   1 block   publish each slot's iteration record; scan the
+//                         slots' child / tile counts into offsets
+//   k_expand    hot       one lane per child: Philox controls, RK4 with checks,
+//                         region counters, admission (propagateFrontier_kernel1/2)
+//   k_append    scan      admitted children -> tree in slot order, goal test
+//                         (updateFrontier_kernel, first half)
+//   k_score     1 block per slot: region scores (Graph::updateVertices)
+//   k_frontier  scan      re-admission + repeat counts -> next repeat list
+//                         (updateFrontier_kernel second half + findInd/repeatInd)
+// Work of all slots is flattened: children, append tiles and frontier tiles
+// are numbered across slots by the advance kernel's offsets, and each item
+// finds its slot by binary search.  With profiling off the five launches and
+// the status copy are replayed as one captured HIP graph.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "kpax_handle.h"
+
+using namespace kpax;
+using namespace kpax::dev;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Slot set-up
+// ---------------------------------------------------------------------------
+
+// Clears the region state of the listed slots (fill kernel rather than
+// hipMemsetAsync, so every per-plan launch is capturable).
+__global__ __launch_bounds__(256) void k_reset_regions(RegionView rv, const uint32_t* slots, uint32_t n, uint32_t R1,
+                                                       uint32_t R2)
+{
+    const uint64_t total = (uint64_t)n * R2;
+    for(uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x)
+    {
+        const uint32_t j = (uint32_t)(i / R2), e = (uint32_t)(i - (uint64_t)j * R2);
+        const uint64_t q = slots[j];
+        rv.r2_state[q * R2 + e] = ~0ull;
+        if(e < R1)
+        {
+            const uint64_t o = q * R1 + e;
+            rv.valid[o]      = 0;
+            rv.total[o]      = 0;
+            rv.covered[o]    = 0;
+            rv.score[o]      = 0.0f;
+        }
+    }
+}
+
+// Root and first iteration record of each listed slot (KPAX.cu:60-88).  One
+// block per slot; starts holds 16 floats per listed slot.
+__global__ __launch_bounds__(256) void k_init(Ctl* ctl, const QueryDev* qd, TreeView tree, uint32_t* R,
+                                              int32_t* slot_status, const uint32_t* slots, const float* starts, int S,
+                                              int C, uint32_t T, RegionParams RP, PlanParams PP)
+{
+    const uint32_t q   = slots[blockIdx.x];
+    const QueryDev d   = qd[q];
+    const float* start = starts + 16 * (size_t)blockIdx.x;
+    const size_t b     = d.base;
+    const int tid      = threadIdx.x;
+    if(tid < S) tree.st[(size_t)tid * T + b] = start[tid];
+    if(tid < C) tree.ct[(size_t)tid * T + b] = 0.0f;
+    for(uint32_t i = tid; i < PP.root_repeat && i < d.cap; i += blockDim.x) R[b + i] = 0u;
+    if(tid == 0)
+    {
+        tree.steps[b]  = 0;
+        tree.parent[b] = -1;
+        float p[3]     = {start[0], start[1], start[2]};
+        int r1, r2;
+        locate(RP, p, &r1, &r2);
+        tree.reg[b] = r1;
+        Ctl c;
+        memset(&c, 0, sizeof(c));
+        c.nxt          = make_iter(1u, 1u, PP.root_repeat, d.cap, PP);
+        c.cur          = c.nxt;
+        c.cur.status   = kNotStarted;  // nothing runs until k_advance publishes nxt
+        c.goal         = INT_MAX;
+        ctl[q]         = c;
+        slot_status[q] = c.nxt.status;
+    }
+}
+
+// Everything k_expand reads that is fixed for a launch lives in one
+// device-side block behind a single __restrict__ pointer: as by-value kernel
+// arguments the structures exceed the scalar register budget and get spilled
+// into vector lanes, costing a readlane at every use inside the step loop,
+// while values in the block can be re-read with scalar loads whenever the
+// compiler needs the registers.  `s0` is slot 0's record, query and grid,
+// copied by k_advance each iteration for the single-slot form.
+struct SlotUniform {
+    Iter cur;
+    QueryDev qd;
+    GridView G;
+};
+
+struct ExpandArgs {
+    Ctl* ctl;
+    const QueryDev* qd;
+    BatchCtl* bc;
+    const uint32_t* child_off;
+    const GridView* envs;
+    TreeView tree;
+    ChildView ch;
+    const uint32_t* R;
+    RegionView rv;
+    ModelParams P;
+    RegionParams RP;
+    PlanParams PP;
+    GridParams GP;
+    uint32_t T, Q, R2;
+    uint32_t chunk;  // children per work ticket (multiple of 64)
+    SlotUniform s0;
+};
+
+// ---------------------------------------------------------------------------
+// Advance: one thread per slot publishes nxt -> cur; the block scans the
+// running slots' children, append tiles and frontier tiles (an upper bound:
+// the tree grows by at most M) into offsets, resets the work tickets and
+// moves the scan epoch on.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, BatchCtl* bc, uint32_t* child_off,
+                                                  uint32_t* atile_off, uint32_t* ftile_off, int32_t* slot_status,
+                                                  uint32_t Q, uint64_t* st_append, uint64_t* st_frontier,
+                                                  uint32_t n_status, PlanParams PP, const GridView* envs,
+                                                  SlotUniform* s0)
+{
+    __shared__ uint32_t lds[16];
+    __shared__ uint32_t s_epoch;
+    const uint32_t q = threadIdx.x;
+    uint32_t m = 0, at = 0, ft = 0, run = 0;
+    if(q < Q)
+    {
+        Ctl* c       = &ctl[q];
+        const Iter n = c->nxt;
+        c->cur       = n;
+        if(q == 0)
+        {
+            s0->cur = n;
+            s0->qd  = qd[0];
+            s0->G   = envs[qd[0].env];
+        }
+        if(n.status == kRunning)
+        {
+            c->expansions += n.M;
+            c->Fn       = 0;
+            c->f_accept = f_accept(n.t, n.ts, qd[q].cap, PP.eps);  // kept when no child is admitted
+            m           = n.M;
+            at          = (n.M + kTile - 1) / kTile;
+            ft          = (uint32_t)(((uint64_t)n.ts + n.M + kTile - 1) / kTile);
+            run         = 1;
+        }
+    }
+    uint32_t tm, ta, tf, tr;
+    const uint32_t em = block_excl_scan<16>(m, &tm, lds);
+    const uint32_t ea = block_excl_scan<16>(at, &ta, lds);
+    const uint32_t ef = block_excl_scan<16>(ft, &tf, lds);
+    (void)block_excl_scan<16>(run, &tr, lds);
+    if(q < Q)
+    {
+        child_off[q] = em;
+        atile_off[q] = ea;
+        ftile_off[q] = ef;
+    }
+    if(q == 0)
+    {
+        child_off[Q]     = tm;
+        atile_off[Q]     = ta;
+        ftile_off[Q]     = tf;
+        bc->children     = tm;
+        bc->atiles       = ta;
+        bc->ftiles       = tf;
+        bc->running      = tr;
+        bc->tile_expand  = 0;
+        bc->tile_append  = 0;
+        bc->tile_frontier = 0;
+        uint32_t e       = bc->epoch + 1;
+        if(e > kEpochMask) e = 1;
+        bc->epoch      = e;
+        s_epoch        = e;
+        slot_status[Q] = (int32_t)tr;
+    }
+    __syncthreads();
+    // epoch 1 starts a new cycle of epochs: words left from the last cycle
+    // could carry the same epoch, so the status arrays are cleared once
+    if(s_epoch == 1)
+        for(uint32_t i = threadIdx.x; i < n_status; i += blockDim.x)
+        {
+            st_append[i]   = 0;
+            st_frontier[i] = 0;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Expansion: propagateFrontier_kernel1/2 (KPAX.cu:358-439) with counter-based
+// draws.  Waves are persistent and advance their 64 children one integration
+// step per trip through the loop; a lane whose child has finished (all its
+// steps, or an invalid step) writes the child out and takes the next one
+// from the wave's chunk of children, and the wave takes a new chunk from a
+// global ticket when its chunk runs out.  Children draw 5..20 steps and many
+// stop early, so refilling keeps the lanes of a wave busy instead of waiting
+// for the longest edge.  Each child's arithmetic is that of propagate().
+//
+// MULTI = false: one slot (slot 0); its record, seed and grid are uniform
+// and region counters are summed in LDS.  MULTI = true: each lane looks up
+// the slot of its child (children of all slots are numbered consecutively)
+// and carries that slot's iteration, node range and grid pointers.
+// ---------------------------------------------------------------------------
+// Five resident waves per SIMD (96 VGPRs for the quadcopter) hide more of
+// the grid lookups' latency than four; the batched form carries a slot's
+// state per lane and keeps four.
+template<int MODEL, bool MULTI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 : 5))) void k_expand(
+  const ExpandArgs* __restrict__ A)
+{
+    const TreeView& tree   = A->tree;
+    const ChildView& ch    = A->ch;
+    const uint32_t* R      = A->R;
+    const RegionView& rv   = A->rv;
+    const ModelParams& P   = A->P;
+    const RegionParams& RP = A->RP;
+    const uint32_t T       = A->T;
+    constexpr int S = Dims<MODEL>::S, C = Dims<MODEL>::C;
+    constexpr uint32_t kNone = 0xffffffffu;
+    // Region counters of the single-slot form are summed in LDS (total in
+    // [0, R1), valid in [R1, 2 R1)) and added to HBM once per work-group:
+    // siblings land in the same few regions, so per-child HBM atomics would
+    // serialise on them.
+    HIP_DYNAMIC_SHARED(int32_t, s_hist)
+    const uint32_t total = MULTI ? A->bc->children : (A->s0.cur.status == kRunning ? A->s0.cur.M : 0u);
+    if(total == 0) return;
+    const int R1 = RP.r1_count;
+    // slot-uniform in the single-slot form, per lane in the batched form
+    uint32_t t = 0, fan = 1, k0 = 0, k1 = 0, qword = 0, slot = 0, base = 0;
+    GridView G;
+    bool local = false;
+    if constexpr(!MULTI)
+    {
+        t     = A->s0.cur.t;
+        fan   = A->s0.cur.fan;
+        base  = A->s0.qd.base;
+        k0    = (uint32_t)A->s0.qd.seed;
+        k1    = (uint32_t)(A->s0.qd.seed >> 32);
+        qword = A->s0.qd.query;
+        G     = A->s0.G;
+        local           = R1 <= kLdsRegions;
+        if(local)
+        {
+            for(int r = threadIdx.x; r < 2 * R1; r += blockDim.x) s_hist[r] = 0;
+            __syncthreads();
+        }
+    }
+    else
+        G.gp = A->GP;  // grid parameters are shared; pointers are set per child
+    const int lane       = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t c = kNone, parent = 0, admit_draw = 0;
+    int i = 0, n = 0;
+    float x[S], u[C];
+    uint32_t q_next = 0, q_end = 0;  // the wave's current chunk [q_next, q_end)
+    bool drained        = false;
+    uint32_t steps_done = 0, steps_slot = 0;  // integration steps not yet added to ctl[steps_slot]
+    for(;;)
+    {
+        if(!drained)
+        {
+            uint64_t need = __ballot(c == kNone);
+            while(need)
+            {
+                if(q_next >= q_end)
+                {
+                    const uint32_t chunk = A->chunk;
+                    uint32_t b           = 0;
+                    if(lane == 0) b = atomicAdd(&A->bc->tile_expand, chunk);
+                    b = __shfl(b, 0, 64);
+                    if(b >= total)
+                    {
+                        drained = true;
+                        break;
+                    }
+                    q_next = b;
+                    q_end  = total - b > chunk ? b + chunk : total;
+                }
+                const uint32_t avail = q_end - q_next;
+                const uint32_t rank  = (uint32_t)__popcll(need & below);
+                if(c == kNone && rank < avail)
+                {
+                    const uint32_t k = q_next + rank;
+                    uint32_t ci      = k;
+                    if constexpr(MULTI)
+                    {
+                        const uint32_t s = segment_of(A->child_off, A->Q, k);
+                        ci               = k - A->child_off[s];
+                        t                = A->ctl[s].cur.t;
+                        fan              = A->ctl[s].cur.fan;
+                        const QueryDev& d = A->qd[s];
+                        base             = d.base;
+                        k0               = (uint32_t)d.seed;
+                        k1               = (uint32_t)(d.seed >> 32);
+                        qword            = d.query;
+                        const GridView& g = A->envs[d.env];
+                        G.start          = g.start;
+                        G.items          = g.items;
+                        G.lo             = g.lo;
+                        G.hi             = g.hi;
+                        if(s != steps_slot && steps_done)
+                        {
+                            atomicAdd((unsigned long long*)&A->ctl[steps_slot].steps_done,
+                                      (unsigned long long)steps_done);
+                            steps_done = 0;
+                        }
+                        steps_slot = s;
+                        slot       = s;
+                    }
+                    c      = ci;
+                    parent            = R[base + c / fan];
+                    const uint32_t pn = base + parent;
+#pragma unroll
+                    for(int d = 0; d < S; ++d) x[d] = tree.st[(size_t)d * T + pn];
+                    const uint4 dc        = philox(c, t, qword, PURPOSE_CONTROL, k0, k1);
+                    const uint4 ds        = philox(c, t, qword, PURPOSE_STEPS_ADMIT, k0, k1);
+                    const uint32_t dcw[4] = {dc.x, dc.y, dc.z, dc.w};
+#pragma unroll
+                    for(int j = 0; j < C; ++j) u[j] = P.ctrl_min[j] + u01(dcw[j]) * P.ctrl_rng[j];
+                    int kk = (int)(u01(ds.x) * (float)P.span);
+                    if(kk > P.span - 1) kk = P.span - 1;
+                    n          = P.steps_min + kk;
+                    i          = 0;
+                    admit_draw = ds.y;
+                }
+                const uint32_t wanted = (uint32_t)__popcll(need);
+                q_next += wanted < avail ? wanted : avail;
+                need = __ballot(c == kNone);
+            }
+        }
+        if(__ballot(c != kNone) == 0) break;
+        if(c != kNone)
+        {
+            const bool ok = step<MODEL>(P, G, x, u);
+            ++i;
+            if(!ok || i == n)
+            {
+                steps_done += (uint32_t)i;
+                int r1, r2;
+                locate(RP, x, &r1, &r2);
+                const size_t ro = MULTI ? (size_t)slot * R1 : 0;
+                if(local)
+                {
+                    atomicAdd(&s_hist[r1], 1);
+                    if(ok) atomicAdd(&s_hist[R1 + r1], 1);
+                }
+                else
+                {
+                    atomicAdd(&rv.total[ro + r1], 1);
+                    if(ok) atomicAdd(&rv.valid[ro + r1], 1);
+                }
+                uint32_t flags = 0;
+                if(ok)
+                {
+                    flags = kValid | claim_cell(rv.r2_state + (MULTI ? (size_t)slot * A->R2 : 0), r2, t, c);
+                    if(u01(admit_draw) < rv.score[ro + r1]) flags |= kScoreAdmit;
+                }
+                const uint32_t o = base + c;
+#pragma unroll
+                for(int d = 0; d < S; ++d) ch.st[(size_t)d * T + o] = x[d];
+#pragma unroll
+                for(int d = 0; d < C; ++d) ch.ct[(size_t)d * T + o] = u[d];
+                ch.steps[o]  = n;
+                ch.parent[o] = (int32_t)parent;
+                ch.reg[o]    = r1;
+                ch.r2[o]     = r2;
+                ch.adm[o]    = (uint8_t)flags;
+                c            = kNone;
+            }
+        }
+    }
+    if constexpr(!MULTI)
+    {
+        const uint32_t ws = wave_sum(steps_done);
+        if(lane == 0 && ws) atomicAdd((unsigned long long*)&A->ctl[0].steps_done, (unsigned long long)ws);
+        if(local)
+        {
+            __syncthreads();
+            for(int r = threadIdx.x; r < R1; r += blockDim.x)
+            {
+                const int32_t a = s_hist[r], b = s_hist[R1 + r];
+                if(a) atomicAdd(&rv.total[r], a);
+                if(b) atomicAdd(&rv.valid[r], b);
+            }
+        }
+    }
+    else
+    {
+        // one atomic per wave when every lane's remainder belongs to one slot
+        const uint64_t has = __ballot(steps_done != 0);
+        if(has)
+        {
+            const int leader  = __builtin_ctzll(has);
+            const uint32_t ls = __shfl(steps_slot, leader, 64);
+            if(__ballot(steps_done != 0 && steps_slot != ls) == 0)
+            {
+                const uint32_t s = wave_sum(steps_done);
+                if(lane == leader) atomicAdd((unsigned long long*)&A->ctl[ls].steps_done, (unsigned long long)s);
+            }
+            else if(steps_done)
+                atomicAdd((unsigned long long*)&A->ctl[steps_slot].steps_done, (unsigned long long)steps_done);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Region scores (Graph.cu:233-358, the evident intent): raw Syclop score per
+// R1 region, their sum in a fixed order (1024 strided partial sums, then a
+// pairwise tree), normalised eps + score / total, or 1 for regions with no
+// valid child.  One block of 1024 threads per slot; with ctl == nullptr one
+// block scores the arrays as given (parity operator).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_score(Ctl* ctl, RegionView rv, RegionParams RP, float eps)
+{
+    __shared__ float part[1024];
+    const uint32_t q = blockIdx.x;
+    if(ctl && ctl[q].cur.status != kRunning) return;
+    const size_t o    = (size_t)q * RP.r1_count;
+    const int32_t* va = rv.valid + o;
+    const int32_t* to = rv.total + o;
+    const int32_t* co = rv.covered + o;
+    float* sc         = rv.score + o;
+    const int t       = threadIdx.x;
+    float s           = 0.0f;
+    for(int r = t; r < RP.r1_count; r += 1024)
+    {
+        float raw   = 0.0f;
+        const int v = va[r];
+        if(v > 0)
+        {
+            const int n0    = to[r];
+            const float cov = (float)co[r] / (float)RP.r2_per;
+            const float ev  = eps + (float)v;
+            const float fv  = ev / (ev + (float)(n0 - v)) * RP.vol;
+            const float n   = (float)n0;
+            raw             = (fv * fv) * (fv * fv) / ((1.0f + cov) * (1.0f + n * n));
+        }
+        sc[r] = raw;
+        s += raw;
+    }
+    part[t] = s;
+    __syncthreads();
+    for(int off = 512; off >= 1; off >>= 1)
+    {
+        if(t < off) part[t] = part[t] + part[t + off];
+        __syncthreads();
+    }
+    const float tot = part[0];
+    for(int r = t; r < RP.r1_count; r += 1024) sc[r] = va[r] == 0 ? 1.0f : eps + sc[r] / tot;
+    if(t == 0 && ctl) ctl[q].score_total = tot;
+}
+
+// ---------------------------------------------------------------------------
+// Append: compaction of the admitted children into the tree in slot order
+// and the goal test (updateFrontier_kernel, KPAX.cu:458-502).  Tiles of all
+// slots are numbered consecutively; each slot's tiles form one look-back
+// segment.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, BatchCtl* bc, const uint32_t* atile_off,
+                                                uint32_t Q, ChildView ch, TreeView tree, RegionView rv,
+                                                uint64_t* status, PlanParams PP, int S, int C, uint32_t T,
+                                                uint32_t R1, uint32_t R2)
+{
+    __shared__ uint32_t lds[4];
+    __shared__ uint32_t s_tile, s_pre;
+    const uint32_t ntiles = bc->atiles, epoch = bc->epoch;
+    for(;;)
+    {
+        const uint32_t tile = next_tile(&bc->tile_append, &s_tile);
+        if(tile >= ntiles) break;
+        const uint32_t q      = segment_of(atile_off, Q, tile);
+        const uint32_t first  = atile_off[q], lt = tile - first, nt = atile_off[q + 1] - first;
+        const Iter it         = ctl[q].cur;
+        const QueryDev d      = qd[q];
+        const uint32_t M      = it.M, ts = it.ts, cap = d.cap;
+        const size_t base     = d.base;
+        const uint64_t* r2s   = rv.r2_state + (size_t)q * R2;
+        int32_t* cov          = rv.covered + (size_t)q * R1;
+        const uint32_t cbase  = lt * kTile + threadIdx.x * 4;
+        uint32_t f[4], cnt = 0;
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            const uint32_t c = cbase + j;
+            const uint32_t a = c < M ? ch.adm[base + c] : 0u;
+            // a candidate that holds its cell's key was the first to reach it:
+            // it covers the cell and is admitted (KPAX.cu:394-395)
+            bool first_in_cell = false;
+            if(a & kCandidate)
+            {
+                first_in_cell = r2s[ch.r2[base + c]] == cell_key(it.t, c);
+                if(first_in_cell) atomicAdd(&cov[ch.reg[base + c]], 1);
+            }
+            f[j] = (a & kValid) && ((a & kScoreAdmit) || first_in_cell) ? 1u : 0u;
+            cnt += f[j];
+        }
+        uint32_t agg;
+        const uint32_t loc = block_excl_scan_256(cnt, &agg, lds);
+        const uint32_t pre = tile_prefix(status, tile, agg, first, epoch, &s_pre);
+        uint32_t dst       = ts + pre + loc;
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            if(!f[j]) continue;
+            const size_t c = base + cbase + j;
+            if(dst >= cap)
+            {
+                ctl[q].overflow = 1;  // cannot happen while M <= cap - ts; never write past the range
+                break;
+            }
+            const size_t o = base + dst;
+            for(int dd = 0; dd < S; ++dd) tree.st[(size_t)dd * T + o] = ch.st[(size_t)dd * T + c];
+            for(int dd = 0; dd < C; ++dd) tree.ct[(size_t)dd * T + o] = ch.ct[(size_t)dd * T + c];
+            tree.steps[o]  = ch.steps[c];
+            tree.parent[o] = ch.parent[c];
+            tree.reg[o]    = ch.reg[c];
+            const float dx = ch.st[c] - d.goal[0], dy = ch.st[(size_t)T + c] - d.goal[1],
+                        dz = ch.st[2 * (size_t)T + c] - d.goal[2];
+            if(__builtin_sqrtf(dx * dx + dy * dy + dz * dz) < PP.goal_radius) atomicMin(&ctl[q].goal, (int32_t)dst);
+            ++dst;
+        }
+        if(lt == nt - 1 && threadIdx.x == 0)
+        {
+            const uint32_t Fn = pre + agg;
+            ctl[q].Fn         = Fn;
+            ctl[q].f_accept   = f_accept(it.t, ts + Fn, cap, PP.eps);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Next frontier: weights of all tree nodes (re-admission of old nodes with
+// probability score + fAccept, KPAX.cu:505-517; repeat count 15 / 1 of new
+// nodes by their region's valid count, KPAX.cu:474-479), scanned and
+// run-length expanded into the repeat list (Planner.cu:52-80).  A slot whose
+// goal was reached ends here instead.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_frontier(Ctl* ctl, const QueryDev* qd, BatchCtl* bc,
+                                                  const uint32_t* ftile_off, int32_t* slot_status, uint32_t Q,
+                                                  TreeView tree, uint32_t* R, RegionView rv, uint64_t* status,
+                                                  PlanParams PP, uint32_t R1)
+{
+    __shared__ uint32_t lds[4];
+    __shared__ uint32_t s_tile, s_pre;
+    const uint32_t ntiles = bc->ftiles, epoch = bc->epoch;
+    for(;;)
+    {
+        const uint32_t tile = next_tile(&bc->tile_frontier, &s_tile);
+        if(tile >= ntiles) break;
+        const uint32_t q     = segment_of(ftile_off, Q, tile);
+        const uint32_t first = ftile_off[q], lt = tile - first;
+        const Iter it        = ctl[q].cur;
+        const int32_t goal   = ctl[q].goal;
+        const uint32_t Fn    = ctl[q].Fn;
+        if(goal != INT_MAX)
+        {
+            if(lt == 0 && threadIdx.x == 0)
+            {
+                Iter n            = it;
+                n.ts              = it.ts + Fn;
+                n.status          = kDoneFound;
+                ctl[q].nxt        = n;
+                ctl[q].iterations = it.t;
+                slot_status[q]    = kDoneFound;
+            }
+            continue;
+        }
+        const uint32_t tsn = it.ts + Fn, nt = (tsn + kTile - 1) / kTile;
+        if(lt >= nt) continue;  // the slot's tile budget assumed every child admitted
+        const QueryDev d       = qd[q];
+        const float fa         = ctl[q].f_accept;
+        const size_t base      = d.base;
+        const uint32_t k0      = (uint32_t)d.seed, k1 = (uint32_t)(d.seed >> 32);
+        const float* score     = rv.score + (size_t)q * R1;
+        const int32_t* valid   = rv.valid + (size_t)q * R1;
+        const uint32_t nbase   = lt * kTile + threadIdx.x * 4;
+        uint32_t w[4], cnt = 0;
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            const uint32_t node = nbase + j;
+            w[j]                = 0;
+            if(node < tsn)
+            {
+                const int r = tree.reg[base + node];
+                if(node < it.ts)
+                {
+                    const uint4 dr = philox(node, it.t, d.query, PURPOSE_READMIT, k0, k1);
+                    w[j]           = u01(dr.x) <= score[r] + fa ? 1u : 0u;
+                }
+                else
+                    w[j] = valid[r] < PP.sparse_valid_below ? PP.sparse_repeat : PP.dense_repeat;
+            }
+            cnt += w[j];
+        }
+        uint32_t agg;
+        const uint32_t loc = block_excl_scan_256(cnt, &agg, lds);
+        const uint32_t pre = tile_prefix(status, tile, agg, first, epoch, &s_pre);
+        uint32_t pos       = pre + loc;
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+            for(uint32_t k = 0; k < w[j]; ++k, ++pos)
+                if(pos < d.cap) R[base + pos] = nbase + j;
+        if(lt == nt - 1 && threadIdx.x == 0)
+        {
+            const Iter n      = make_iter(it.t + 1, tsn, pre + agg, d.cap, PP);
+            ctl[q].nxt        = n;
+            ctl[q].iterations = it.t;
+            slot_status[q]    = n.status;
+        }
+    }
+}
+
+// Goal -> root walk of each listed slot (the reference walks it concurrently
+// from every thread that reached the goal, KPAX.cu:482-501).  Lane 0 follows
+// the parent chain; then the block gathers the rows.  Rows go out in
+// goal -> root order; the host reverses them.
+__global__ __launch_bounds__(64) void k_path(Ctl* ctl, const QueryDev* qd, const uint32_t* slots, TreeView tree, int S,
+                                             int C, uint32_t T, float dt, float* rows, int32_t* steps, int32_t* idx,
+                                             uint32_t cap)
+{
+    __shared__ uint32_t s_len;
+    const uint32_t q   = slots[blockIdx.x];
+    const size_t base  = qd[q].base;
+    const int D        = S + C + 1;
+    int32_t* my_idx    = idx + (size_t)q * cap;
+    if(threadIdx.x == 0)
+    {
+        int32_t j    = ctl[q].goal;
+        uint32_t len = 0;
+        if(j != INT_MAX)
+            while(j != -1 && len < cap)
+            {
+                my_idx[len++] = j;
+                j             = tree.parent[base + j];
+            }
+        s_len          = (j == -1) ? len : 0u;
+        ctl[q].path_len = s_len;
+    }
+    __syncthreads();
+    const uint32_t len = s_len;
+    float* my_rows     = rows + (size_t)q * cap * D;
+    for(uint32_t e = threadIdx.x; e < len * (uint32_t)D; e += blockDim.x)
+    {
+        const uint32_t r = e / D, col = e - r * D;
+        const size_t node = base + (uint32_t)my_idx[r];
+        float v;
+        if(col < (uint32_t)S)
+            v = tree.st[(size_t)col * T + node];
+        else if(col < (uint32_t)(S + C))
+            v = tree.ct[(size_t)(col - S) * T + node];
+        else
+            v = dt * (float)tree.steps[node];
+        my_rows[e] = v;
+    }
+    for(uint32_t r = threadIdx.x; r < len; r += blockDim.x) steps[(size_t)q * cap + r] = tree.steps[base + my_idx[r]];
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+size_t expand_lds_bytes(const kpax_handle* h)
+{
+    return h->Qmax == 1 && h->RP.r1_count <= kLdsRegions ? 2 * (size_t)h->RP.r1_count * sizeof(int32_t) : 0;
+}
+
+// Work-groups of k_expand that fit on the device at once: its waves are
+// persistent, so more would only wait for the first ones to drain the queue.
+template<int MODEL, bool MULTI>
+uint32_t expand_blocks_for(const kpax_handle* h)
+{
+    int per_cu = 0;
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand<MODEL, MULTI>, 256, expand_lds_bytes(h)) !=
+         hipSuccess ||
+       per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)per_cu * (uint32_t)h->cu_count;
+}
+
+template<int MODEL>
+uint32_t expand_blocks_model(const kpax_handle* h)
+{
+    return h->Qmax == 1 ? expand_blocks_for<MODEL, false>(h) : expand_blocks_for<MODEL, true>(h);
+}
+
+uint32_t expand_blocks(const kpax_handle* h)
+{
+    switch(h->cfg.model)
+    {
+        case KPAX_MODEL_DOUBLE_INTEGRATOR: return expand_blocks_model<KPAX_MODEL_DOUBLE_INTEGRATOR>(h);
+        case KPAX_MODEL_DUBINS_AIRPLANE: return expand_blocks_model<KPAX_MODEL_DUBINS_AIRPLANE>(h);
+        default: return expand_blocks_model<KPAX_MODEL_QUADCOPTER>(h);
+    }
+}
+
+int upload_expand_args(kpax_handle* h)
+{
+    ExpandArgs a;
+    a.ctl       = h->ctl;
+    a.qd        = h->qd;
+    a.bc        = h->bc;
+    a.child_off = h->child_off;
+    a.envs      = h->envs_dev;
+    a.tree      = h->tree;
+    a.ch        = h->ch;
+    a.R         = h->R;
+    a.rv        = h->rv;
+    a.P         = h->P;
+    a.RP        = h->RP;
+    a.PP        = h->PP;
+    a.GP        = h->GPm;
+    a.T         = h->T;
+    a.Q         = h->Qmax;
+    a.R2        = h->R2;
+    a.chunk     = h->expand_chunk;
+    memset(&a.s0, 0, sizeof(a.s0));
+    a.s0.cur.status = kNotStarted;
+    HIPC(hipMemcpy(h->expand_args, &a, sizeof(a), hipMemcpyHostToDevice));
+    return KPAX_OK;
+}
+
+template<int MODEL>
+void launch_expand_model(kpax_handle* h)
+{
+    const ExpandArgs* A = (const ExpandArgs*)h->expand_args;
+    if(h->Qmax == 1)
+        hipLaunchKernelGGL((k_expand<MODEL, false>), dim3(h->expand_blocks), dim3(256), expand_lds_bytes(h), h->stream,
+                           A);
+    else
+        hipLaunchKernelGGL((k_expand<MODEL, true>), dim3(h->expand_blocks), dim3(256), 0, h->stream, A);
+}
+
+void launch_expand(kpax_handle* h)
+{
+    switch(h->cfg.model)
+    {
+        case KPAX_MODEL_DOUBLE_INTEGRATOR: launch_expand_model<KPAX_MODEL_DOUBLE_INTEGRATOR>(h); break;
+        case KPAX_MODEL_DUBINS_AIRPLANE: launch_expand_model<KPAX_MODEL_DUBINS_AIRPLANE>(h); break;
+        default: launch_expand_model<KPAX_MODEL_QUADCOPTER>(h); break;
+    }
+}
+
+// The five launches of one iteration and the copy of the slot statuses into
+// the pinned buffer of parity p.  With `ev` set, phase boundaries are recorded
+// between the launches.
+int enqueue_iteration(kpax_handle* h, int p, bool ev)
+{
+    const uint32_t Q = h->Qmax, R1 = (uint32_t)h->RP.r1_count;
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc, h->child_off, h->atile_off,
+                       h->ftile_off, h->slot_status, Q, h->st_append, h->st_frontier, h->n_status, h->PP, h->envs_dev,
+                       &((ExpandArgs*)h->expand_args)->s0);
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
+    launch_expand(h);
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][1], h->stream));
+    hipLaunchKernelGGL(k_append, dim3(h->scan_blocks), dim3(256), 0, h->stream, h->ctl, h->qd, h->bc, h->atile_off, Q,
+                       h->ch, h->tree, h->rv, h->st_append, h->PP, h->S, h->C, h->T, R1, h->R2);
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][2], h->stream));
+    hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][3], h->stream));
+    hipLaunchKernelGGL(k_frontier, dim3(h->scan_blocks), dim3(256), 0, h->stream, h->ctl, h->qd, h->bc, h->ftile_off,
+                       h->slot_status, Q, h->tree, h->R, h->rv, h->st_frontier, h->PP, R1);
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(h->h_status + (size_t)p * (Q + 1), h->slot_status, (Q + 1) * sizeof(int32_t),
+                        hipMemcpyDeviceToHost, h->stream));
+    return KPAX_OK;
+}
+
+// Captures one iteration per status parity as a HIP graph.  A capture the
+// runtime refuses leaves the handle on direct launches of the same kernels.
+void capture_graphs(kpax_handle* h)
+{
+    h->graph_tried = true;
+    const char* env = std::getenv("KPAX_GRAPH");
+    if(env && env[0] == '0')
+    {
+        h->use_graph = false;
+        return;
+    }
+    for(int p = 0; p < 2; ++p)
+    {
+        hipGraph_t g = nullptr;
+        if(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            h->use_graph = false;
+            return;
+        }
+        const int rc       = enqueue_iteration(h, p, false);
+        const hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if(rc != KPAX_OK || e != hipSuccess || !g ||
+           hipGraphInstantiate(&h->graph_exec[p], g, nullptr, nullptr, 0) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            if(g) (void)hipGraphDestroy(g);
+            h->use_graph = false;
+            return;
+        }
+        (void)hipGraphDestroy(g);
+    }
+}
+
+// First-fit allocator of node ranges in [0, T).
+struct NodeRanges {
+    std::map<uint32_t, uint32_t> free_;  // start -> end
+    explicit NodeRanges(uint32_t T) { free_[0] = T; }
+    bool alloc(uint32_t n, uint32_t* base)
+    {
+        for(auto it = free_.begin(); it != free_.end(); ++it)
+            if(it->second - it->first >= n)
+            {
+                *base               = it->first;
+                const uint32_t end  = it->second;
+                free_.erase(it);
+                if(*base + n < end) free_[*base + n] = end;
+                return true;
+            }
+        return false;
+    }
+    void release(uint32_t base, uint32_t n)
+    {
+        uint32_t lo = base, hi = base + n;
+        auto nx = free_.lower_bound(lo);
+        if(nx != free_.end() && nx->first == hi)
+        {
+            hi = nx->second;
+            nx = free_.erase(nx);
+        }
+        if(nx != free_.begin())
+        {
+            auto pv = std::prev(nx);
+            if(pv->second == lo)
+            {
+                lo = pv->first;
+                free_.erase(pv);
+            }
+        }
+        free_[lo] = hi;
+    }
+};
+
+struct QueryIn {
+    const float* start;
+    const float* goal;
+    uint64_t seed;
+    uint32_t cap;
+    uint32_t env;
+};
+
+void account_iteration(kpax_handle* h, int p)
+{
+    if(h->h_status[(size_t)p * (h->Qmax + 1) + h->Qmax] <= 0) return;  // no slot was running
+    float ms = 0.0f;
+    for(int k = 0; k < 4; ++k)
+        if(hipEventElapsedTime(&ms, h->ph_ev[p][k], h->ph_ev[p][k + 1]) == hipSuccess)
+        {
+            h->stats.phase_ms[k] += ms;
+            h->stats.phase_launches[k] += 1;
+        }
+}
+
+// Runs queries through the handle's slots until every one has finished.
+// Results land in out[i] exactly as a plan of query i alone would return.
+int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
+{
+    const auto t0   = std::chrono::steady_clock::now();
+    const uint32_t Q = h->Qmax;
+    const int S = h->S, C = h->C, D = S + C + 1;
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        memset(&out[i], 0, sizeof(out[i]));
+        out[i].goal_index  = -1;
+        out[i].verified    = -1;
+        out[i].state_dim   = S;
+        out[i].control_dim = C;
+    }
+    std::vector<uint32_t> waiting;
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        const QueryIn& q = qs[i];
+        bool ok          = q.start && q.goal && q.cap >= 1 && q.cap <= h->T && q.env < (uint32_t)kMaxEnvs &&
+                  h->envs[q.env].start != nullptr;
+        for(int d = 0; ok && d < S; ++d) ok = std::isfinite(q.start[d]);
+        for(int d = 0; ok && d < 3; ++d) ok = std::isfinite(q.goal[d]);
+        if(ok)
+            waiting.push_back(i);
+        else
+            out[i].status = KPAX_INVALID_ARG;
+    }
+    HIPC(hipSetDevice(h->cfg.device));
+    if(!h->graph_tried) capture_graphs(h);
+    const bool prof  = h->profiling;
+    const bool graph = h->use_graph && !prof && !h->trace;
+    NodeRanges ranges(h->T);
+    std::vector<int64_t> slot_query(Q, -1);
+    std::vector<uint32_t> slot_base(Q, 0), slot_cap(Q, 0);
+    std::vector<uint64_t> admit_iter(Q, 0);
+    std::vector<uint32_t> free_slots;
+    for(uint32_t s = Q; s-- > 0;) free_slots.push_back(s);  // lowest slot first
+    uint32_t active = 0;
+    uint64_t it     = 0;  // iterations launched by this call
+
+    auto admit = [&](bool first) -> int {
+        uint32_t n_init = 0;
+        for(size_t k = 0; k < waiting.size() && !free_slots.empty();)
+        {
+            const uint32_t qi = waiting[k];
+            uint32_t base     = 0;
+            if(!ranges.alloc(qs[qi].cap, &base))
+            {
+                ++k;
+                continue;
+            }
+            const uint32_t s = free_slots.back();
+            free_slots.pop_back();
+            QueryDev& d = h->h_qd[s];
+            memset(&d, 0, sizeof(d));
+            d.base  = base;
+            d.cap   = qs[qi].cap;
+            d.env   = qs[qi].env;
+            d.query = 0;  // the same draws as a plan of this query alone
+            d.seed  = qs[qi].seed;
+            for(int j = 0; j < 3; ++j) d.goal[j] = qs[qi].goal[j];
+            h->h_init_slots[n_init] = s;
+            for(int j = 0; j < 16; ++j) h->h_init_starts[16 * (size_t)n_init + j] = j < S ? qs[qi].start[j] : 0.0f;
+            ++n_init;
+            slot_query[s] = qi;
+            slot_base[s]  = base;
+            slot_cap[s]   = qs[qi].cap;
+            admit_iter[s] = it + 1;
+            ++active;
+            waiting.erase(waiting.begin() + (long)k);
+        }
+        if(!n_init) return KPAX_OK;
+        if(prof && first) HIPC(hipEventRecord(h->init_ev[0], h->stream));
+        HIPC(hipMemcpyAsync(h->qd, h->h_qd, Q * sizeof(QueryDev), hipMemcpyHostToDevice, h->stream));
+        HIPC(hipMemcpyAsync(h->d_init_slots, h->h_init_slots, n_init * sizeof(uint32_t), hipMemcpyHostToDevice,
+                            h->stream));
+        HIPC(hipMemcpyAsync(h->d_init_starts, h->h_init_starts, 16 * (size_t)n_init * sizeof(float),
+                            hipMemcpyHostToDevice, h->stream));
+        const uint32_t R1 = (uint32_t)h->RP.r1_count;
+        hipLaunchKernelGGL(k_reset_regions,
+                           dim3(std::min<uint32_t>(blocks_for((uint64_t)n_init * h->R2, 256), 8 * h->cu_count)),
+                           dim3(256), 0, h->stream, h->rv, h->d_init_slots, n_init, R1, h->R2);
+        hipLaunchKernelGGL(k_init, dim3(n_init), dim3(256), 0, h->stream, h->ctl, h->qd, h->tree, h->R,
+                           h->slot_status, h->d_init_slots, h->d_init_starts, S, C, h->T, h->RP, h->PP);
+        HIPC(hipGetLastError());
+        if(prof && first) HIPC(hipEventRecord(h->init_ev[1], h->stream));
+        return KPAX_OK;
+    };
+
+    auto harvest = [&](const std::vector<uint32_t>& done) -> int {
+        const uint32_t nd = (uint32_t)done.size();
+        for(uint32_t k = 0; k < nd; ++k) h->h_path_slots[k] = done[k];
+        HIPC(hipMemcpyAsync(h->d_path_slots, h->h_path_slots, nd * sizeof(uint32_t), hipMemcpyHostToDevice,
+                            h->stream));
+        hipLaunchKernelGGL(k_path, dim3(nd), dim3(64), 0, h->stream, h->ctl, h->qd, h->d_path_slots, h->tree, S, C,
+                           h->T, h->cfg.dt, h->path_rows, h->path_steps, h->path_idx, h->path_cap);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(h->h_ctl, h->ctl, Q * sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+        const size_t pc = h->path_cap;
+        for(uint32_t s : done)
+        {
+            HIPC(hipMemcpyAsync(h->h_path_rows + s * pc * D, h->path_rows + s * pc * D, pc * D * sizeof(float),
+                                hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipMemcpyAsync(h->h_path_steps + s * pc, h->path_steps + s * pc, pc * sizeof(int32_t),
+                                hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipMemcpyAsync(h->h_path_idx + s * pc, h->path_idx + s * pc, pc * sizeof(int32_t),
+                                hipMemcpyDeviceToHost, h->stream));
+        }
+        HIPC(hipStreamSynchronize(h->stream));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for(uint32_t s : done)
+        {
+            const Ctl& c     = h->h_ctl[s];
+            kpax_result& r   = out[slot_query[s]];
+            if(c.overflow) return fail(h, KPAX_DEVICE_ERROR, "tree index past the configured capacity");
+            switch(c.nxt.status)
+            {
+                case kDoneFound: r.status = KPAX_OK; break;
+                case kDoneBudget: r.status = KPAX_BUDGET_EXHAUSTED; break;
+                default: r.status = KPAX_NOT_FOUND; break;  // max_iter reached
+            }
+            r.iterations       = c.iterations;
+            r.tree_size        = c.nxt.ts;
+            r.expansions       = c.expansions;
+            r.integrated_steps = c.steps_done;
+            r.plan_ms          = ms;
+            const uint32_t len = c.path_len;
+            if(r.status == KPAX_OK && c.goal != INT_MAX)
+            {
+                r.goal_index = c.goal;
+                r.n_rows     = len;
+                r.rows       = (float*)malloc(std::max<size_t>((size_t)len * D, 1) * sizeof(float));
+                r.steps      = (int32_t*)malloc(std::max<size_t>(len, 1) * sizeof(int32_t));
+                r.node_index = (int32_t*)malloc(std::max<size_t>(len, 1) * sizeof(int32_t));
+                if(!r.rows || !r.steps || !r.node_index) return fail(h, KPAX_NO_MEMORY, "result allocation failed");
+                for(uint32_t k = 0; k < len; ++k)
+                {
+                    const uint32_t src = len - 1 - k;  // root first
+                    memcpy(r.rows + (size_t)k * D, h->h_path_rows + (s * pc + src) * D, D * sizeof(float));
+                    r.steps[k]      = h->h_path_steps[s * pc + src];
+                    r.node_index[k] = h->h_path_idx[s * pc + src];
+                }
+            }
+            h->stats.expansions += c.expansions;
+            h->stats.integrated_steps += c.steps_done;
+            h->stats.plans += 1;
+            ranges.release(slot_base[s], slot_cap[s]);
+            slot_query[s] = -1;
+            free_slots.push_back(s);
+            --active;
+        }
+        // keep the lowest free slot on top
+        std::sort(free_slots.begin(), free_slots.end(), std::greater<uint32_t>());
+        return KPAX_OK;
+    };
+
+    auto newly_done = [&](uint64_t j, std::vector<uint32_t>& done) {
+        const int32_t* st = h->h_status + (size_t)(j & 1) * (Q + 1);
+        for(uint32_t s = 0; s < Q; ++s)
+            if(slot_query[s] >= 0 && admit_iter[s] <= j && st[s] != kRunning &&
+               std::find(done.begin(), done.end(), s) == done.end())
+                done.push_back(s);
+    };
+
+    int rc = admit(true);
+    if(rc != KPAX_OK) return rc;
+    uint64_t accounted = 0;
+    while(active > 0)
+    {
+        ++it;
+        const int p = (int)(it & 1);
+        if(graph)
+        {
+            HIPC(hipGraphLaunch(h->graph_exec[p], h->stream));
+            h->stats.graph_launches += 1;
+        }
+        else if((rc = enqueue_iteration(h, p, prof)) != KPAX_OK)
+            return rc;
+        HIPC(hipEventRecord(h->it_ev[p], h->stream));
+        h->stats.iterations += 1;
+        std::vector<uint32_t> done;
+        if(h->trace)
+        {
+            HIPC(hipStreamSynchronize(h->stream));
+            HIPC(hipMemcpy(h->h_ctl, h->ctl, Q * sizeof(Ctl), hipMemcpyDeviceToHost));
+            for(uint32_t s = 0; s < std::min<uint32_t>(Q, 4); ++s)
+            {
+                const Iter& x = h->h_ctl[s].nxt;
+                std::fprintf(stderr, "kpax: slot %u after iteration %llu: next t=%u ts=%u W=%u M=%u fan=%u status=%d\n",
+                             s, (unsigned long long)it, x.t, x.ts, x.W, x.M, x.fan, x.status);
+            }
+        }
+        // One iteration stays queued behind the one whose statuses are read,
+        // so the GPU never waits for the host; a slot that finished is
+        // skipped by the iterations queued after it.
+        if(it >= 2)
+        {
+            HIPC(hipEventSynchronize(h->it_ev[(it - 1) & 1]));
+            if(prof && accounted < it - 1) account_iteration(h, (int)((it - 1) & 1));
+            accounted = std::max(accounted, it - 1);
+            newly_done(it - 1, done);
+        }
+        if(!done.empty())
+        {
+            HIPC(hipStreamSynchronize(h->stream));
+            if(prof && accounted < it) account_iteration(h, p);
+            accounted = it;
+            newly_done(it, done);
+            if((rc = harvest(done)) != KPAX_OK) return rc;
+            if((rc = admit(false)) != KPAX_OK) return rc;
+        }
+    }
+    HIPC(hipStreamSynchronize(h->stream));
+    if(prof)
+    {
+        float ms = 0.0f;
+        if(hipEventElapsedTime(&ms, h->init_ev[0], h->init_ev[1]) == hipSuccess)
+        {
+            h->stats.phase_ms[4] += ms;
+            h->stats.phase_launches[4] += 1;
+        }
+    }
+    // FP64 re-verification of every returned trajectory (host, after timing)
+    if(h->cfg.verify)
+    {
+        std::vector<uint32_t> todo;
+        for(uint32_t i = 0; i < n; ++i)
+            if(out[i].n_rows > 0) todo.push_back(i);
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for(size_t k; (k = next.fetch_add(1)) < todo.size();)
+            {
+                kpax_result& r   = out[todo[k]];
+                const QueryIn& q = qs[todo[k]];
+                const Env& e     = h->envs[q.env];
+                float dev        = 0.0f;
+                const bool ok = verify_path_fp64(h->cfg, e.host.data(), e.n_obs, r.rows, r.steps, r.n_rows, q.goal,
+                                                 &dev);
+                r.verified       = ok ? 1 : 0;
+                r.verify_max_dev = dev;
+            }
+        };
+        const unsigned nt = std::min<unsigned>((unsigned)todo.size(),
+                                               std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+        if(nt <= 1)
+            work();
+        else
+        {
+            std::vector<std::thread> pool;
+            for(unsigned k = 0; k < nt; ++k) pool.emplace_back(work);
+            for(auto& th : pool) th.join();
+        }
+    }
+    return KPAX_OK;
+}
+
+void drop_graphs(kpax_handle* h)
+{
+    for(auto& e : h->graph_exec)
+        if(e)
+        {
+            (void)hipGraphExecDestroy(e);
+            e = nullptr;
+        }
+    h->graph_tried = false;
+}
+
+int apply_launch(kpax_handle* h, const kpax_launch& l)
+{
+    if(l.expand_blocks < 1 || l.expand_blocks > (1u << 20) || l.expand_chunk < 64 || l.expand_chunk > 4096 ||
+       l.expand_chunk % 64 != 0 || l.scan_blocks < 1 || l.scan_blocks > (1u << 20))
+        return fail(h, KPAX_INVALID_ARG, "launch shape out of range");
+    HIPC(hipStreamSynchronize(h->stream));
+    h->expand_blocks = l.expand_blocks;
+    h->expand_chunk  = l.expand_chunk;
+    h->scan_blocks   = l.scan_blocks;
+    h->use_graph     = l.use_graph != 0;
+    drop_graphs(h);
+    const int rc = upload_expand_args(h);
+    if(rc == KPAX_OK && !l.use_graph) h->graph_tried = true;  // stay on direct launches
+    return rc;
+}
+
+kpax_launch current_launch(const kpax_handle* h)
+{
+    kpax_launch l;
+    l.expand_blocks = h->expand_blocks;
+    l.expand_chunk  = h->expand_chunk;
+    l.scan_blocks   = h->scan_blocks;
+    l.use_graph     = h->use_graph ? 1 : 0;
+    return l;
+}
+
+int check_obstacles(kpax_handle* h, const float* aabb6, uint32_t n)
+{
+    if(n && !aabb6) return fail(h, KPAX_INVALID_ARG, "obstacles pointer is NULL");
+    for(uint32_t i = 0; i < n; ++i)
+        for(int d = 0; d < 3; ++d)
+        {
+            const float a = aabb6[6 * (size_t)i + d], b = aabb6[6 * (size_t)i + 3 + d];
+            if(!std::isfinite(a) || !std::isfinite(b) || a > b)
+                return fail(h, KPAX_INVALID_ARG, "obstacle " + std::to_string(i) + " is not a finite min<=max box");
+        }
+    return KPAX_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t node_capacity, kpax_handle** out)
+{
+    if(!cfg || !out) return KPAX_INVALID_ARG;
+    *out = nullptr;
+    if(max_queries < 1 || max_queries > kMaxSlots || node_capacity < 1 || node_capacity > (1ull << 31))
+        return KPAX_INVALID_ARG;
+    if(!validate_config(*cfg).empty()) return KPAX_INVALID_ARG;
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) return KPAX_NO_DEVICE;
+    kpax_handle* h = new(std::nothrow) kpax_handle();
+    if(!h) return KPAX_NO_MEMORY;
+    h->cfg   = *cfg;
+    h->trace = std::getenv("KPAX_TRACE") != nullptr;
+    h->S     = state_dim(cfg->model);
+    h->C     = control_dim(cfg->model);
+    h->P     = make_model_params(*cfg);
+    h->RP    = make_region_params(*cfg);
+    h->GPm   = make_grid_params(*cfg);
+    h->PP    = make_plan_params(*cfg);
+    h->T     = (uint32_t)node_capacity;
+    h->Qmax  = max_queries;
+    h->G     = (uint32_t)h->GPm.D[0] * h->GPm.D[1] * h->GPm.D[2];
+    h->R2    = (uint32_t)h->RP.r1_count * h->RP.r2_per;
+    h->envs.resize(kMaxEnvs);
+    int rc = KPAX_OK;
+#define TRY(x)               \
+    do                       \
+    {                        \
+        rc = (x);            \
+        if(rc != KPAX_OK)    \
+        {                    \
+            kpax_destroy(h); \
+            return rc;       \
+        }                    \
+    } while(0)
+    if(hipSetDevice(cfg->device) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    hipDeviceProp_t prop;
+    if(hipGetDeviceProperties(&prop, cfg->device) == hipSuccess && prop.multiProcessorCount > 0)
+        h->cu_count = prop.multiProcessorCount;
+    if(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    h->expand_blocks = expand_blocks(h);
+    h->expand_chunk  = 64;
+    const uint32_t Q = h->Qmax;
+    const size_t T   = h->T;
+    h->scan_blocks   = std::min<uint32_t>(blocks_for(T, kTile) + Q, 4 * h->cu_count);
+    TRY(dalloc(h, &h->tree.st, T * h->S));
+    TRY(dalloc(h, &h->tree.ct, T * h->C));
+    TRY(dalloc(h, &h->tree.steps, T));
+    TRY(dalloc(h, &h->tree.parent, T));
+    TRY(dalloc(h, &h->tree.reg, T));
+    TRY(dalloc(h, &h->ch.st, T * h->S));
+    TRY(dalloc(h, &h->ch.ct, T * h->C));
+    TRY(dalloc(h, &h->ch.steps, T));
+    TRY(dalloc(h, &h->ch.parent, T));
+    TRY(dalloc(h, &h->ch.reg, T));
+    TRY(dalloc(h, &h->ch.r2, T));
+    TRY(dalloc(h, &h->ch.adm, T));
+    TRY(dalloc(h, &h->R, T));
+    const size_t r1 = (size_t)h->RP.r1_count * Q, r2 = (size_t)h->R2 * Q;
+    TRY(dalloc(h, &h->rv.valid, r1));
+    TRY(dalloc(h, &h->rv.total, r1));
+    TRY(dalloc(h, &h->rv.covered, r1));
+    TRY(dalloc(h, &h->rv.score, r1));
+    TRY(dalloc(h, &h->rv.r2_state, r2));
+    TRY(dalloc(h, &h->ctl, Q));
+    TRY(dalloc(h, &h->qd, Q));
+    TRY(dalloc(h, &h->bc, 1));
+    TRY(dalloc(h, &h->child_off, Q + 1));
+    TRY(dalloc(h, &h->atile_off, Q + 1));
+    TRY(dalloc(h, &h->ftile_off, Q + 1));
+    TRY(dalloc(h, &h->slot_status, Q + 1));
+    h->n_status = (uint32_t)(T / kTile + Q + 2);
+    TRY(dalloc(h, &h->st_append, h->n_status));
+    TRY(dalloc(h, &h->st_frontier, h->n_status));
+    TRY(dalloc(h, &h->d_init_slots, Q));
+    TRY(dalloc(h, &h->d_init_starts, 16 * (size_t)Q));
+    TRY(dalloc(h, &h->d_path_slots, Q));
+    TRY(dalloc(h, &h->envs_dev, kMaxEnvs));
+    h->path_cap     = cfg->max_iter + 2;
+    const size_t pc = (size_t)Q * h->path_cap;
+    const int D     = h->S + h->C + 1;
+    TRY(dalloc(h, &h->path_rows, pc * D));
+    TRY(dalloc(h, &h->path_steps, pc));
+    TRY(dalloc(h, &h->path_idx, pc));
+    TRY(dalloc(h, (char**)&h->expand_args, sizeof(ExpandArgs)));
+    auto pinned = [&](void** p, size_t bytes) {
+        return hipHostMalloc(p, std::max<size_t>(bytes, 1), hipHostMallocDefault) == hipSuccess ? KPAX_OK
+                                                                                                : KPAX_NO_MEMORY;
+    };
+    TRY(pinned((void**)&h->h_status, 2 * (Q + 1) * sizeof(int32_t)));
+    TRY(pinned((void**)&h->h_ctl, Q * sizeof(Ctl)));
+    TRY(pinned((void**)&h->h_qd, Q * sizeof(QueryDev)));
+    TRY(pinned((void**)&h->h_init_slots, Q * sizeof(uint32_t)));
+    TRY(pinned((void**)&h->h_init_starts, 16 * (size_t)Q * sizeof(float)));
+    TRY(pinned((void**)&h->h_path_slots, Q * sizeof(uint32_t)));
+    TRY(pinned((void**)&h->h_path_rows, pc * D * sizeof(float)));
+    TRY(pinned((void**)&h->h_path_steps, pc * sizeof(int32_t)));
+    TRY(pinned((void**)&h->h_path_idx, pc * sizeof(int32_t)));
+    for(int i = 0; i < 2; ++i)
+    {
+        if(hipEventCreateWithFlags(&h->it_ev[i], hipEventDisableTiming) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+        if(hipEventCreate(&h->init_ev[i]) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+        for(int k = 0; k < 6; ++k)
+            if(hipEventCreate(&h->ph_ev[i][k]) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    }
+    // every slot idle; scan state at epoch 0 (never current)
+    std::vector<Ctl> c0(Q);
+    for(Ctl& c : c0)
+    {
+        memset(&c, 0, sizeof(c));
+        c.cur.status = kNotStarted;
+        c.nxt.status = kNotStarted;
+        c.goal       = INT_MAX;
+    }
+    if(hipMemcpy(h->ctl, c0.data(), Q * sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    std::vector<int32_t> s0(Q + 1, kNotStarted);
+    s0[Q] = 0;
+    if(hipMemcpy(h->slot_status, s0.data(), (Q + 1) * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+        TRY(KPAX_DEVICE_ERROR);
+    if(hipMemset(h->bc, 0, sizeof(BatchCtl)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    if(hipMemset(h->qd, 0, Q * sizeof(QueryDev)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    if(hipMemset(h->st_append, 0, h->n_status * sizeof(uint64_t)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    if(hipMemset(h->st_frontier, 0, h->n_status * sizeof(uint64_t)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    if(hipMemset(h->envs_dev, 0, kMaxEnvs * sizeof(GridView)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
+    TRY(upload_expand_args(h));
+    // an empty environment 0 until kpax_set_obstacles
+    TRY(build_grid(h, 0, nullptr, 0));
+#undef TRY
+    *out = h;
+    return KPAX_OK;
+}
+
+int kpax_create(const kpax_config* cfg, kpax_handle** out)
+{
+    if(!cfg) return KPAX_INVALID_ARG;
+    return kpax_create_batch(cfg, 1, cfg->max_tree, out);
+}
+
+int kpax_destroy(kpax_handle* h)
+{
+    if(!h) return KPAX_INVALID_ARG;
+    if(h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graphs(h);
+    for(void* p : h->allocs) (void)hipFree(p);
+    for(Env& e : h->envs) free_env(e);
+    void* pinned[] = {h->h_status,     h->h_ctl,       h->h_qd,         h->h_init_slots, h->h_init_starts,
+                      h->h_path_slots, h->h_path_rows, h->h_path_steps, h->h_path_idx};
+    for(void* p : pinned)
+        if(p) (void)hipHostFree(p);
+    for(int i = 0; i < 2; ++i)
+    {
+        if(h->it_ev[i]) (void)hipEventDestroy(h->it_ev[i]);
+        if(h->init_ev[i]) (void)hipEventDestroy(h->init_ev[i]);
+        for(hipEvent_t e : h->ph_ev[i])
+            if(e) (void)hipEventDestroy(e);
+    }
+    if(h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return KPAX_OK;
+}
+
+const char* kpax_last_error(const kpax_handle* h)
+{
+    if(!h) return "null handle";
+    return h->err.c_str();
+}
+
+int kpax_set_environment(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n)
+{
+    if(!h) return KPAX_INVALID_ARG;
+    if(env >= (uint32_t)kMaxEnvs) return fail(h, KPAX_INVALID_ARG, "environment index out of range");
+    const int rc = check_obstacles(h, aabb6, n);
+    if(rc != KPAX_OK) return rc;
+    HIPC(hipSetDevice(h->cfg.device));
+    return build_grid(h, env, aabb6, n);
+}
+
+int kpax_set_obstacles(kpax_handle* h, const float* aabb6, uint32_t n)
+{
+    return kpax_set_environment(h, 0, aabb6, n);
+}
+
+int kpax_plan(kpax_handle* h, const float* start, const float* goal, uint64_t seed, kpax_result* out)
+{
+    if(!h || !start || !goal || !out) return fail(h, KPAX_INVALID_ARG, "null argument");
+    QueryIn q{start, goal, seed, std::min<uint32_t>(h->cfg.max_tree, h->T), 0};
+    const int rc = run_queries(h, &q, 1, out);
+    if(rc != KPAX_OK) return rc;
+    if(out->status == KPAX_INVALID_ARG) return fail(h, KPAX_INVALID_ARG, "start or goal is not finite");
+    h->last_tree = out->tree_size;
+    h->planned   = true;
+    return KPAX_OK;
+}
+
+int kpax_plan_batch(kpax_handle* h, const kpax_query* queries, uint32_t n, kpax_result* out)
+{
+    if(!h || (n && (!queries || !out))) return fail(h, KPAX_INVALID_ARG, "null argument");
+    if(n == 0) return KPAX_OK;
+    std::vector<QueryIn> qs(n);
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        const kpax_query& q = queries[i];
+        qs[i]               = QueryIn{q.start, q.goal, q.seed, q.budget ? q.budget : h->cfg.max_tree, q.env};
+    }
+    const int rc = run_queries(h, qs.data(), n, out);
+    h->planned   = false;  // slot 0's tree is no longer one single plan's
+    return rc;
+}
+
+void kpax_result_free(kpax_result* r)
+{
+    if(!r) return;
+    free(r->rows);
+    free(r->steps);
+    free(r->node_index);
+    r->rows       = nullptr;
+    r->steps      = nullptr;
+    r->node_index = nullptr;
+    r->n_rows     = 0;
+}
+
+int kpax_get_tree(kpax_handle* h, uint32_t cap, float* states, float* controls, int32_t* steps, int32_t* parents,
+                  int32_t* regions, uint32_t* tree_size)
+{
+    if(!h) return KPAX_INVALID_ARG;
+    if(!h->planned) return fail(h, KPAX_INVALID_ARG, "no single plan has run on this handle");
+    const uint32_t ts = h->last_tree;
+    const size_t T    = h->T;  // a single plan occupies slot 0 at node 0
+    if(tree_size) *tree_size = ts;
+    if(cap < ts && (states || controls)) return fail(h, KPAX_INVALID_ARG, "cap is smaller than the tree");
+    HIPC(hipSetDevice(h->cfg.device));
+    for(int d = 0; states && d < h->S; ++d)
+        HIPC(hipMemcpy(states + (size_t)d * cap, h->tree.st + (size_t)d * T, (size_t)ts * 4, hipMemcpyDeviceToHost));
+    for(int d = 0; controls && d < h->C; ++d)
+        HIPC(hipMemcpy(controls + (size_t)d * cap, h->tree.ct + (size_t)d * T, (size_t)ts * 4,
+                       hipMemcpyDeviceToHost));
+    if(steps) HIPC(hipMemcpy(steps, h->tree.steps, (size_t)ts * 4, hipMemcpyDeviceToHost));
+    if(parents) HIPC(hipMemcpy(parents, h->tree.parent, (size_t)ts * 4, hipMemcpyDeviceToHost));
+    if(regions) HIPC(hipMemcpy(regions, h->tree.reg, (size_t)ts * 4, hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
+int kpax_get_regions(kpax_handle* h, int32_t* valid, int32_t* total, int32_t* covered, float* scores)
+{
+    if(!h) return KPAX_INVALID_ARG;
+    HIPC(hipSetDevice(h->cfg.device));
+    const size_t n = (size_t)h->RP.r1_count * 4;  // slot 0
+    if(valid) HIPC(hipMemcpy(valid, h->rv.valid, n, hipMemcpyDeviceToHost));
+    if(total) HIPC(hipMemcpy(total, h->rv.total, n, hipMemcpyDeviceToHost));
+    if(covered) HIPC(hipMemcpy(covered, h->rv.covered, n, hipMemcpyDeviceToHost));
+    if(scores) HIPC(hipMemcpy(scores, h->rv.score, n, hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
+int kpax_set_profiling(kpax_handle* h, int32_t on)
+{
+    if(!h) return KPAX_INVALID_ARG;
+    h->profiling = on != 0;
+    return KPAX_OK;
+}
+
+int kpax_get_stats(kpax_handle* h, kpax_stats* out)
+{
+    if(!h || !out) return KPAX_INVALID_ARG;
+    *out = h->stats;
+    return KPAX_OK;
+}
+
+int kpax_reset_stats(kpax_handle* h)
+{
+    if(!h) return KPAX_INVALID_ARG;
+    memset(&h->stats, 0, sizeof(h->stats));
+    return KPAX_OK;
+}
+
+int kpax_get_launch(kpax_handle* h, kpax_launch* out)
+{
+    if(!h || !out) return KPAX_INVALID_ARG;
+    *out = current_launch(h);
+    return KPAX_OK;
+}
+
+int kpax_set_launch(kpax_handle* h, const kpax_launch* launch)
+{
+    if(!h || !launch) return KPAX_INVALID_ARG;
+    HIPC(hipSetDevice(h->cfg.device));
+    return apply_launch(h, *launch);
+}
+
+// Coordinate search over the launch shape: each knob in turn takes the
+// candidate whose best-of-`repeats` wall time over the sample batch is
+// lowest, the others held at their current best.
+int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_t repeats, kpax_launch* best)
+{
+    if(!h || !queries || n == 0) return fail(h, KPAX_INVALID_ARG, "autotune needs sample queries");
+    repeats = std::max<uint32_t>(repeats, 1);
+    HIPC(hipSetDevice(h->cfg.device));
+    const bool prof = h->profiling;
+    h->profiling    = false;
+    std::vector<kpax_result> ref(n), got(n);
+    auto run = [&](const kpax_launch& l, double* ms, std::vector<kpax_result>& out) -> int {
+        int rc = apply_launch(h, l);
+        if(rc != KPAX_OK) return rc;
+        *ms = 1e300;
+        for(uint32_t r = 0; r < repeats; ++r)
+        {
+            for(auto& x : out) kpax_result_free(&x);
+            const auto t0 = std::chrono::steady_clock::now();
+            if((rc = kpax_plan_batch(h, queries, n, out.data())) != KPAX_OK) return rc;
+            *ms = std::min(*ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        return KPAX_OK;
+    };
+    auto same = [&](const std::vector<kpax_result>& a, const std::vector<kpax_result>& b) {
+        for(uint32_t i = 0; i < n; ++i)
+            if(a[i].status != b[i].status || a[i].tree_size != b[i].tree_size ||
+               a[i].iterations != b[i].iterations || a[i].expansions != b[i].expansions ||
+               a[i].goal_index != b[i].goal_index)
+                return false;
+        return true;
+    };
+    const uint32_t cu = (uint32_t)h->cu_count;
+    const uint32_t occ = std::max<uint32_t>(1, expand_blocks(h) / cu);
+    kpax_launch cur = current_launch(h);
+    cur.expand_blocks = occ * cu;
+    double cur_ms     = 0.0;
+    int rc            = run(cur, &cur_ms, ref);
+    std::vector<std::vector<kpax_launch>> knobs(4);
+    for(uint32_t k = 1; k <= occ; ++k) knobs[0].push_back(kpax_launch{k * cu, 0, 0, 0});
+    for(uint32_t c : {64u, 128u, 256u, 512u}) knobs[1].push_back(kpax_launch{0, c, 0, 0});
+    for(uint32_t m : {1u, 2u, 4u, 8u}) knobs[2].push_back(kpax_launch{0, 0, m * cu, 0});
+    for(int g : {1, 0}) knobs[3].push_back(kpax_launch{0, 0, 0, g});
+    for(int k = 0; rc == KPAX_OK && k < 4; ++k)
+        for(const kpax_launch& v : knobs[k])
+        {
+            kpax_launch cand = cur;
+            if(k == 0) cand.expand_blocks = v.expand_blocks;
+            if(k == 1) cand.expand_chunk = v.expand_chunk;
+            if(k == 2) cand.scan_blocks = v.scan_blocks;
+            if(k == 3) cand.use_graph = v.use_graph;
+            if(memcmp(&cand, &cur, sizeof(cand)) == 0) continue;
+            double ms = 0.0;
+            if((rc = run(cand, &ms, got)) != KPAX_OK) break;
+            if(!same(ref, got))
+            {
+                rc = fail(h, KPAX_DEVICE_ERROR, "a launch shape changed the planner's results");
+                break;
+            }
+            if(ms < cur_ms)
+            {
+                cur    = cand;
+                cur_ms = ms;
+            }
+        }
+    for(auto& x : ref) kpax_result_free(&x);
+    for(auto& x : got) kpax_result_free(&x);
+    h->profiling = prof;
+    if(rc != KPAX_OK) return rc;
+    rc = apply_launch(h, cur);
+    if(rc == KPAX_OK && best) *best = cur;
+    return rc;
+}
+
+int kpax_scores(kpax_handle* h, const int32_t* valid_count, const int32_t* total, const int32_t* covered, float* scores)
+{
+    if(!h || !valid_count || !total || !covered || !scores) return fail(h, KPAX_INVALID_ARG, "null argument");
+    HIPC(hipSetDevice(h->cfg.device));
+    const size_t r1 = (size_t)h->RP.r1_count;
+    Scratch sc;
+    RegionView rv{};
+    rv.valid   = sc.get<int32_t>(r1);
+    rv.total   = sc.get<int32_t>(r1);
+    rv.covered = sc.get<int32_t>(r1);
+    rv.score   = sc.get<float>(r1);
+    if(!rv.valid || !rv.total || !rv.covered || !rv.score) return fail(h, KPAX_NO_MEMORY, "scratch allocation failed");
+    HIPC(hipMemcpy(rv.valid, valid_count, r1 * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(rv.total, total, r1 * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(rv.covered, covered, r1 * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_score, dim3(1), dim3(1024), 0, h->stream, (Ctl*)nullptr, rv, h->RP, h->PP.eps);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+    HIPC(hipMemcpy(scores, rv.score, r1 * 4, hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
+}  // extern "C"

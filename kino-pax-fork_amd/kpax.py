@@ -21,7 +21,7 @@ LIB_PATH = os.environ.get("KPAX_LIB", os.path.join(HERE, "libkpax.so"))
 
 STATUS = {0: "OK", 1: "NOT_FOUND", 2: "BUDGET_EXHAUSTED", -1: "INVALID_ARG", -2: "DEVICE_ERROR",
           -3: "NO_MEMORY", -4: "NO_DEVICE", -5: "IO_ERROR"}
-OK, NOT_FOUND, BUDGET_EXHAUSTED = 0, 1, 2
+OK, NOT_FOUND, BUDGET_EXHAUSTED, INVALID_ARG = 0, 1, 2, -1
 MODEL_DOUBLE_INTEGRATOR, MODEL_DUBINS_AIRPLANE, MODEL_QUADCOPTER = 1, 2, 3
 N_PHASES = 5
 PHASES = ("expand", "append", "score", "frontier", "init")
@@ -81,7 +81,18 @@ class _Result(ctypes.Structure):
 
 class _Stats(ctypes.Structure):
     _fields_ = [("phase_ms", ctypes.c_double * N_PHASES), ("phase_launches", ctypes.c_uint64 * N_PHASES),
-                ("expansions", ctypes.c_uint64), ("integrated_steps", ctypes.c_uint64), ("plans", ctypes.c_uint64)]
+                ("expansions", ctypes.c_uint64), ("integrated_steps", ctypes.c_uint64), ("plans", ctypes.c_uint64),
+                ("iterations", ctypes.c_uint64), ("graph_launches", ctypes.c_uint64)]
+
+
+class _Launch(ctypes.Structure):
+    _fields_ = [("expand_blocks", ctypes.c_uint32), ("expand_chunk", ctypes.c_uint32),
+                ("scan_blocks", ctypes.c_uint32), ("use_graph", ctypes.c_int32)]
+
+
+class _Query(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_void_p), ("goal", ctypes.c_void_p), ("seed", ctypes.c_uint64),
+                ("budget", ctypes.c_uint32), ("env", ctypes.c_uint32)]
 
 
 class KpaxError(RuntimeError):
@@ -105,6 +116,12 @@ def lib():
         "kpax_config_workload": (i32, [i32, P]),
         "kpax_model_dims": (i32, [i32, P, P]),
         "kpax_create": (i32, [P, P]),
+        "kpax_create_batch": (i32, [P, u32, u64, P]),
+        "kpax_set_environment": (i32, [P, u32, P, u32]),
+        "kpax_plan_batch": (i32, [P, P, u32, P]),
+        "kpax_get_launch": (i32, [P, P]),
+        "kpax_set_launch": (i32, [P, P]),
+        "kpax_autotune": (i32, [P, P, u32, u32, P]),
         "kpax_destroy": (i32, [P]),
         "kpax_last_error": (ctypes.c_char_p, [P]),
         "kpax_set_obstacles": (i32, [P, P, u32]),
@@ -239,15 +256,44 @@ class PlanResult:
         return self.rows[::-1].copy()
 
 
-class KPAX:
-    """MI355X Kino-PAX planner (mirrors the reference's KPAX class)."""
+def _plan_result(r, S, C):
+    D = S + C + 1
+    n = r.n_rows
+    rows = np.ctypeslib.as_array(r.rows, (n * D,)).reshape(n, D).copy() if n else np.zeros((0, D), np.float32)
+    steps = np.ctypeslib.as_array(r.steps, (n,)).copy() if n else np.zeros(0, np.int32)
+    idx = np.ctypeslib.as_array(r.node_index, (n,)).copy() if n else np.zeros(0, np.int32)
+    return PlanResult(r.status, rows, steps, idx, r.goal_index, r.iterations, r.tree_size, r.expansions,
+                      r.integrated_steps, r.plan_ms, r.verified, r.verify_max_dev, S, C)
 
-    def __init__(self, config=2):
+
+@dataclass
+class Query:
+    """One query of plan_batch: start state, goal centre, seed, tree budget (0 = config max_tree), environment."""
+    start: np.ndarray
+    goal: np.ndarray
+    seed: int = 0
+    budget: int = 0
+    env: int = 0
+
+
+class KPAX:
+    """MI355X Kino-PAX planner (mirrors the reference's KPAX class).
+
+    KPAX(config) plans one query at a time; KPAX(config, max_queries=Q,
+    node_capacity=T) holds Q query slots over a pool of T tree nodes for
+    plan_batch."""
+
+    def __init__(self, config=2, max_queries=None, node_capacity=None):
         cfg = config_workload(config) if isinstance(config, int) else config.copy()
         self.config = cfg
         self.S, self.C = model_dims(cfg.model)
         h = ctypes.c_void_p()
-        rc = lib().kpax_create(ctypes.byref(cfg), ctypes.byref(h))
+        if max_queries is None and node_capacity is None:
+            rc = lib().kpax_create(ctypes.byref(cfg), ctypes.byref(h))
+        else:
+            q = int(max_queries or 1)
+            t = int(node_capacity or cfg.max_tree * q)
+            rc = lib().kpax_create_batch(ctypes.byref(cfg), q, t, ctypes.byref(h))
         if rc != OK:
             raise KpaxError(f"kpax_create failed: {STATUS.get(rc, rc)}")
         self._h = h
@@ -298,15 +344,57 @@ class KPAX:
         g = _f32(np.asarray(goal, np.float32).ravel()[:3])
         r = _Result()
         self._check(lib().kpax_plan(self._h, _p(x0), _p(g), int(seed) & (2**64 - 1), ctypes.byref(r)), "plan")
-        D = self.S + self.C + 1
-        n = r.n_rows
-        rows = np.ctypeslib.as_array(r.rows, (n * D,)).reshape(n, D).copy() if n else np.zeros((0, D), np.float32)
-        steps = np.ctypeslib.as_array(r.steps, (n,)).copy() if n else np.zeros(0, np.int32)
-        idx = np.ctypeslib.as_array(r.node_index, (n,)).copy() if n else np.zeros(0, np.int32)
-        out = PlanResult(r.status, rows, steps, idx, r.goal_index, r.iterations, r.tree_size, r.expansions,
-                         r.integrated_steps, r.plan_ms, r.verified, r.verify_max_dev, self.S, self.C)
+        out = _plan_result(r, self.S, self.C)
         lib().kpax_result_free(ctypes.byref(r))
         return out
+
+    def set_environment(self, env, obstacles):
+        """Obstacles of environment `env` for batched queries (environment 0 = set_obstacles)."""
+        obs = _f32(obstacles).reshape(-1, 6)
+        self._check(lib().kpax_set_environment(self._h, int(env), _p(obs), obs.shape[0]), "set_environment")
+        if env == 0:
+            self._obs = obs
+
+    def _queries(self, queries):
+        keep = []
+        arr = (_Query * max(len(queries), 1))()
+        for i, q in enumerate(queries):
+            x0 = _f32(np.asarray(q.start, np.float32).ravel()[: self.S])
+            g = _f32(np.asarray(q.goal, np.float32).ravel()[:3])
+            keep += [x0, g]
+            arr[i] = _Query(x0.ctypes.data, g.ctypes.data, int(q.seed) & (2**64 - 1), int(q.budget), int(q.env))
+        return arr, keep
+
+    def plan_batch(self, queries):
+        """Plan a list of Query on the handle's slots -> list of PlanResult (same order)."""
+        n = len(queries)
+        arr, keep = self._queries(queries)
+        res = (_Result * max(n, 1))()
+        self._check(lib().kpax_plan_batch(self._h, arr, n, res), "plan_batch")
+        out = []
+        for i in range(n):
+            out.append(_plan_result(res[i], self.S, self.C))
+            lib().kpax_result_free(ctypes.byref(res[i]))
+        return out
+
+    def launch(self):
+        """Launch shape: expand_blocks, expand_chunk, scan_blocks, use_graph."""
+        l = _Launch()
+        self._check(lib().kpax_get_launch(self._h, ctypes.byref(l)), "get_launch")
+        return {k: getattr(l, k) for k, _ in _Launch._fields_}
+
+    def set_launch(self, **kw):
+        cur = self.launch()
+        cur.update(kw)
+        l = _Launch(*(int(cur[k]) for k, _ in _Launch._fields_))
+        self._check(lib().kpax_set_launch(self._h, ctypes.byref(l)), "set_launch")
+
+    def autotune(self, queries, repeats=2):
+        """Pick the fastest launch shape on sample queries; returns it (and applies it)."""
+        arr, keep = self._queries(queries)
+        l = _Launch()
+        self._check(lib().kpax_autotune(self._h, arr, len(queries), int(repeats), ctypes.byref(l)), "autotune")
+        return {k: getattr(l, k) for k, _ in _Launch._fields_}
 
     def tree(self):
         """Tree of the last plan: states (S, n), controls (C, n), steps, parents, regions."""
@@ -346,7 +434,8 @@ class KPAX:
         self._check(lib().kpax_get_stats(self._h, ctypes.byref(s)), "get_stats")
         return {"phase_ms": dict(zip(PHASES, list(s.phase_ms))),
                 "phase_launches": dict(zip(PHASES, list(s.phase_launches))),
-                "expansions": s.expansions, "integrated_steps": s.integrated_steps, "plans": s.plans}
+                "expansions": s.expansions, "integrated_steps": s.integrated_steps, "plans": s.plans,
+                "iterations": s.iterations, "graph_launches": s.graph_launches}
 
     def reset_stats(self):
         self._check(lib().kpax_reset_stats(self._h), "reset_stats")

@@ -217,6 +217,16 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t)
 }
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+// graph capture is not emulated: the library then launches the same kernels directly
+typedef struct emu_graph_t* hipGraph_t;
+typedef struct emu_graph_exec_t* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorUnknown; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return hipErrorUnknown; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, void*, size_t) { return hipErrorUnknown; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorUnknown; }
 inline const char* hipGetErrorString(hipError_t) { return "emulated"; }
 
 // Persistent workers, one per work-item slot of the largest work-group.

@@ -88,5 +88,61 @@ def main():
             check(r.status == kpax.OK and r.verified == 1 and r.goal_index == o["goal_index"],
                   f"plan workload {k} reaches the goal with a verified path")
 
+    batch_checks()
+
+
+def oracle_path(o, cfg):
+    """Root -> goal rows [state, control, dt * steps] of an oracle plan."""
+    j, nodes = o["goal_index"], []
+    while j != -1:
+        nodes.append(j)
+        j = int(o["parents"][j])
+    nodes.reverse()
+    S, C = cfg.S, cfg.C
+    rows = np.zeros((len(nodes), S + C + 1), np.float32)
+    for r, j in enumerate(nodes):
+        rows[r, :S] = o["states"][:, j]
+        rows[r, S:S + C] = o["controls"][:, j]
+        rows[r, S + C] = np.float32(cfg.dt) * np.float32(o["steps"][j])
+    return rows
+
+
+def batch_checks():
+    """Uneven budgets over two environments on three slots: queries wait for
+    node ranges, reuse the ranges and slots of finished ones, and each result
+    equals the oracle's plan of that query alone."""
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=1500, max_iter=10)
+    small(cfg)
+    obs1 = kpax.generate_environment(2, 99, 60, cfg.ws_min, cfg.ws_max, start[:3], list(goal[:3]) + [cfg.goal_radius])
+    envs = [obs, obs1]
+    near = np.array([1.6, 0.5, 1.0], np.float32)
+    qs = [kpax.Query(start, near, seed=11, budget=1024, env=0),
+          kpax.Query(start, goal, seed=12, budget=700, env=1),
+          kpax.Query(start, near, seed=13, budget=1500, env=1),
+          kpax.Query(start, goal, seed=14, budget=1, env=0),
+          kpax.Query(start, near, seed=15, budget=900, env=0)]
+    with kpax.KPAX(cfg, max_queries=3, node_capacity=3000) as p:
+        p.set_environment(0, obs)
+        p.set_environment(1, obs1)
+        res = p.plan_batch(qs)
+        p.set_launch(expand_blocks=3, expand_chunk=192, scan_blocks=2)
+        res2 = p.plan_batch(qs)
+    check(all(a.status == b.status and a.tree_size == b.tree_size and a.expansions == b.expansions
+              and a.integrated_steps == b.integrated_steps and np.array_equal(a.rows, b.rows)
+              for a, b in zip(res, res2)), "another launch shape gives the same batch results")
+    for i, (q, r) in enumerate(zip(qs, res)):
+        c = cfg.copy()
+        c.max_tree = q.budget
+        o = oracle.plan(c, envs[q.env], q.start, q.goal, seed=q.seed, threads=4)
+        same = (r.status == o["status"] and r.iterations == o["iterations"] and r.tree_size == o["tree_size"]
+                and r.goal_index == o["goal_index"] and r.expansions == o["expansions"])
+        if same and r.status == kpax.OK:
+            same = np.array_equal(r.rows.view(np.uint32), oracle_path(o, c).view(np.uint32)) and r.verified == 1
+        check(same, f"batch query {i} (budget {q.budget}, env {q.env}): status {r.status} iterations "
+                    f"{r.iterations} tree {r.tree_size} vs oracle {o['status']} {o['iterations']} {o['tree_size']}")
+    check(any(r.status == kpax.OK for r in res) and any(r.status != kpax.OK for r in res),
+          "batch holds both found and unfound queries")
+
+
 if __name__ == "__main__":
     main()

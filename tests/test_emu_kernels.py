@@ -20,7 +20,8 @@ def emu_lib(tmp_path_factory):
         pytest.skip("no g++")
     out = str(tmp_path_factory.mktemp("emu") / "libkpax_emu.so")
     cmd = ["g++", "-std=c++20", "-x", "c++", "-I", EMU, "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
-           "-ffp-contract=off", "-fPIC", "-shared", "-pthread", os.path.join(SRC, "kpax_device.hip"), "-x", "none",
+           "-ffp-contract=off", "-fPIC", "-shared", "-pthread", os.path.join(SRC, "kpax_planner.hip"),
+           os.path.join(SRC, "kpax_ops.hip"), "-x", "none",
            os.path.join(SRC, "kpax_host.cpp"), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]

@@ -1,0 +1,154 @@
+"""GPU: batched queries on one handle (kpax_plan_batch).
+
+Each query of a batch must return exactly what a plan of that query alone
+returns (same seed, environment and tree cap): the status, iteration count,
+tree size, goal node, expansion and step counters, and the path rows bit for
+bit.  Checked against the CPU oracle for small budgets and against single
+GPU plans for larger ones.
+"""
+import numpy as np
+import pytest
+
+from workloads import workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _envs(kpax, cfg, start, goal, n_env):
+    return [kpax.generate_environment(2, 100 + e, 100 + 150 * e, cfg.ws_min, cfg.ws_max, start[:3],
+                                      list(goal[:3]) + [cfg.goal_radius]) for e in range(n_env)]
+
+
+def _same(r, o):
+    return (r.status == o.status and r.iterations == o.iterations and r.tree_size == o.tree_size
+            and r.goal_index == o.goal_index and r.expansions == o.expansions
+            and r.integrated_steps == o.integrated_steps and r.rows.shape == o.rows.shape
+            and np.array_equal(r.rows.view(np.uint32), o.rows.view(np.uint32)) and np.array_equal(r.steps, o.steps))
+
+
+def _oracle_rows(o, cfg):
+    j, nodes = o["goal_index"], []
+    while j != -1:
+        nodes.append(j)
+        j = int(o["parents"][j])
+    nodes.reverse()
+    S, C = cfg.S, cfg.C
+    rows = np.zeros((len(nodes), S + C + 1), np.float32)
+    for r, j in enumerate(nodes):
+        rows[r, :S] = o["states"][:, j]
+        rows[r, S:S + C] = o["controls"][:, j]
+        rows[r, S + C] = np.float32(cfg.dt) * np.float32(o["steps"][j])
+    return rows
+
+
+def test_batch_matches_oracle_per_query(kpax, oracle):
+    cfg, obs, start, goal = workload(2, seed=0, max_iter=40)
+    envs = [obs] + _envs(kpax, cfg, start, goal, 2)
+    near = np.array([3.0, 3.0, 1.0], np.float32)
+    qs = [kpax.Query(start, near, seed=21, budget=1 << 15, env=0),
+          kpax.Query(start, goal, seed=22, budget=20000, env=1),
+          kpax.Query(start, near, seed=23, budget=1 << 16, env=2),
+          kpax.Query(start, goal, seed=24, budget=1, env=0),      # the root alone fills the budget
+          kpax.Query(start, near, seed=25, budget=50000, env=1),
+          kpax.Query(start, goal, seed=26, budget=1 << 14, env=2)]
+    print("batch vs oracle: 6 queries on 3 slots, 3 environments", flush=True)
+    with kpax.KPAX(cfg, max_queries=3, node_capacity=100000) as p:
+        for e, o in enumerate(envs):
+            p.set_environment(e, o)
+        res = p.plan_batch(qs)
+    statuses = set()
+    for i, (q, r) in enumerate(zip(qs, res)):
+        c = cfg.copy()
+        c.max_tree = q.budget
+        o = oracle.plan(c, envs[q.env], q.start, q.goal, seed=q.seed, threads=8)
+        print(f"  query {i}: gpu {r.status} it {r.iterations} ts {r.tree_size}; oracle {o['status']} "
+              f"it {o['iterations']} ts {o['tree_size']}", flush=True)
+        assert (r.status, r.iterations, r.tree_size, r.expansions) == \
+            (o["status"], o["iterations"], o["tree_size"], o["expansions"])
+        assert r.goal_index == o["goal_index"]
+        if r.status == kpax.OK:
+            assert np.array_equal(r.rows.view(np.uint32), _oracle_rows(o, c).view(np.uint32))
+            assert r.verified == 1
+        statuses.add(r.status)
+    assert kpax.OK in statuses and kpax.BUDGET_EXHAUSTED in statuses
+
+
+def test_batch_matches_single_plans(kpax):
+    cfg, obs, start, goal = workload(2, seed=0, max_iter=60)
+    envs = [obs] + _envs(kpax, cfg, start, goal, 3)
+    rng = np.random.default_rng(5)
+    budgets = np.exp(rng.uniform(np.log(16384), np.log(1 << 18), 20)).astype(np.uint32)
+    qs = []
+    for i, b in enumerate(budgets):
+        g = goal if i % 3 else np.array([5.0, 5.0, 1.0], np.float32)
+        qs.append(kpax.Query(start, g, seed=1000 + i, budget=int(b), env=i % 4))
+    print(f"batch vs single: 20 queries, budgets {budgets.min()}..{budgets.max()}", flush=True)
+    with kpax.KPAX(cfg, max_queries=8, node_capacity=1 << 20) as p:
+        for e, o in enumerate(envs):
+            p.set_environment(e, o)
+        res = p.plan_batch(qs)
+        st = p.stats()
+    assert st["graph_launches"] > 0, "the iteration was not captured as a HIP graph"
+    for i, (q, r) in enumerate(zip(qs, res)):
+        c = cfg.copy()
+        c.max_tree = q.budget
+        with kpax.KPAX(c) as s:
+            s.set_obstacles(envs[q.env])
+            o = s.plan(q.start, q.goal, seed=q.seed)
+        assert _same(r, o), (i, r.status, r.iterations, r.tree_size, o.status, o.iterations, o.tree_size)
+    assert sum(r.status == kpax.OK for r in res) >= 5
+
+
+def test_batch_graph_and_direct_launches_agree(kpax):
+    cfg, obs, start, goal = workload(2, seed=0, max_iter=30)
+    qs = [kpax.Query(start, goal if i % 2 else np.array([4.0, 4.0, 1.0], np.float32), seed=50 + i,
+                     budget=30000 + 7000 * i) for i in range(10)]
+    with kpax.KPAX(cfg, max_queries=4, node_capacity=200000) as p:
+        p.set_obstacles(obs)
+        graphed = p.plan_batch(qs)
+        n_graph = p.stats()["graph_launches"]
+        p.set_profiling(True)  # profiling launches the kernels one by one
+        direct = p.plan_batch(qs)
+        p.set_profiling(False)
+        again = p.plan_batch(qs)  # the handle's state carries over between calls
+    assert n_graph > 0
+    for a, b, c in zip(graphed, direct, again):
+        assert _same(a, b) and _same(a, c)
+
+
+def test_batch_rejects_impossible_queries_and_runs_the_rest(kpax):
+    cfg, obs, start, goal = workload(2, seed=0, max_iter=20)
+    bad_start = np.array(start, np.float32)
+    bad_start[0] = np.nan
+    qs = [kpax.Query(start, goal, seed=1, budget=20000),
+          kpax.Query(start, goal, seed=2, budget=300000),      # above the node capacity
+          kpax.Query(start, goal, seed=3, budget=20000, env=7),  # environment never set
+          kpax.Query(bad_start, goal, seed=4, budget=20000),
+          kpax.Query(start, goal, seed=5, budget=1)]            # the root alone fills the budget
+    with kpax.KPAX(cfg, max_queries=2, node_capacity=100000) as p:
+        p.set_obstacles(obs)
+        res = p.plan_batch(qs)
+        single = p.plan(start, goal, seed=1)  # a batch handle also plans one query at a time
+    assert [r.status for r in res[1:4]] == [kpax.INVALID_ARG] * 3
+    assert res[0].status in (kpax.NOT_FOUND, kpax.BUDGET_EXHAUSTED, kpax.OK)
+    assert res[4].status == kpax.BUDGET_EXHAUSTED and res[4].tree_size == 1
+    c = cfg.copy()
+    c.max_tree = 20000
+    with kpax.KPAX(c) as s:
+        s.set_obstacles(obs)
+        assert _same(res[0], s.plan(start, goal, seed=1))
+    assert single.status in (kpax.OK, kpax.NOT_FOUND, kpax.BUDGET_EXHAUSTED)
+
+
+def test_autotune_keeps_results_and_applies_the_shape(kpax):
+    cfg, obs, start, goal = workload(2, seed=0, max_iter=30)
+    qs = [kpax.Query(start, goal, seed=300 + i, budget=40000) for i in range(6)]
+    with kpax.KPAX(cfg, max_queries=6, node_capacity=240000) as p:
+        p.set_obstacles(obs)
+        before = p.plan_batch(qs)
+        best = p.autotune(qs, repeats=1)
+        assert p.launch() == best
+        assert best["expand_chunk"] % 64 == 0 and best["expand_blocks"] > 0 and best["scan_blocks"] > 0
+        after = p.plan_batch(qs)
+    for a, b in zip(before, after):
+        assert _same(a, b)
