@@ -246,6 +246,60 @@ int kpax_set_launch(kpax_handle* h, const kpax_launch* launch);
 int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_t repeats,
                   kpax_launch* best);
 
+/* ---- learned control proposal (no counterpart in the reference, whose
+ * controls are uniform draws: statePropagatorSpatialHash.cu:236-241) ------ */
+
+/* Network: 80 features (see kpax_mlp_features) -> 128 GELU -> 128 GELU -> 8
+ * outputs: the mean of the 4 controls normalised to [-1, 1], then their log
+ * sigma.  Parameters are fp32 in the flat order W1[80][128], b1[128],
+ * W2[128][128], b2[128], W3[128][8], b3[8] (W[in][out]); the GPU computes
+ * with bf16 copies and fp32 accumulation.  Quadcopter (C = 4) only. */
+#define KPAX_MLP_FEATURES 80
+#define KPAX_MLP_OUTPUTS  8
+#define KPAX_MLP_PARAMS   27912
+
+/* He-uniform initial parameters from a seed (log sigma biases at log 1/2). */
+int kpax_mlp_init_params(uint64_t seed, float* params);
+
+/* Use the network as a proposal: each child draws its control from its
+ * parent's Gaussian (clamped to the control box) with probability `mix` and
+ * uniformly otherwise.  mix = 0 or params = NULL restores the plain planner,
+ * whose results are then unchanged bit for bit. */
+int kpax_set_proposal(kpax_handle* h, const float* params, float mix);
+
+/* Features of n states (S x n SoA) with their goals (3 x n) in environment
+ * `env`: 3 scaled positions, 3 angles / pi, 3 body velocities / vel bound,
+ * 3 rates / 4, 3 goal-relative positions, 64 occupancy bits of the 4x4x4
+ * window of a 16^3 lattice around the position, one zero.  out: n x 80. */
+int kpax_mlp_features(kpax_handle* h, uint32_t env, const float* states_soa,
+                      const float* goals_soa, uint32_t n, float* out);
+
+/* Raw network outputs (n x 8) of n feature rows (n x 80) under `params`. */
+int kpax_mlp_infer(kpax_handle* h, const float* params, const float* features, uint32_t n,
+                   float* out);
+
+/* Controls (n x 4) from given means and sigmas (n x 8: 4 means, 4 sigmas) and
+ * four 32-bit random words per item (n x 4): Box-Muller normals, clamp to
+ * [-1, 1], map onto the control box.  The sampler the expansion uses. */
+int kpax_propose_sample(kpax_handle* h, const float* mean_sigma, const uint32_t* words,
+                        uint32_t n, float* controls);
+
+/* Trainer: the network's forward and backward passes and Adam on the GPU. */
+typedef struct kpax_trainer kpax_trainer;
+int kpax_trainer_create(int32_t device, uint32_t max_batch, kpax_trainer** out);
+int kpax_trainer_destroy(kpax_trainer* t);
+const char* kpax_trainer_last_error(const kpax_trainer* t);
+int kpax_trainer_set_params(kpax_trainer* t, const float* params);
+int kpax_trainer_get_params(kpax_trainer* t, float* params);
+/* Loss (mean over rows of the Gaussian negative log-likelihood of the
+ * targets Y, n x 4 normalised controls, log sigma floored at -4) and its
+ * gradient with respect to every parameter, for features X (n x 80). */
+int kpax_trainer_gradients(kpax_trainer* t, const float* X, const float* Y, uint32_t n,
+                           float* grads, float* loss);
+/* Adam over `epochs` shuffled passes of full batches; epoch_loss[epochs]. */
+int kpax_trainer_fit(kpax_trainer* t, const float* X, const float* Y, uint32_t n, uint32_t epochs,
+                     uint32_t batch, float lr, uint64_t seed, float* epoch_loss);
+
 /* ---- hot-path operators (parity entry points, host buffers) ------------- */
 
 /* Propagate n edges with injected controls and step counts: the body of

@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libkpax.so")
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["kpax_planner.hip", "kpax_ops.hip"]
+HIP_SOURCES = ["kpax_planner.hip", "kpax_ops.hip", "kpax_mlp.hip"]
 HOST_SOURCES = ["kpax_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KPAX_ARCH", "gfx950")

@@ -46,7 +46,7 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
     return make_uint4(c0, c1, c2, c3);
 }
 
-enum : uint32_t { PURPOSE_CONTROL = 0, PURPOSE_STEPS_ADMIT = 1, PURPOSE_READMIT = 2 };
+enum : uint32_t { PURPOSE_CONTROL = 0, PURPOSE_STEPS_ADMIT = 1, PURPOSE_READMIT = 2, PURPOSE_PROPOSAL = 3 };
 
 __device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
 
@@ -100,6 +100,7 @@ struct GridView {
     const int32_t* items;
     const float4* lo;
     const float4* hi;
+    const uint32_t* occ;  // occupancy bitmap (learned proposal's features)
 };
 
 __device__ __forceinline__ int cell_of(float x, float mn, float cell, int D)
@@ -377,6 +378,13 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+#pragma unroll
+    for(int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+__device__ __forceinline__ float wave_sum_f(float x)
 {
 #pragma unroll
     for(int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);

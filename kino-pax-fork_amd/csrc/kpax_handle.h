@@ -52,6 +52,7 @@ struct Env {
     int32_t* items  = nullptr;
     float4* lo      = nullptr;
     float4* hi      = nullptr;
+    uint32_t* occ   = nullptr;  // occupancy bitmap of the learned proposal's features
     uint32_t n_obs  = 0;
     uint32_t k      = 0;
     std::vector<float> host;
@@ -92,6 +93,7 @@ struct kpax_handle {
     uint32_t* child_off     = nullptr;  // Qmax + 1 offsets, written by the advance kernel
     uint32_t* atile_off     = nullptr;
     uint32_t* ftile_off     = nullptr;
+    uint32_t* ptile_off     = nullptr;
     int32_t* slot_status    = nullptr;  // Qmax statuses + the running count
     uint64_t* st_append     = nullptr;
     uint64_t* st_frontier   = nullptr;
@@ -125,6 +127,10 @@ struct kpax_handle {
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     bool graph_tried        = false;
     bool use_graph          = true;
+    // learned proposal (kpax_set_proposal)
+    void* mlp               = nullptr;  // bf16 network layouts
+    float* prop             = nullptr;  // per node: 4 means then 4 sigmas, column stride T
+    float mix               = 0.0f;
     // environments
     std::vector<kpax::Env> envs;
     // last single-query plan (for kpax_get_tree / kpax_get_regions)
@@ -172,6 +178,7 @@ inline dev::GridView grid_view(const kpax_handle* h, uint32_t env)
     g.items = h->envs[env].items;
     g.lo    = h->envs[env].lo;
     g.hi    = h->envs[env].hi;
+    g.occ   = h->envs[env].occ;
     return g;
 }
 
@@ -194,5 +201,13 @@ struct Scratch {
 // Builds environment `env`'s CSR grid on the device (kpax_ops.hip).
 int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n);
 void free_env(Env& e);
+// Occupancy bitmap of environment `env` for the proposal's features (kpax_mlp.hip).
+int build_occupancy(kpax_handle* h, uint32_t env);
+// Proposal pass over the nodes appended this iteration, or over the roots of
+// the n_init slots listed in d_init_slots (kpax_mlp.hip).
+int launch_propose(kpax_handle* h, uint32_t n_init);
+// After a change of what the iteration launches: drops the captured graphs
+// and rewrites the expansion's argument block (kpax_planner.hip).
+int refresh_launch_state(kpax_handle* h);
 
 }  // namespace kpax

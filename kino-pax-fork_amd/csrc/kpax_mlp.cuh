@@ -1,0 +1,199 @@
+// kpax_mlp.cuh — device pieces of the learned control proposal: the
+// network's input features, bf16 conversion, the MFMA tile primitive and the
+// Gaussian sampler that turns a node's (mean, sigma) into a control.
+//
+// Network: 80 inputs (79 features + one zero pad) -> 128 GELU -> 128 GELU ->
+// 8 outputs (mean of the 4 normalised controls, log sigma of each).  Weights
+// are bf16 on the matrix cores with fp32 accumulation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "kpax_device.cuh"
+
+namespace kpax {
+
+constexpr int kMlpIn = 80, kMlpHidden = 128, kMlpOut = 8;
+constexpr int kMlpFeatures = 79;
+// flat fp32 parameter layout: W1 [80][128], b1 [128], W2 [128][128], b2 [128],
+// W3 [128][8], b3 [8] (W[k][n]: input k, output n)
+constexpr int kMlpW1 = 0, kMlpB1 = kMlpW1 + kMlpIn * kMlpHidden, kMlpW2 = kMlpB1 + kMlpHidden,
+              kMlpB2 = kMlpW2 + kMlpHidden * kMlpHidden, kMlpW3 = kMlpB2 + kMlpHidden,
+              kMlpB3 = kMlpW3 + kMlpHidden * kMlpOut, kMlpParams = kMlpB3 + kMlpOut;
+constexpr int kOccDim = 16;  // occupancy bitmap: 16^3 cells over the workspace
+
+// bf16 copies of the network on the device.  The part up to `b3` is the
+// inference layout (transposed weights, rows padded so the matrix-core reads
+// of 32 lanes spread over the LDS banks) and is copied verbatim into LDS;
+// the rest is what the backward pass reads.
+constexpr int kLdW1 = 88, kLdH = 136, kLdW3b = 16;
+struct MlpDev {
+    uint16_t w1t[kMlpHidden][kLdW1];  // W1^T [out][in], in < 80
+    uint16_t w2t[kMlpHidden][kLdH];   // W2^T [out][in]
+    uint16_t w3t[32][kLdH];           // W3^T [out][in], out < 8
+    float b1[kMlpHidden], b2[kMlpHidden], b3[32];
+    uint16_t w2[kMlpHidden][kLdH];    // W2 [in][out]
+    uint16_t w3[kMlpHidden][kLdW3b];  // W3 [in][out], out < 8
+};
+constexpr size_t kMlpLdsWeights = offsetof(MlpDev, w2);
+
+// Feature scaling, folded on the host from the configuration.
+struct FeatParams {
+    float mid[3], half[3];  // workspace centre and half extent
+    float occ_mn[3], occ_cell[3];
+    float inv_pi, inv_vel, rate_scale;
+    int32_t S;
+};
+
+namespace dev {
+
+// ---------------------------------------------------------------------------
+// bf16 as raw 16-bit patterns (round to nearest even; NaN stays NaN)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
+
+// ---------------------------------------------------------------------------
+// Matrix-core tile: D(32x32, fp32) = A(32x16) * B(16x32) + C, bf16 operands.
+// Lane l holds A row (l % 32), k = 8 (l / 32) .. +7, and B column (l % 32),
+// the same k; its 16 results are column (l % 32), rows
+// 8 (i / 4) + 4 (l / 32) + (i % 4), i = 0..15.
+// ---------------------------------------------------------------------------
+#ifndef KPAX_HOST_EMULATION
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f32x16 mfma_32x32x16(uint4 a, uint4 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+#endif
+
+__device__ __forceinline__ int mfma_row(int i, int lane) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); }
+
+// GELU, tanh form.
+__device__ __forceinline__ float gelu(float x)
+{
+    const float k = 0.7978845608028654f;  // sqrt(2 / pi)
+    return 0.5f * x * (1.0f + tanhf(k * (x + 0.044715f * x * x * x)));
+}
+
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    const float k = 0.7978845608028654f;
+    const float u = k * (x + 0.044715f * x * x * x);
+    const float th = tanhf(u);
+    return 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * k * (1.0f + 3.0f * 0.044715f * x * x);
+}
+
+// ---------------------------------------------------------------------------
+// Features of one node: scaled state, goal-relative position, 4x4x4
+// occupancy window around the position (cells outside the workspace count
+// as occupied).  f[79] = 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int occ_cell_index(float x, float mn, float cell)
+{
+    float v = (x - mn) / cell + 0.5f;
+    if(!(v >= -8.0f)) v = -8.0f;  // NaN too
+    if(v > 24.0f) v = 24.0f;
+    return (int)floorf(v) - 2;
+}
+
+__device__ __forceinline__ float occ_bit(const uint32_t* occ, int a, int b, int c)
+{
+    if(a < 0 || b < 0 || c < 0 || a >= kOccDim || b >= kOccDim || c >= kOccDim) return 1.0f;
+    const int i = (a * kOccDim + b) * kOccDim + c;
+    return (occ[i >> 5] >> (i & 31)) & 1u ? 1.0f : 0.0f;
+}
+
+// x: the node's state (S values); writes 80 features.
+__device__ __forceinline__ void features(const FeatParams& F, const uint32_t* occ, const float* x, const float* goal,
+                                         float* f)
+{
+#pragma unroll
+    for(int d = 0; d < 3; ++d) f[d] = (x[d] - F.mid[d]) / F.half[d];
+#pragma unroll
+    for(int d = 0; d < 3; ++d) f[3 + d] = x[3 + d] * F.inv_pi;
+#pragma unroll
+    for(int d = 0; d < 3; ++d) f[6 + d] = x[6 + d] * F.inv_vel;
+#pragma unroll
+    for(int d = 0; d < 3; ++d) f[9 + d] = x[9 + d] * F.rate_scale;
+#pragma unroll
+    for(int d = 0; d < 3; ++d) f[12 + d] = (goal[d] - x[d]) / F.half[d];
+    const int a0 = occ_cell_index(x[0], F.occ_mn[0], F.occ_cell[0]);
+    const int b0 = occ_cell_index(x[1], F.occ_mn[1], F.occ_cell[1]);
+    const int c0 = occ_cell_index(x[2], F.occ_mn[2], F.occ_cell[2]);
+    for(int i = 0; i < 4; ++i)
+        for(int j = 0; j < 4; ++j)
+            for(int k = 0; k < 4; ++k) f[15 + 16 * i + 4 * j + k] = occ_bit(occ, a0 + i, b0 + j, c0 + k);
+    f[79] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Sampler.  ln_f32: natural log of x in (0, 1] from its exponent and
+// 2 atanh((m - 1) / (m + 1)) of the mantissa m in [sqrt(1/2), sqrt(2)) (odd
+// series to s^11, |s| <= 0.172), every operation written out so the host
+// restatement is bit-identical.  Box-Muller turns two 24-bit uniforms into
+// two standard normals.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float ln_f32(float x)
+{
+    const uint32_t bits = __builtin_bit_cast(uint32_t, x);
+    int e               = (int)(bits >> 23) - 127;
+    float m             = __builtin_bit_cast(float, (bits & 0x007fffffu) | 0x3f800000u);
+    if(m > 1.41421356f)
+    {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    const float s  = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p        = __builtin_fmaf(s2, 0.09090909f, 0.11111111f);
+    p              = __builtin_fmaf(s2, p, 0.14285714f);
+    p              = __builtin_fmaf(s2, p, 0.2f);
+    p              = __builtin_fmaf(s2, p, 0.33333333f);
+    const float lm = 2.0f * __builtin_fmaf(s * s2, p, s);
+    return __builtin_fmaf((float)e, 0.69314718f, lm);
+}
+
+__device__ __forceinline__ void box_muller(uint32_t w0, uint32_t w1, float* z0, float* z1)
+{
+    const float u1 = 1.0f - u01(w0);  // (0, 1]
+    const float u2 = u01(w1);
+    const float r  = __builtin_sqrtf(-2.0f * ln_f32(u1));
+    float sn, cs;
+    sincos_f32(6.28318531f * u2, &sn, &cs);
+    *z0 = r * cs;
+    *z1 = r * sn;
+}
+
+// Control from the node's Gaussian: normalised u = clamp(mean + sigma z,
+// -1, 1) mapped onto [ctrl_min, ctrl_min + ctrl_rng].
+template<int C>
+__device__ __forceinline__ void propose_control(const float* mean, const float* sigma, uint4 w, const ModelParams& P,
+                                                float* u)
+{
+    float z[4];
+    box_muller(w.x, w.y, &z[0], &z[1]);
+    box_muller(w.z, w.w, &z[2], &z[3]);
+#pragma unroll
+    for(int j = 0; j < C; ++j)
+    {
+        float un = mean[j] + sigma[j] * z[j];
+        un       = un < -1.0f ? -1.0f : (un > 1.0f ? 1.0f : un);
+        u[j]     = P.ctrl_min[j] + (un + 1.0f) * 0.5f * P.ctrl_rng[j];
+    }
+}
+
+}  // namespace dev
+}  // namespace kpax

@@ -218,6 +218,7 @@ void free_env(Env& e)
     if(e.items) (void)hipFree(e.items);
     if(e.lo) (void)hipFree(e.lo);
     if(e.hi) (void)hipFree(e.hi);
+    if(e.occ) (void)hipFree(e.occ);
     e = Env{};
 }
 
@@ -270,6 +271,10 @@ int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n)
                            e.start, cursor, e.items);
     hipLaunchKernelGGL(k_grid_sort, dim3(blocks_for(G, 256)), dim3(256), 0, h->stream, e.start, e.items, G);
     HIPC(hipGetLastError());
+    {
+        const int rc = build_occupancy(h, env);
+        if(rc != KPAX_OK) return rc;
+    }
     const GridView g = grid_view(h, env);
     HIPC(hipMemcpyAsync(h->envs_dev + env, &g, sizeof(g), hipMemcpyHostToDevice, h->stream));
     HIPC(hipStreamSynchronize(h->stream));

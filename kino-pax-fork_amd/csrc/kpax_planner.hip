@@ -40,6 +40,7 @@
 #include <vector>
 
 #include "kpax_handle.h"
+#include "kpax_mlp.cuh"
 
 using namespace kpax;
 using namespace kpax::dev;
@@ -134,6 +135,8 @@ struct ExpandArgs {
     GridParams GP;
     uint32_t T, Q, R2;
     uint32_t chunk;  // children per work ticket (multiple of 64)
+    float mix;       // probability of the learned proposal (0: uniform controls only)
+    const float* prop;
     SlotUniform s0;
 };
 
@@ -147,12 +150,12 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
                                                   uint32_t* atile_off, uint32_t* ftile_off, int32_t* slot_status,
                                                   uint32_t Q, uint64_t* st_append, uint64_t* st_frontier,
                                                   uint32_t n_status, PlanParams PP, const GridView* envs,
-                                                  SlotUniform* s0)
+                                                  SlotUniform* s0, uint32_t* ptile_off)
 {
     __shared__ uint32_t lds[16];
     __shared__ uint32_t s_epoch;
     const uint32_t q = threadIdx.x;
-    uint32_t m = 0, at = 0, ft = 0, run = 0;
+    uint32_t m = 0, at = 0, ft = 0, pt = 0, run = 0;
     if(q < Q)
     {
         Ctl* c       = &ctl[q];
@@ -172,25 +175,30 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
             m           = n.M;
             at          = (n.M + kTile - 1) / kTile;
             ft          = (uint32_t)(((uint64_t)n.ts + n.M + kTile - 1) / kTile);
+            pt          = (n.M + 31) / 32;
             run         = 1;
         }
     }
-    uint32_t tm, ta, tf, tr;
+    uint32_t tm, ta, tf, tp, tr;
     const uint32_t em = block_excl_scan<16>(m, &tm, lds);
     const uint32_t ea = block_excl_scan<16>(at, &ta, lds);
     const uint32_t ef = block_excl_scan<16>(ft, &tf, lds);
+    const uint32_t ep = block_excl_scan<16>(pt, &tp, lds);
     (void)block_excl_scan<16>(run, &tr, lds);
     if(q < Q)
     {
         child_off[q] = em;
         atile_off[q] = ea;
         ftile_off[q] = ef;
+        ptile_off[q] = ep;
     }
     if(q == 0)
     {
         child_off[Q]     = tm;
         atile_off[Q]     = ta;
         ftile_off[Q]     = tf;
+        ptile_off[Q]     = tp;
+        bc->ptiles       = tp;
         bc->children     = tm;
         bc->atiles       = ta;
         bc->ftiles       = tf;
@@ -225,10 +233,12 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
 // stop early, so refilling keeps the lanes of a wave busy instead of waiting
 // for the longest edge.  Each child's arithmetic is that of propagate().
 //
-// MULTI = false: one slot (slot 0); its record, seed and grid are uniform
-// and region counters are summed in LDS.  MULTI = true: each lane looks up
-// the slot of its child (children of all slots are numbered consecutively)
-// and carries that slot's iteration, node range and grid pointers.
+// MULTI = false: one slot (slot 0); its record, seed and grid are uniform.
+// MULTI = true: each lane looks up the slot of its child (children of all
+// slots are numbered consecutively) and carries that slot's iteration, node
+// range and grid pointers.  Every child's R1 region and integrated step
+// count go out with it; k_append sums them per query, so the expansion makes
+// no counter atomics.
 // ---------------------------------------------------------------------------
 // Five resident waves per SIMD (96 VGPRs for the quadcopter) hide more of
 // the grid lookups' latency than four; the batched form carries a slot's
@@ -246,18 +256,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
     const uint32_t T       = A->T;
     constexpr int S = Dims<MODEL>::S, C = Dims<MODEL>::C;
     constexpr uint32_t kNone = 0xffffffffu;
-    // Region counters of the single-slot form are summed in LDS (total in
-    // [0, R1), valid in [R1, 2 R1)) and added to HBM once per work-group:
-    // siblings land in the same few regions, so per-child HBM atomics would
-    // serialise on them.
-    HIP_DYNAMIC_SHARED(int32_t, s_hist)
     const uint32_t total = MULTI ? A->bc->children : (A->s0.cur.status == kRunning ? A->s0.cur.M : 0u);
     if(total == 0) return;
     const int R1 = RP.r1_count;
     // slot-uniform in the single-slot form, per lane in the batched form
     uint32_t t = 0, fan = 1, k0 = 0, k1 = 0, qword = 0, slot = 0, base = 0;
     GridView G;
-    bool local = false;
     if constexpr(!MULTI)
     {
         t     = A->s0.cur.t;
@@ -267,12 +271,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
         k1    = (uint32_t)(A->s0.qd.seed >> 32);
         qword = A->s0.qd.query;
         G     = A->s0.G;
-        local           = R1 <= kLdsRegions;
-        if(local)
-        {
-            for(int r = threadIdx.x; r < 2 * R1; r += blockDim.x) s_hist[r] = 0;
-            __syncthreads();
-        }
     }
     else
         G.gp = A->GP;  // grid parameters are shared; pointers are set per child
@@ -283,7 +281,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
     float x[S], u[C];
     uint32_t q_next = 0, q_end = 0;  // the wave's current chunk [q_next, q_end)
     bool drained        = false;
-    uint32_t steps_done = 0, steps_slot = 0;  // integration steps not yet added to ctl[steps_slot]
     for(;;)
     {
         if(!drained)
@@ -327,14 +324,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
                         G.items          = g.items;
                         G.lo             = g.lo;
                         G.hi             = g.hi;
-                        if(s != steps_slot && steps_done)
-                        {
-                            atomicAdd((unsigned long long*)&A->ctl[steps_slot].steps_done,
-                                      (unsigned long long)steps_done);
-                            steps_done = 0;
-                        }
-                        steps_slot = s;
-                        slot       = s;
+                        slot             = s;
                     }
                     c      = ci;
                     parent            = R[base + c / fan];
@@ -346,6 +336,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
                     const uint32_t dcw[4] = {dc.x, dc.y, dc.z, dc.w};
 #pragma unroll
                     for(int j = 0; j < C; ++j) u[j] = P.ctrl_min[j] + u01(dcw[j]) * P.ctrl_rng[j];
+                    if(A->mix > 0.0f && u01(ds.z) < A->mix)
+                    {
+                        // the parent's Gaussian instead (learned proposal)
+                        const uint4 dp = philox(c, t, qword, PURPOSE_PROPOSAL, k0, k1);
+                        float mean[C], sg[C];
+#pragma unroll
+                        for(int j = 0; j < C; ++j)
+                        {
+                            mean[j] = A->prop[(size_t)j * T + pn];
+                            sg[j]   = A->prop[(size_t)(4 + j) * T + pn];
+                        }
+                        propose_control<C>(mean, sg, dp, P, u);
+                    }
                     int kk = (int)(u01(ds.x) * (float)P.span);
                     if(kk > P.span - 1) kk = P.span - 1;
                     n          = P.steps_min + kk;
@@ -364,21 +367,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
             ++i;
             if(!ok || i == n)
             {
-                steps_done += (uint32_t)i;
                 int r1, r2;
                 locate(RP, x, &r1, &r2);
                 const size_t ro = MULTI ? (size_t)slot * R1 : 0;
-                if(local)
-                {
-                    atomicAdd(&s_hist[r1], 1);
-                    if(ok) atomicAdd(&s_hist[R1 + r1], 1);
-                }
-                else
-                {
-                    atomicAdd(&rv.total[ro + r1], 1);
-                    if(ok) atomicAdd(&rv.valid[ro + r1], 1);
-                }
-                uint32_t flags = 0;
+                uint32_t flags  = 0;
                 if(ok)
                 {
                     flags = kValid | claim_cell(rv.r2_state + (MULTI ? (size_t)slot * A->R2 : 0), r2, t, c);
@@ -389,45 +381,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
                 for(int d = 0; d < S; ++d) ch.st[(size_t)d * T + o] = x[d];
 #pragma unroll
                 for(int d = 0; d < C; ++d) ch.ct[(size_t)d * T + o] = u[d];
-                ch.steps[o]  = n;
+                ch.steps[o]  = i;  // steps integrated: n for a valid child
                 ch.parent[o] = (int32_t)parent;
                 ch.reg[o]    = r1;
                 ch.r2[o]     = r2;
                 ch.adm[o]    = (uint8_t)flags;
                 c            = kNone;
             }
-        }
-    }
-    if constexpr(!MULTI)
-    {
-        const uint32_t ws = wave_sum(steps_done);
-        if(lane == 0 && ws) atomicAdd((unsigned long long*)&A->ctl[0].steps_done, (unsigned long long)ws);
-        if(local)
-        {
-            __syncthreads();
-            for(int r = threadIdx.x; r < R1; r += blockDim.x)
-            {
-                const int32_t a = s_hist[r], b = s_hist[R1 + r];
-                if(a) atomicAdd(&rv.total[r], a);
-                if(b) atomicAdd(&rv.valid[r], b);
-            }
-        }
-    }
-    else
-    {
-        // one atomic per wave when every lane's remainder belongs to one slot
-        const uint64_t has = __ballot(steps_done != 0);
-        if(has)
-        {
-            const int leader  = __builtin_ctzll(has);
-            const uint32_t ls = __shfl(steps_slot, leader, 64);
-            if(__ballot(steps_done != 0 && steps_slot != ls) == 0)
-            {
-                const uint32_t s = wave_sum(steps_done);
-                if(lane == leader) atomicAdd((unsigned long long*)&A->ctl[ls].steps_done, (unsigned long long)s);
-            }
-            else if(steps_done)
-                atomicAdd((unsigned long long*)&A->ctl[steps_slot].steps_done, (unsigned long long)steps_done);
         }
     }
 }
@@ -480,10 +440,15 @@ __global__ __launch_bounds__(1024) void k_score(Ctl* ctl, RegionView rv, RegionP
 }
 
 // ---------------------------------------------------------------------------
-// Append: compaction of the admitted children into the tree in slot order
-// and the goal test (updateFrontier_kernel, KPAX.cu:458-502).  Tiles of all
-// slots are numbered consecutively; each slot's tiles form one look-back
-// segment.
+// Append: the region counters of the iteration's children (total for every
+// child, at its failing state if invalid; valid for valid ones:
+// KPAX.cu:386-390) and their integrated steps, compaction of the admitted children into the tree in slot
+// order and the goal test (updateFrontier_kernel, KPAX.cu:458-502).  Tiles
+// of all slots are numbered consecutively; each slot's tiles form one
+// look-back segment.  A tile's children belong to one slot, so its counters
+// are summed in LDS (total in [0, R1), valid in [R1, 2 R1)) and only the
+// regions it touched reach HBM: siblings land in the same few regions, and
+// per-child device-scope atomics would serialise on them.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, BatchCtl* bc, const uint32_t* atile_off,
                                                 uint32_t Q, ChildView ch, TreeView tree, RegionView rv,
@@ -492,11 +457,18 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
 {
     __shared__ uint32_t lds[4];
     __shared__ uint32_t s_tile, s_pre;
+    HIP_DYNAMIC_SHARED(int32_t, s_hist)
     const uint32_t ntiles = bc->atiles, epoch = bc->epoch;
+    const bool local      = R1 <= (uint32_t)kLdsRegions;
     for(;;)
     {
         const uint32_t tile = next_tile(&bc->tile_append, &s_tile);
         if(tile >= ntiles) break;
+        if(local)
+        {
+            for(uint32_t r = threadIdx.x; r < 2 * R1; r += blockDim.x) s_hist[r] = 0;
+            __syncthreads();
+        }
         const uint32_t q      = segment_of(atile_off, Q, tile);
         const uint32_t first  = atile_off[q], lt = tile - first, nt = atile_off[q + 1] - first;
         const Iter it         = ctl[q].cur;
@@ -505,13 +477,30 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
         const size_t base     = d.base;
         const uint64_t* r2s   = rv.r2_state + (size_t)q * R2;
         int32_t* cov          = rv.covered + (size_t)q * R1;
+        int32_t* tot          = rv.total + (size_t)q * R1;
+        int32_t* val          = rv.valid + (size_t)q * R1;
         const uint32_t cbase  = lt * kTile + threadIdx.x * 4;
-        uint32_t f[4], cnt = 0;
+        uint32_t f[4], cnt = 0, steps = 0;
 #pragma unroll
         for(int j = 0; j < 4; ++j)
         {
             const uint32_t c = cbase + j;
             const uint32_t a = c < M ? ch.adm[base + c] : 0u;
+            if(c < M)
+            {
+                steps += (uint32_t)ch.steps[base + c];
+                const int r = ch.reg[base + c];
+                if(local)
+                {
+                    atomicAdd(&s_hist[r], 1);
+                    if(a & kValid) atomicAdd(&s_hist[R1 + r], 1);
+                }
+                else
+                {
+                    atomicAdd(&tot[r], 1);
+                    if(a & kValid) atomicAdd(&val[r], 1);
+                }
+            }
             // a candidate that holds its cell's key was the first to reach it:
             // it covers the cell and is admitted (KPAX.cu:394-395)
             bool first_in_cell = false;
@@ -523,7 +512,10 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
             f[j] = (a & kValid) && ((a & kScoreAdmit) || first_in_cell) ? 1u : 0u;
             cnt += f[j];
         }
-        uint32_t agg;
+        uint32_t agg, steps_tile;
+        (void)block_excl_scan_256(steps, &steps_tile, lds);
+        if(threadIdx.x == 0 && steps_tile)
+            atomicAdd((unsigned long long*)&ctl[q].steps_done, (unsigned long long)steps_tile);
         const uint32_t loc = block_excl_scan_256(cnt, &agg, lds);
         const uint32_t pre = tile_prefix(status, tile, agg, first, epoch, &s_pre);
         uint32_t dst       = ts + pre + loc;
@@ -553,6 +545,16 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
             const uint32_t Fn = pre + agg;
             ctl[q].Fn         = Fn;
             ctl[q].f_accept   = f_accept(it.t, ts + Fn, cap, PP.eps);
+        }
+        if(local)
+        {
+            __syncthreads();
+            for(uint32_t r = threadIdx.x; r < R1; r += blockDim.x)
+            {
+                const int32_t a = s_hist[r], b = s_hist[R1 + r];
+                if(a) atomicAdd(&tot[r], a);
+                if(b) atomicAdd(&val[r], b);
+            }
         }
     }
 }
@@ -689,9 +691,9 @@ __global__ __launch_bounds__(64) void k_path(Ctl* ctl, const QueryDev* qd, const
 // Host side
 // ===========================================================================
 
-size_t expand_lds_bytes(const kpax_handle* h)
+size_t append_lds_bytes(const kpax_handle* h)
 {
-    return h->Qmax == 1 && h->RP.r1_count <= kLdsRegions ? 2 * (size_t)h->RP.r1_count * sizeof(int32_t) : 0;
+    return h->RP.r1_count <= kLdsRegions ? 2 * (size_t)h->RP.r1_count * sizeof(int32_t) : 0;
 }
 
 // Work-groups of k_expand that fit on the device at once: its waves are
@@ -700,7 +702,7 @@ template<int MODEL, bool MULTI>
 uint32_t expand_blocks_for(const kpax_handle* h)
 {
     int per_cu = 0;
-    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand<MODEL, MULTI>, 256, expand_lds_bytes(h)) !=
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand<MODEL, MULTI>, 256, 0) !=
          hipSuccess ||
        per_cu < 1)
         per_cu = 1;
@@ -743,6 +745,8 @@ int upload_expand_args(kpax_handle* h)
     a.Q         = h->Qmax;
     a.R2        = h->R2;
     a.chunk     = h->expand_chunk;
+    a.mix       = h->mix;
+    a.prop      = h->prop;
     memset(&a.s0, 0, sizeof(a.s0));
     a.s0.cur.status = kNotStarted;
     HIPC(hipMemcpy(h->expand_args, &a, sizeof(a), hipMemcpyHostToDevice));
@@ -754,8 +758,7 @@ void launch_expand_model(kpax_handle* h)
 {
     const ExpandArgs* A = (const ExpandArgs*)h->expand_args;
     if(h->Qmax == 1)
-        hipLaunchKernelGGL((k_expand<MODEL, false>), dim3(h->expand_blocks), dim3(256), expand_lds_bytes(h), h->stream,
-                           A);
+        hipLaunchKernelGGL((k_expand<MODEL, false>), dim3(h->expand_blocks), dim3(256), 0, h->stream, A);
     else
         hipLaunchKernelGGL((k_expand<MODEL, true>), dim3(h->expand_blocks), dim3(256), 0, h->stream, A);
 }
@@ -778,12 +781,18 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
     const uint32_t Q = h->Qmax, R1 = (uint32_t)h->RP.r1_count;
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc, h->child_off, h->atile_off,
                        h->ftile_off, h->slot_status, Q, h->st_append, h->st_frontier, h->n_status, h->PP, h->envs_dev,
-                       &((ExpandArgs*)h->expand_args)->s0);
+                       &((ExpandArgs*)h->expand_args)->s0, h->ptile_off);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
     launch_expand(h);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][1], h->stream));
-    hipLaunchKernelGGL(k_append, dim3(h->scan_blocks), dim3(256), 0, h->stream, h->ctl, h->qd, h->bc, h->atile_off, Q,
+    hipLaunchKernelGGL(k_append, dim3(h->scan_blocks), dim3(256), append_lds_bytes(h), h->stream, h->ctl, h->qd, h->bc,
+                       h->atile_off, Q,
                        h->ch, h->tree, h->rv, h->st_append, h->PP, h->S, h->C, h->T, R1, h->R2);
+    if(h->mix > 0.0f)
+    {
+        const int rc = launch_propose(h, 0);
+        if(rc != KPAX_OK) return rc;
+    }
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][2], h->stream));
     hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][3], h->stream));
@@ -975,6 +984,11 @@ int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
         hipLaunchKernelGGL(k_init, dim3(n_init), dim3(256), 0, h->stream, h->ctl, h->qd, h->tree, h->R,
                            h->slot_status, h->d_init_slots, h->d_init_starts, S, C, h->T, h->RP, h->PP);
         HIPC(hipGetLastError());
+        if(h->mix > 0.0f)
+        {
+            const int rc = launch_propose(h, n_init);
+            if(rc != KPAX_OK) return rc;
+        }
         if(prof && first) HIPC(hipEventRecord(h->init_ev[1], h->stream));
         return KPAX_OK;
     };
@@ -1157,6 +1171,16 @@ void drop_graphs(kpax_handle* h)
     h->graph_tried = false;
 }
 
+}  // namespace
+
+int kpax::refresh_launch_state(kpax_handle* h)
+{
+    drop_graphs(h);
+    return upload_expand_args(h);
+}
+
+namespace {
+
 int apply_launch(kpax_handle* h, const kpax_launch& l)
 {
     if(l.expand_blocks < 1 || l.expand_blocks > (1u << 20) || l.expand_chunk < 64 || l.expand_chunk > 4096 ||
@@ -1273,6 +1297,7 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->child_off, Q + 1));
     TRY(dalloc(h, &h->atile_off, Q + 1));
     TRY(dalloc(h, &h->ftile_off, Q + 1));
+    TRY(dalloc(h, &h->ptile_off, Q + 1));
     TRY(dalloc(h, &h->slot_status, Q + 1));
     h->n_status = (uint32_t)(T / kTile + Q + 2);
     TRY(dalloc(h, &h->st_append, h->n_status));

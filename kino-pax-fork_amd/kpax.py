@@ -119,6 +119,18 @@ def lib():
         "kpax_create_batch": (i32, [P, u32, u64, P]),
         "kpax_set_environment": (i32, [P, u32, P, u32]),
         "kpax_plan_batch": (i32, [P, P, u32, P]),
+        "kpax_mlp_init_params": (i32, [u64, P]),
+        "kpax_set_proposal": (i32, [P, P, ctypes.c_float]),
+        "kpax_mlp_features": (i32, [P, u32, P, P, u32, P]),
+        "kpax_mlp_infer": (i32, [P, P, P, u32, P]),
+        "kpax_propose_sample": (i32, [P, P, P, u32, P]),
+        "kpax_trainer_create": (i32, [i32, u32, P]),
+        "kpax_trainer_destroy": (i32, [P]),
+        "kpax_trainer_last_error": (ctypes.c_char_p, [P]),
+        "kpax_trainer_set_params": (i32, [P, P]),
+        "kpax_trainer_get_params": (i32, [P, P]),
+        "kpax_trainer_gradients": (i32, [P, P, P, u32, P, P]),
+        "kpax_trainer_fit": (i32, [P, P, P, u32, u32, u32, ctypes.c_float, u64, P]),
         "kpax_get_launch": (i32, [P, P]),
         "kpax_set_launch": (i32, [P, P]),
         "kpax_autotune": (i32, [P, P, u32, u32, P]),
@@ -266,6 +278,76 @@ def _plan_result(r, S, C):
                       r.integrated_steps, r.plan_ms, r.verified, r.verify_max_dev, S, C)
 
 
+MLP_FEATURES, MLP_OUTPUTS, MLP_PARAMS = 80, 8, 27912
+
+
+def mlp_init_params(seed=1):
+    """Initial network parameters (flat fp32, layout of include/kpax.h)."""
+    p = np.zeros(MLP_PARAMS, np.float32)
+    if lib().kpax_mlp_init_params(int(seed), _p(p)) != OK:
+        raise KpaxError("mlp_init_params failed")
+    return p
+
+
+class Trainer:
+    """The proposal network's forward/backward passes and Adam on the GPU."""
+
+    def __init__(self, max_batch=8192, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().kpax_trainer_create(int(device), int(max_batch), ctypes.byref(h))
+        if rc != OK:
+            raise KpaxError(f"kpax_trainer_create failed: {STATUS.get(rc, rc)}")
+        self._t = h
+
+    def close(self):
+        if getattr(self, "_t", None):
+            lib().kpax_trainer_destroy(self._t)
+            self._t = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != OK:
+            msg = lib().kpax_trainer_last_error(self._t)
+            raise KpaxError(f"{what}: {STATUS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def set_params(self, params):
+        p = _f32(params, (MLP_PARAMS,))
+        self._check(lib().kpax_trainer_set_params(self._t, _p(p)), "set_params")
+
+    def params(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        self._check(lib().kpax_trainer_get_params(self._t, _p(p)), "get_params")
+        return p
+
+    def gradients(self, X, Y):
+        X = _f32(X).reshape(-1, MLP_FEATURES)
+        Y = _f32(Y).reshape(-1, 4)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        self._check(lib().kpax_trainer_gradients(self._t, _p(X), _p(Y), X.shape[0], _p(g), ctypes.byref(loss)),
+                    "gradients")
+        return float(loss.value), g
+
+    def fit(self, X, Y, epochs=10, batch=4096, lr=1e-3, seed=0):
+        X = _f32(X).reshape(-1, MLP_FEATURES)
+        Y = _f32(Y).reshape(-1, 4)
+        losses = np.zeros(epochs, np.float32)
+        self._check(lib().kpax_trainer_fit(self._t, _p(X), _p(Y), X.shape[0], int(epochs), int(batch),
+                                           float(lr), int(seed), _p(losses)), "fit")
+        return losses
+
+
 @dataclass
 class Query:
     """One query of plan_batch: start state, goal centre, seed, tree budget (0 = config max_tree), environment."""
@@ -375,6 +457,36 @@ class KPAX:
         for i in range(n):
             out.append(_plan_result(res[i], self.S, self.C))
             lib().kpax_result_free(ctypes.byref(res[i]))
+        return out
+
+    # ---- learned control proposal ---------------------------------------------
+    def set_proposal(self, params, mix=0.5):
+        """Draw each child's control from its parent's learned Gaussian with
+        probability `mix` (params None or mix 0: the plain uniform sampler)."""
+        p = None if params is None else _f32(params, (MLP_PARAMS,))
+        self._check(lib().kpax_set_proposal(self._h, _p(p), float(mix if p is not None else 0.0)), "set_proposal")
+
+    def mlp_features(self, env, states, goals):
+        """Features (n, 80) of states (S, n) with goals (3, n) in environment `env`."""
+        st = _f32(states).reshape(self.S, -1)
+        n = st.shape[1]
+        g = _f32(goals).reshape(3, n)
+        out = np.zeros((n, MLP_FEATURES), np.float32)
+        self._check(lib().kpax_mlp_features(self._h, int(env), _p(st), _p(g), n, _p(out)), "mlp_features")
+        return out
+
+    def mlp_infer(self, params, X):
+        X = _f32(X).reshape(-1, MLP_FEATURES)
+        out = np.zeros((X.shape[0], MLP_OUTPUTS), np.float32)
+        self._check(lib().kpax_mlp_infer(self._h, _p(_f32(params, (MLP_PARAMS,))), _p(X), X.shape[0], _p(out)),
+                    "mlp_infer")
+        return out
+
+    def propose_sample(self, mean_sigma, words):
+        ms = _f32(mean_sigma).reshape(-1, 8)
+        w = np.ascontiguousarray(words, np.uint32).reshape(-1, 4)
+        out = np.zeros((ms.shape[0], 4), np.float32)
+        self._check(lib().kpax_propose_sample(self._h, _p(ms), _p(w), ms.shape[0], _p(out)), "propose_sample")
         return out
 
     def launch(self):

@@ -1,0 +1,118 @@
+"""Learned control proposal: training data from the plain planner, training on
+the GPU, and the comparison with the plain sampler.
+
+The reference samples every control uniformly (statePropagatorSpatialHash.cu:
+236-241); it has no counterpart to any of this.  The pipeline:
+
+1. `serving_queries` draws start/goal pairs in free space of seeded forests.
+2. The plain planner (`KPAX.plan_batch`, uniform controls) solves them; every
+   edge of every returned path is a training example: the features of the
+   edge's parent node (state, goal-relative position, 4x4x4 occupancy window)
+   and the control that edge applied, normalised to [-1, 1].
+3. `train` fits the network (80 -> 128 -> 128 -> 8, Gaussian negative
+   log-likelihood) with the GPU trainer's forward/backward kernels and Adam.
+4. `evaluate` plans held-out queries with the plain sampler and with the
+   50/50 mixture of learned and uniform controls, on the same GPU, seeds and
+   budgets, and reports success rate, iterations and time to solution.
+"""
+import statistics
+import time
+
+import numpy as np
+
+import kpax
+
+
+def forest_environments(cfg, n_env, seed, n_min=100, n_max=600):
+    """Seeded forests over the configured workspace."""
+    rng = np.random.default_rng(seed)
+    return [kpax.generate_environment(2, int(seed) * 1000 + e, int(rng.integers(n_min, n_max + 1)), cfg.ws_min,
+                                      cfg.ws_max) for e in range(n_env)]
+
+
+def _free(p, obs, margin):
+    if obs.shape[0] == 0:
+        return True
+    lo, hi = obs[:, :3] - margin, obs[:, 3:] + margin
+    return not np.any(np.all((p >= lo) & (p <= hi), axis=1))
+
+
+def serving_queries(cfg, envs, n, seed, budget=0, min_dist=4.0, budget_range=None):
+    """n start/goal pairs at least `min_dist` apart, both clear of obstacles by
+    half a metre and of the workspace faces by 0.5, each in environment i % len(envs)."""
+    rng = np.random.default_rng(seed)
+    lo = np.array(cfg.ws_min, np.float64) + 0.5
+    hi = np.array(cfg.ws_max, np.float64) - 0.5
+    out = []
+    S = 12 if cfg.model == kpax.MODEL_QUADCOPTER else 6
+    for i in range(n):
+        env = i % len(envs)
+        while True:
+            a, b = rng.uniform(lo, hi), rng.uniform(lo, hi)
+            if np.linalg.norm(a - b) >= min_dist and _free(a, envs[env], 0.5) and _free(b, envs[env], 0.5):
+                break
+        st = np.zeros(S, np.float32)
+        st[:3] = a
+        bud = budget
+        if budget_range is not None:
+            bud = int(np.exp(rng.uniform(np.log(budget_range[0]), np.log(budget_range[1]))))
+        out.append(kpax.Query(st, b.astype(np.float32), seed=int(seed) * 100003 + i, budget=bud, env=env))
+    return out
+
+
+def normalise_controls(cfg, u):
+    lo = np.array(cfg.ctrl_min[:4], np.float32)
+    hi = np.array(cfg.ctrl_max[:4], np.float32)
+    return (2.0 * (u - lo) / (hi - lo) - 1.0).astype(np.float32)
+
+
+def dataset(planner, cfg, queries, results):
+    """Features X (m, 80) and normalised controls Y (m, 4) of every edge on the
+    returned paths: the parent row's state and the child row's control."""
+    S, C = planner.S, planner.C
+    st, gl, ev, ct = [], [], [], []
+    for q, r in zip(queries, results):
+        if r.status != kpax.OK or r.rows.shape[0] < 2:
+            continue
+        st.append(r.rows[:-1, :S])
+        ct.append(r.rows[1:, S:S + C])
+        gl.append(np.tile(np.asarray(q.goal, np.float32)[:3], (r.rows.shape[0] - 1, 1)))
+        ev.append(np.full(r.rows.shape[0] - 1, q.env))
+    if not st:
+        return np.zeros((0, kpax.MLP_FEATURES), np.float32), np.zeros((0, 4), np.float32)
+    st, gl, ev, ct = np.concatenate(st), np.concatenate(gl), np.concatenate(ev), np.concatenate(ct)
+    X = np.zeros((st.shape[0], kpax.MLP_FEATURES), np.float32)
+    for e in np.unique(ev):
+        sel = ev == e
+        X[sel] = planner.mlp_features(int(e), st[sel].T.copy(), gl[sel].T.copy())
+    return X, normalise_controls(cfg, ct)
+
+
+def train(X, Y, epochs=20, batch=4096, lr=2e-3, seed=0, init_seed=1):
+    """Fit the network on the GPU; returns (parameters, per-epoch loss)."""
+    batch = min(batch, max(32, (X.shape[0] // 32) * 32))
+    with kpax.Trainer(max_batch=batch) as t:
+        t.set_params(kpax.mlp_init_params(init_seed))
+        losses = t.fit(X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        return t.params(), losses
+
+
+def evaluate(planner, queries, params=None, mix=0.5):
+    """Plan `queries` one at a time (the single-query form) with the plain
+    sampler (params None) or the mixture; success = goal reached within the
+    budget and the FP64 re-check passed."""
+    planner.set_proposal(params, mix if params is not None else 0.0)
+    res = planner.plan_batch(queries[:1])  # warm-up
+    t0 = time.perf_counter()
+    res = [planner.plan_batch([q])[0] for q in queries]
+    wall = time.perf_counter() - t0
+    planner.set_proposal(None)
+    ok = [r for r in res if r.status == kpax.OK and r.verified == 1]
+    return {
+        "success": len(ok) / len(res),
+        "plans_per_sec": len(res) / wall,
+        "p50_ms_found": statistics.median(r.plan_ms for r in ok) if ok else None,
+        "mean_iterations_found": statistics.mean(r.iterations for r in ok) if ok else None,
+        "mean_expansions": statistics.mean(r.expansions for r in res),
+        "results": res,
+    }

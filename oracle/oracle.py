@@ -178,3 +178,49 @@ def plan(cfg, obstacles, start, goal, seed, threads=0, want_tree=True):
     if want_tree:
         out.update(states=st[:, :ts], controls=ct[:, :ts], steps=steps[:ts], parents=par[:ts], regions=reg[:ts])
     return out
+
+
+# ---- learned control proposal (the build's own definition; see mlp_oracle.py)
+def _prop_lib():
+    L = lib()
+    if not getattr(L, "_prop_sigs", False):
+        P, u32 = ctypes.c_void_p, ctypes.c_uint32
+        for name, res, args in (("oracle_occupancy", None, [P, P, u32, P]),
+                                ("oracle_mlp_features", None, [P, P, u32, P, P, u32, P]),
+                                ("oracle_propose_sample", None, [P, P, P, u32, P]),
+                                ("oracle_ln_f32", ctypes.c_float, [ctypes.c_float])):
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L._prop_sigs = True
+    return L
+
+
+def occupancy(cfg, obstacles):
+    o, no = _obs(obstacles)
+    bits = np.zeros(128, np.uint32)
+    _prop_lib().oracle_occupancy(ctypes.byref(cfg), _p(o), no, _p(bits))
+    return bits
+
+
+def mlp_features(cfg, obstacles, states, goals):
+    o, no = _obs(obstacles)
+    st = np.ascontiguousarray(states, np.float32)
+    g = np.ascontiguousarray(goals, np.float32)
+    n = st.shape[1]
+    out = np.zeros((n, 80), np.float32)
+    _prop_lib().oracle_mlp_features(ctypes.byref(cfg), _p(o), no, _p(st), _p(g), n, _p(out))
+    return out
+
+
+def propose_sample(cfg, mean_sigma, words):
+    ms = np.ascontiguousarray(mean_sigma, np.float32)
+    w = np.ascontiguousarray(words, np.uint32)
+    n = ms.shape[0]
+    out = np.zeros((n, 4), np.float32)
+    _prop_lib().oracle_propose_sample(ctypes.byref(cfg), _p(ms), _p(w), n, _p(out))
+    return out
+
+
+def ln_f32(x):
+    return float(_prop_lib().oracle_ln_f32(float(x)))

@@ -57,6 +57,7 @@ struct emu_group {
     std::unique_ptr<std::barrier<>> bar;
     std::vector<std::unique_ptr<std::barrier<>>> wave_bar;
     std::vector<uint64_t> xbuf;  // 64 slots per wave
+    std::vector<uint32_t> mbuf;  // matrix-core operands: 64 lanes x 8 words per wave
 };
 inline emu_group* emu_g = nullptr;
 
@@ -141,6 +142,54 @@ template<class T> inline T atomicCAS(T* p, T cmp, T v)
 #define __HIP_MEMORY_SCOPE_AGENT 0
 #define __hip_atomic_load(p, order, scope) __atomic_load_n((p), (order))
 #define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), (order))
+
+// ---- matrix-core tile (kpax_mlp.cuh declares the lane layout) ----------------
+#define KPAX_HOST_EMULATION 1
+struct f32x16 {
+    float v[16];
+    float& operator[](int i) { return v[i]; }
+    float operator[](int i) const { return v[i]; }
+};
+
+inline void emu_wave_barrier() { emu_g->wave_bar[threadIdx.x / 64]->arrive_and_wait(); }
+
+inline float emu_bf16(uint32_t word, unsigned half)
+{
+    const uint32_t b = half ? (word >> 16) : (word & 0xffffu);
+    const uint32_t f = b << 16;
+    float x;
+    std::memcpy(&x, &f, 4);
+    return x;
+}
+
+// D(32x32) = A(32x16) B(16x32) + C: lane l supplies A row l%32 and B column
+// l%32 for k = 8 (l / 32) .. +7 and receives column l%32, rows
+// 8 (i / 4) + 4 (l / 32) + (i % 4).  All 64 lanes of the wave call it.
+inline f32x16 mfma_32x32x16(uint4 a, uint4 b, f32x16 c)
+{
+    const unsigned t = threadIdx.x, w = t / 64, lane = t % 64;
+    auto& bar        = *emu_g->wave_bar[w];
+    uint32_t* buf    = emu_g->mbuf.data() + (size_t)w * 64 * 8;
+    bar.arrive_and_wait();
+    const uint32_t in[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    for(int j = 0; j < 8; ++j) buf[lane * 8 + j] = in[j];
+    bar.arrive_and_wait();
+    f32x16 d;
+    const unsigned col = lane % 32;
+    for(int i = 0; i < 16; ++i)
+    {
+        const unsigned row = 8 * (i / 4) + 4 * (lane / 32) + (i % 4);
+        float s            = c[i];
+        for(unsigned k = 0; k < 16; ++k)
+        {
+            const unsigned la = row + 32 * (k / 8), lb = col + 32 * (k / 8), e = k % 8;
+            s += emu_bf16(buf[la * 8 + e / 2], e % 2) * emu_bf16(buf[lb * 8 + 4 + e / 2], e % 2);
+        }
+        d[i] = s;
+    }
+    bar.arrive_and_wait();
+    return d;
+}
 
 // ---- runtime ----------------------------------------------------------------
 typedef int hipError_t;
@@ -302,6 +351,7 @@ void emu_launch(void (*k)(KArgs...), dim3 grid, dim3 block, size_t shmem, Args..
         for(unsigned w = 0; w < nw; ++w)
             g.wave_bar.push_back(std::make_unique<std::barrier<>>(std::min(64u, block.x - 64 * w)));
         g.xbuf.assign(64 * nw, 0);
+        g.mbuf.assign(64 * 8 * nw, 0);
         emu_g = &g;
         emu_workers().run(block.x, [&, b](unsigned t) {
             threadIdx = {t, 0, 0};

@@ -89,6 +89,58 @@ def main():
                   f"plan workload {k} reaches the goal with a verified path")
 
     batch_checks()
+    mlp_checks()
+
+
+def mlp_checks():
+    """The learned proposal's kernels: features and sampler bit-exact against
+    the oracle, network forward and gradients within bf16 tolerance, and a
+    planner run with the proposal mixed in."""
+    import mlp_oracle
+    rng = np.random.default_rng(3)
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=1500, max_iter=6)
+    small(cfg)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        x0, _, _ = random_states(cfg, 100, rng)
+        x0[:, :5] = np.array([[-1, 5, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0]]).T  # outside the workspace
+        g = np.tile(np.array([[9.5], [9.5], [1.0]], np.float32), (1, 100))
+        f = p.mlp_features(0, x0, g)
+        check(np.array_equal(f.view(np.uint32), oracle.mlp_features(cfg, obs, x0, g).view(np.uint32)),
+              "proposal features bit-exact")
+        ms = np.concatenate([rng.uniform(-1.5, 1.5, (300, 4)), rng.uniform(0.01, 1.0, (300, 4))], 1).astype(np.float32)
+        w = rng.integers(0, 2**32, (300, 4), dtype=np.uint64).astype(np.uint32)
+        w[0] = [0, 0, 0, 0]
+        w[1] = [0xFFFFFFFF] * 4
+        u = p.propose_sample(ms, w)
+        check(np.array_equal(u.view(np.uint32), oracle.propose_sample(cfg, ms, w).view(np.uint32)),
+              "proposal sampler bit-exact")
+        params = kpax.mlp_init_params(5)
+        out = p.mlp_infer(params, f)
+        ref, _ = mlp_oracle.forward(params, f)
+        err = np.abs(out - ref).max() / max(1e-6, np.abs(ref).max())
+        check(err < 1e-2, f"network forward within bf16 tolerance (rel err {err:.2e})")
+        p.set_proposal(params, 0.5)
+        r1 = p.plan(start, goal, seed=9)
+        r2 = p.plan(start, goal, seed=9)
+        p.set_proposal(None)
+        r0 = p.plan(start, goal, seed=9)
+    o = oracle.plan(cfg, obs, start, goal, seed=9, threads=4)
+    check(r1.tree_size == r2.tree_size and r1.expansions == r2.expansions, "proposal planner is deterministic")
+    check(r0.tree_size == o["tree_size"] and r0.expansions == o["expansions"],
+          "proposal off gives the plain planner again")
+    check(r1.tree_size != r0.tree_size or r1.expansions != r0.expansions, "proposal changes the tree")
+    X = f[:100]
+    Y = rng.uniform(-1, 1, (100, 4)).astype(np.float32)
+    with kpax.Trainer(max_batch=256) as t:
+        t.set_params(params)
+        loss, grad = t.gradients(X, Y)
+        rl, rg = mlp_oracle.loss_and_grads(params, X, Y)
+        gerr = np.linalg.norm(grad - rg) / np.linalg.norm(rg)
+        check(abs(loss - rl) < 1e-4 * (1 + abs(rl)) and gerr < 2e-2,
+              f"gradients match the oracle (loss {loss:.5f} vs {rl:.5f}, rel err {gerr:.2e})")
+        losses = t.fit(np.tile(X, (3, 1)), np.tile(Y, (3, 1)), epochs=3, batch=64, lr=1e-3, seed=1)
+        check(np.all(np.isfinite(losses)), f"fit runs: losses {losses}")
 
 
 def oracle_path(o, cfg):

@@ -21,7 +21,7 @@ def emu_lib(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("emu") / "libkpax_emu.so")
     cmd = ["g++", "-std=c++20", "-x", "c++", "-I", EMU, "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
            "-ffp-contract=off", "-fPIC", "-shared", "-pthread", os.path.join(SRC, "kpax_planner.hip"),
-           os.path.join(SRC, "kpax_ops.hip"), "-x", "none",
+           os.path.join(SRC, "kpax_ops.hip"), os.path.join(SRC, "kpax_mlp.hip"), "-x", "none",
            os.path.join(SRC, "kpax_host.cpp"), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]

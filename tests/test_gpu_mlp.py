@@ -1,0 +1,149 @@
+"""GPU: the learned control proposal.
+
+Features and the sampler are integer/float work the CPU oracle restates
+exactly, so they must match bit for bit.  The network runs on the matrix
+cores with bf16 operands and fp32 accumulation; its outputs and gradients are
+checked against the numpy oracle, which rounds to bf16 at the same points, to
+a relative error of 1e-2 (the accumulation order and the last bit of tanh
+differ; a wrong lane layout or operand gives errors of order 1).
+"""
+import numpy as np
+import pytest
+
+import mlp_oracle
+from workloads import random_states, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _planner(kpax, **kw):
+    cfg, obs, start, goal = workload(2, seed=0, **kw)
+    p = kpax.KPAX(cfg)
+    p.set_obstacles(obs)
+    return p, cfg, obs, start, goal
+
+
+def _states(cfg, n, rng):
+    x0, _, _ = random_states(cfg, n, rng)
+    x0[:3, :50] += np.array([[12.0], [0.0], [0.0]], np.float32)  # outside the workspace
+    return x0
+
+
+def test_features_bit_exact(kpax, oracle, rng):
+    p, cfg, obs, _, goal = _planner(kpax, max_tree=4096)
+    with p:
+        x0 = _states(cfg, 5000, rng)
+        g = rng.uniform(0, 10, (3, 5000)).astype(np.float32)
+        f = p.mlp_features(0, x0, g)
+    want = oracle.mlp_features(cfg, obs, x0, g)
+    assert np.array_equal(f.view(np.uint32), want.view(np.uint32))
+    assert f[:, 15:79].mean() > 0.01  # the forest is seen
+
+
+def test_sampler_bit_exact(kpax, oracle, rng):
+    p, cfg, _, _, _ = _planner(kpax, max_tree=4096)
+    n = 50000
+    ms = np.concatenate([rng.uniform(-1.5, 1.5, (n, 4)), rng.uniform(0.018, 1.65, (n, 4))], 1).astype(np.float32)
+    w = rng.integers(0, 2**32, (n, 4), dtype=np.uint64).astype(np.uint32)
+    w[:2] = [[0] * 4, [0xFFFFFFFF] * 4]
+    with p:
+        u = p.propose_sample(ms, w)
+    assert np.array_equal(u.view(np.uint32), oracle.propose_sample(cfg, ms, w).view(np.uint32))
+
+
+def test_network_forward_matches_oracle(kpax, oracle, rng):
+    p, cfg, obs, _, _ = _planner(kpax, max_tree=4096)
+    params = kpax.mlp_init_params(11)
+    params[mlp_oracle.B1:mlp_oracle.W2] = rng.normal(0, 0.3, 128)
+    with p:
+        x0 = _states(cfg, 3000, rng)
+        g = rng.uniform(0, 10, (3, 3000)).astype(np.float32)
+        X = p.mlp_features(0, x0, g)
+        out = p.mlp_infer(params, X)
+    ref, _ = mlp_oracle.forward(params, X)
+    err = np.abs(out - ref).max() / np.abs(ref).max()
+    print(f"network forward: max rel err {err:.2e}", flush=True)
+    assert err < 1e-2
+
+
+def test_trainer_gradients_match_oracle(kpax, rng):
+    X = rng.normal(0, 1, (1000, 80)).astype(np.float32)
+    X[:, 15:79] = rng.uniform(0, 1, (1000, 64)) < 0.2
+    X[:, 79] = 0
+    Y = rng.uniform(-1, 1, (1000, 4)).astype(np.float32)
+    params = kpax.mlp_init_params(7)
+    params[mlp_oracle.W3:mlp_oracle.B3] *= 5
+    with kpax.Trainer(max_batch=2048) as t:
+        t.set_params(params)
+        loss, g = t.gradients(X, Y)
+    rl, rg = mlp_oracle.loss_and_grads(params, X, Y)
+    assert abs(loss - rl) <= 1e-4 * (1 + abs(rl))
+    for name, a, b in (("W1", mlp_oracle.W1, mlp_oracle.B1), ("b1", mlp_oracle.B1, mlp_oracle.W2),
+                       ("W2", mlp_oracle.W2, mlp_oracle.B2), ("b2", mlp_oracle.B2, mlp_oracle.W3),
+                       ("W3", mlp_oracle.W3, mlp_oracle.B3), ("b3", mlp_oracle.B3, mlp_oracle.NPARAMS)):
+        e = np.linalg.norm(g[a:b] - rg[a:b]) / np.linalg.norm(rg[a:b])
+        print(f"gradient {name}: rel err {e:.2e}", flush=True)
+        assert e < 1e-2, name
+
+
+def test_training_fits_a_known_conditional(kpax, rng):
+    # targets are a fixed function of the features plus noise of sigma 0.1:
+    # the trained network must beat the best constant predictor by far
+    n = 32768
+    X = rng.normal(0, 1, (n, 80)).astype(np.float32)
+    X[:, 79] = 0
+    A = rng.normal(0, 0.3, (80, 4))
+    Y = (np.tanh(X @ A) + 0.1 * rng.standard_normal((n, 4))).clip(-1, 1).astype(np.float32)
+    with kpax.Trainer(max_batch=4096) as t:
+        t.set_params(kpax.mlp_init_params(3))
+        losses = t.fit(X, Y, epochs=12, batch=1024, lr=3e-3, seed=5)
+        params = t.params()
+    print("epoch losses", np.round(losses, 3), flush=True)
+    out, _ = mlp_oracle.forward(params, X[:4096])
+    mse = np.mean((out[:, :4] - Y[:4096]) ** 2)
+    assert losses[-1] < losses[0] - 1.0
+    assert mse < 0.1 * Y.var()
+
+
+def test_planner_with_proposal(kpax, oracle):
+    p, cfg, obs, start, goal = _planner(kpax, max_tree=1 << 18, max_iter=60)
+    params = kpax.mlp_init_params(2)
+    # a network that always asks for full thrust: mean of control 0 at +1, small sigma
+    params[mlp_oracle.W3:mlp_oracle.B3] = 0
+    params[mlp_oracle.B3:mlp_oracle.B3 + 8] = [3, 0, 0, 0, -4, -1, -1, -1]
+    with p:
+        plain = p.plan(start, goal, seed=4)
+        p.set_proposal(params, 0.5)
+        a = p.plan(start, goal, seed=4)
+        ta = p.tree()
+        b = p.plan(start, goal, seed=4)
+        p.set_proposal(None)
+        again = p.plan(start, goal, seed=4)
+    assert (a.status, a.tree_size, a.expansions) == (b.status, b.tree_size, b.expansions)
+    assert (again.status, again.tree_size, again.expansions, again.iterations) == \
+        (plain.status, plain.tree_size, plain.expansions, plain.iterations)
+    full = np.mean(ta["controls"][0, 1:] == np.float32(cfg.ctrl_max[0]))
+    print(f"nodes at full thrust with the proposal: {full:.2f}", flush=True)
+    assert full > 0.1  # the uniform sampler never hits the bound exactly
+    if a.status == kpax.OK:
+        assert a.verified == 1
+
+
+def test_batch_with_proposal_matches_single_plans(kpax):
+    cfg, obs, start, goal = workload(2, seed=0, max_iter=40)
+    params = kpax.mlp_init_params(9)
+    qs = [kpax.Query(start, goal, seed=60 + i, budget=40000 + 9000 * i) for i in range(6)]
+    with kpax.KPAX(cfg, max_queries=3, node_capacity=250000) as p:
+        p.set_obstacles(obs)
+        p.set_proposal(params, 0.5)
+        res = p.plan_batch(qs)
+    for q, r in zip(qs, res):
+        c = cfg.copy()
+        c.max_tree = q.budget
+        with kpax.KPAX(c) as s:
+            s.set_obstacles(obs)
+            s.set_proposal(params, 0.5)
+            o = s.plan(q.start, q.goal, seed=q.seed)
+        assert (r.status, r.tree_size, r.expansions, r.iterations) == (o.status, o.tree_size, o.expansions,
+                                                                       o.iterations)
+        assert np.array_equal(r.rows.view(np.uint32), o.rows.view(np.uint32))

@@ -374,3 +374,109 @@ def test_oracle_quad_plan_path_verifies_in_fp64(kpax, oracle):
     rows[:, -1] = np.float32(cfg.dt) * o["steps"][path]
     ok, dev = kpax.verify_path(cfg, obs, rows, o["steps"][path], list(goal))
     assert ok and dev < 1e-3
+
+
+# ---- learned control proposal: oracle pins ---------------------------------
+def test_ln_f32_matches_log_within_two_ulp(oracle, kpax):
+    xs = np.concatenate([np.float32(2.0) ** -np.arange(0, 25, dtype=np.float32),
+                         np.random.default_rng(0).uniform(2**-24, 1, 2000).astype(np.float32),
+                         np.array([np.float32(0.70710677), np.float32(0.70710683), np.float32(1.0)])])
+    for x in xs:
+        got = np.float32(oracle.ln_f32(float(x)))
+        want = np.float32(np.log(np.float64(x)))
+        ulp = np.spacing(np.abs(want)) if want != 0 else np.float32(1e-45)
+        assert abs(np.float64(got) - np.float64(want)) <= 2 * ulp, (x, got, want)
+
+
+def test_box_muller_normals_are_standard_normal(oracle, kpax):
+    cfg = kpax.config_workload(2)
+    n = 200000
+    rng = np.random.default_rng(1)
+    w = rng.integers(0, 2**32, (n, 4), dtype=np.uint64).astype(np.uint32)
+    sigma = 0.05
+    ms = np.zeros((n, 8), np.float32)
+    ms[:, 4:] = sigma
+    u = oracle.propose_sample(cfg, ms, w)
+    lo, hi = np.array(cfg.ctrl_min[:4]), np.array(cfg.ctrl_max[:4])
+    z = (2 * (u - lo) / (hi - lo) - 1) / sigma
+    assert np.all(np.abs(z.mean(0)) < 0.01) and np.all(np.abs(z.std(0) - 1) < 0.01)
+    from scipy import stats
+    for j in range(4):
+        assert stats.kstest(z[:, j], "norm").pvalue > 1e-3
+    # the two normals of a pair are uncorrelated
+    assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 0.01
+
+
+def test_proposal_clamps_to_the_control_box(oracle, kpax):
+    cfg = kpax.config_workload(2)
+    ms = np.zeros((4, 8), np.float32)
+    ms[:, :4] = [[5, -5, 5, -5]] * 4
+    ms[:, 4:] = 0.01
+    w = np.arange(16, dtype=np.uint32).reshape(4, 4) * 977
+    u = oracle.propose_sample(cfg, ms, w)
+    assert np.array_equal(u, np.tile(np.array([cfg.ctrl_max[0], cfg.ctrl_min[1], cfg.ctrl_max[2], cfg.ctrl_min[3]],
+                                              np.float32), (4, 1)))
+
+
+def test_features_closed_forms_and_occupancy(oracle, kpax):
+    cfg = kpax.config_workload(2)  # [0,10]^2 x [0,5]; lattice cells 0.625 x 0.625 x 0.3125
+    # one obstacle filling lattice cell (3, 4, 5) exactly: only that cell is set
+    cell = np.array([10 / 16, 10 / 16, 5 / 16], np.float32)
+    lo = np.array([3, 4, 5], np.float32) * cell
+    obs = np.concatenate([lo, lo + cell])[None].astype(np.float32)
+    bits = oracle.occupancy(cfg, obs)
+    idx = (3 * 16 + 4) * 16 + 5
+    assert bits[idx >> 5] == 1 << (idx & 31) and np.count_nonzero(bits) == 1
+    # a state at the workspace centre, at rest, goal at the centre too
+    x = np.zeros((12, 2), np.float32)
+    x[:3, 0] = [5, 5, 2.5]
+    x[:3, 1] = [-3, 5, 2.5]  # outside: every window cell left of the workspace is occupied
+    x[3:6, 0] = [np.pi, -np.pi / 2, 0]
+    x[6:9, 0] = [3, -1.5, 0]
+    x[9:12, 0] = [4, 0, -2]
+    g = np.array([[5, 10], [5, 5], [2.5, 2.5]], np.float32)
+    f = oracle.mlp_features(cfg, obs, x, g)
+    assert np.array_equal(f[0, :3], [0, 0, 0]) and np.array_equal(f[0, 12:15], [0, 0, 0])
+    assert np.allclose(f[0, 3:6], [1, -0.5, 0], atol=1e-7)
+    assert np.allclose(f[0, 6:9], [1, -0.5, 0]) and np.allclose(f[0, 9:12], [1, 0, -0.5])
+    assert np.allclose(f[1, 12:15], [13 / 5, 0, 0])
+    occ0 = f[0, 15:79].reshape(4, 4, 4)
+    # centre (8, 8, 8) in cells -> window cells 6..9 on every axis; cell (3,4,5) is outside it
+    assert occ0.sum() == 0
+    occ1 = f[1, 15:79].reshape(4, 4, 4)  # x = -3 -> cells -7..-4: all outside the workspace
+    assert occ1.min() == 1
+    assert f[0, 79] == 0
+
+
+def test_mlp_oracle_gradients_match_finite_differences(oracle, kpax):
+    import mlp_oracle
+    rng = np.random.default_rng(2)
+    p = kpax.mlp_init_params(4).astype(np.float64)
+    p += rng.normal(0, 0.05, p.shape)
+    X = rng.normal(0, 1, (7, 80))
+    X[:, 79] = 0
+    Y = rng.uniform(-1, 1, (7, 4))
+    _, g = mlp_oracle.loss_and_grads(p, X, Y, exact=True)
+    idx = np.concatenate([rng.choice(np.arange(mlp_oracle.W1, mlp_oracle.B1), 30, replace=False),
+                          np.arange(mlp_oracle.B1, mlp_oracle.B1 + 5),
+                          rng.choice(np.arange(mlp_oracle.W2, mlp_oracle.B2), 30, replace=False),
+                          np.arange(mlp_oracle.B2, mlp_oracle.B2 + 5),
+                          np.arange(mlp_oracle.W3, mlp_oracle.W3 + 40), np.arange(mlp_oracle.B3, mlp_oracle.B3 + 8)])
+    h = 1e-6
+    for i in idx:
+        q = p.copy()
+        q[i] += h
+        lp, _ = mlp_oracle.loss_and_grads(q, X, Y, exact=True)
+        q[i] -= 2 * h
+        lm, _ = mlp_oracle.loss_and_grads(q, X, Y, exact=True)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
+
+
+def test_bf16_rounding_is_nearest_even():
+    import mlp_oracle
+    # bf16 keeps 7 fraction bits: the spacing at 1 is 2^-7
+    x = np.array([1.0, 1.0 + 2**-8, 1.0 + 3 * 2**-8, 1.0 + 3 * 2**-9, 1.0 + 2**-9, -2.5], np.float32)
+    r = mlp_oracle.bf16(x)
+    # halfway cases go to the even neighbour: 1 + 2^-8 -> 1, 1 + 3 2^-8 -> 1 + 2^-6
+    assert list(r) == [1.0, 1.0, 1.0 + 2**-6, 1.0 + 2**-7, 1.0, -2.5]
