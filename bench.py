@@ -4,9 +4,23 @@
 A step is one complete plan() of the 12D quadcopter forest workload (400
 columns in [0,10]^2 x [0,5], RK4 dt = 0.02, 2^20-node tree cap, 16^3 regions,
 64^3 collision grid), each step with its own seed.  Inputs (obstacles, grid)
-are resident in HBM before timing starts.  Prints one JSON line.
+are resident in HBM before timing starts.  The timed plans run as they ship
+(each iteration one captured HIP graph, no profiling events); a second,
+profiled pass over the same seeds gives the per-phase device times and the
+expansion kernel's roofline figures (the plans are deterministic, so both
+passes do identical work).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+Also measured, each on the same GPU and stated beside the headline:
+  batched  256 queries over 32 environments (16 forests of 100-600 columns,
+           16 mazes of 2,000 boxes), budgets log-uniform in [16k, 1M] nodes,
+           in one kpax_plan_batch call, against the same queries planned one
+           at a time; every query's result is identical in both forms, so
+           the success rates match by construction
+  proposal the learned control proposal (50/50 mixture with uniform) against
+           the plain sampler on held-out forests and queries at a 2^16 budget
+Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-proposal]
 """
 import argparse
 import json
@@ -22,11 +36,14 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import numpy as np  # noqa: E402
 
 import kpax  # noqa: E402
+import proposal  # noqa: E402
 from workloads import workload  # noqa: E402
 
 METRIC = "p50 time-to-first-solution & plans/sec at ≥95% success; node-expansions/sec"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+PARAMS = os.path.join(ROOT, "kino-pax-fork_amd", "data", "proposal_forest.npy")
+
 
 # Algorithmic bytes per expansion: parent state read (4S), child sample write
 # (4(S+C+1)), parent index (4), admission flag (1), two region counters (8).
@@ -58,6 +75,74 @@ def cpu_baseline(k, env_seed, plans, seeds):
                       f"(OpenMP over expansions); p50 time-to-solution {statistics.median(ms):.0f} ms"}
 
 
+def serving_workload(cfg, n_queries=256, n_env=32, seed=0):
+    rng = np.random.default_rng(seed)
+    envs = []
+    for e in range(n_env):
+        if e % 2 == 0:
+            envs.append(kpax.generate_environment(2, 500 + e, int(rng.integers(100, 601)), cfg.ws_min, cfg.ws_max))
+        else:
+            envs.append(kpax.generate_environment(3, 500 + e, 2000, cfg.ws_min, cfg.ws_max))
+    qs = proposal.serving_queries(cfg, envs, n_queries, seed=7, min_dist=3.0, budget_range=(16384, 1 << 20))
+    return envs, qs
+
+
+def batched(cfg):
+    envs, qs = serving_workload(cfg)
+    T = sum(q.budget for q in qs) + 1024
+    with kpax.KPAX(cfg, max_queries=len(qs), node_capacity=T) as p:
+        for e, o in enumerate(envs):
+            p.set_environment(e, o)
+        p.plan_batch(qs[:4])
+        p.reset_stats()
+        t0 = time.perf_counter()
+        res = p.plan_batch(qs)
+        wall = time.perf_counter() - t0
+        st = p.stats()
+        launch = p.launch()
+    with kpax.KPAX(cfg, max_queries=1, node_capacity=1 << 20) as s:
+        for e, o in enumerate(envs):
+            s.set_environment(e, o)
+        s.plan_batch(qs[:2])
+        t0 = time.perf_counter()
+        seq = [s.plan_batch([q])[0] for q in qs]
+        wall_seq = time.perf_counter() - t0
+    same = all((a.status, a.tree_size, a.expansions, a.iterations) == (b.status, b.tree_size, b.expansions,
+                                                                       b.iterations) for a, b in zip(res, seq))
+    ok = [r for r in res if r.status == kpax.OK and r.verified == 1]
+    lat = [r.plan_ms for r in res]
+    return {"queries": len(qs), "environments": len(envs), "budget_nodes_total": T - 1024,
+            "plans_per_sec": len(qs) / wall, "wall_s": wall, "success_rate": len(ok) / len(qs),
+            "p50_latency_ms": statistics.median(lat), "p99_latency_ms": float(np.percentile(lat, 99)),
+            "node_expansions_per_sec": sum(r.expansions for r in res) / wall,
+            "iterations": st["iterations"], "graph_launches": st["graph_launches"], "launch": launch,
+            "single_query_form": {"plans_per_sec": len(qs) / wall_seq, "wall_s": wall_seq,
+                                  "p50_time_to_solution_ms": statistics.median(r.plan_ms for r in seq),
+                                  "success_rate": sum(r.status == kpax.OK and r.verified == 1 for r in seq) / len(seq)},
+            "batched_over_single_plans_per_sec": (len(qs) / wall) / (len(qs) / wall_seq),
+            "results_identical_to_single_query_form": same}
+
+
+def learned_proposal(cfg):
+    if not os.path.exists(PARAMS):
+        return {"skipped": f"{PARAMS} not found"}
+    params = np.load(PARAMS)
+    c = cfg.copy()
+    c.max_tree = 1 << 16
+    c.max_iter = 100
+    envs = proposal.forest_environments(c, 8, seed=2)
+    qs = proposal.serving_queries(c, envs, 96, seed=4)
+    with kpax.KPAX(c) as p:
+        for e, o in enumerate(envs):
+            p.set_environment(e, o)
+        plain = proposal.evaluate(p, qs, None)
+        learned = proposal.evaluate(p, qs, params, 0.5)
+    strip = lambda d: {k: v for k, v in d.items() if k != "results"}  # noqa: E731
+    return {"workload": "96 held-out queries (start/goal >= 4 m apart) over 8 held-out forests, 2^16-node budget, "
+                        "max_iter 100, single-query form",
+            "plain": strip(plain), "learned_mix_0.5": strip(learned)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +152,8 @@ def main():
     ap.add_argument("--env-seed", type=int, default=0)
     ap.add_argument("--cpu-plans", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-proposal", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the byte counters)")
     args = ap.parse_args()
@@ -92,13 +179,19 @@ def main():
     for s in seeds[: args.warmup]:
         planner.plan(start, goal, seed=s)
 
-    planner.set_profiling(True)
+    # timed: the plans as they ship
     planner.reset_stats()
     results = []
     t0 = time.perf_counter()
     for s in seeds[args.warmup:]:
         results.append(planner.plan(start, goal, seed=s))
     wall = time.perf_counter() - t0
+    graph_launches = planner.stats()["graph_launches"]
+    # profiled pass over the same seeds: per-phase device time from HIP events
+    planner.set_profiling(True)
+    planner.reset_stats()
+    for s in seeds[args.warmup:]:
+        planner.plan(start, goal, seed=s)
     st = planner.stats()
     planner.close()
 
@@ -141,6 +234,7 @@ def main():
         "mean_iterations": statistics.mean(r.iterations for r in results),
         "expansions_per_plan": expansions / len(results),
         "integrated_steps": steps_int,
+        "graph_launches": graph_launches,
         "phase_ms_per_plan": {k: v / max(1, st["plans"]) for k, v in st["phase_ms"].items()},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
@@ -150,6 +244,10 @@ def main():
                           "unit": "TFLOP/s", "frac": achieved_tflops / FP32_PEAK_TFLOPS,
                           "flop_per_step": FLOP_PER_STEP[cfg.model]},
     }
+    if not args.no_batch:
+        line["batched"] = batched(cfg)
+    if not args.no_proposal:
+        line["learned_proposal"] = learned_proposal(cfg)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload, args.env_seed, args.cpu_plans, seeds[args.warmup:])
     print(json.dumps(line), flush=True)

@@ -112,6 +112,16 @@ __device__ __forceinline__ int cell_of(float x, float mn, float cell, int D)
 // (isMotionValidSpatialHash, spatialHash.cu:145-194; broad phase
 // collisionCheck.cu:3-10).  Any obstacle that overlaps the box shares at
 // least one cell with it, so the verdict equals the brute-force one.
+__device__ __forceinline__ bool box_hits(const GridView& g, int k, const float bmin[3], const float bmax[3])
+{
+    const int o    = g.items[k];
+    const float4 a = g.lo[o];
+    const float4 b = g.hi[o];
+    const bool sep = bmax[0] <= a.x || b.x <= bmin[0] || bmax[1] <= a.y || b.y <= bmin[1] || bmax[2] <= a.z ||
+                     b.z <= bmin[2];
+    return !sep;
+}
+
 __device__ __forceinline__ bool seg_free(const GridView& g, const float bmin[3], const float bmax[3])
 {
     int lo[3], hi[3];
@@ -121,22 +131,37 @@ __device__ __forceinline__ bool seg_free(const GridView& g, const float bmin[3],
         lo[d] = cell_of(bmin[d], g.gp.mn[d], g.gp.cell[d], g.gp.D[d]);
         hi[d] = cell_of(bmax[d], g.gp.mn[d], g.gp.cell[d], g.gp.D[d]);
     }
+    if(hi[0] - lo[0] <= 1 && hi[1] - lo[1] <= 1 && hi[2] - lo[2] <= 1)
+    {
+        // An integration step spans at most 2 cells per axis: the cell-start
+        // words of its (up to) four x-rows are loaded together, so the lookup
+        // costs one memory latency instead of one per cell.
+        const int nx = hi[0] - lo[0];
+        int b[4], e[4];
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+        {
+            const int y = lo[1] + (r & 1), z = lo[2] + (r >> 1);
+            const bool in = y <= hi[1] && z <= hi[2];
+            const int c   = lo[0] + g.gp.D[0] * ((in ? y : lo[1]) + g.gp.D[1] * (in ? z : lo[2]));
+            b[r]          = in ? g.start[c] : 0;
+            e[r]          = in ? g.start[c + 1 + nx] : 0;
+        }
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+            for(int k = b[r]; k < e[r]; ++k)
+                if(box_hits(g, k, bmin, bmax)) return false;
+        return true;
+    }
     for(int z = lo[2]; z <= hi[2]; ++z)
         for(int y = lo[1]; y <= hi[1]; ++y)
-            for(int x = lo[0]; x <= hi[0]; ++x)
-            {
-                const int c = x + g.gp.D[0] * (y + g.gp.D[1] * z);
-                const int e = g.start[c + 1];
-                for(int k = g.start[c]; k < e; ++k)
-                {
-                    const int o    = g.items[k];
-                    const float4 a = g.lo[o];
-                    const float4 b = g.hi[o];
-                    const bool sep = bmax[0] <= a.x || b.x <= bmin[0] || bmax[1] <= a.y || b.y <= bmin[1] ||
-                                     bmax[2] <= a.z || b.z <= bmin[2];
-                    if(!sep) return false;
-                }
-            }
+        {
+            // the cells of one x-row are contiguous in the CSR
+            const int c0 = lo[0] + g.gp.D[0] * (y + g.gp.D[1] * z);
+            const int e  = g.start[c0 + 1 + hi[0] - lo[0]];
+            for(int k = g.start[c0]; k < e; ++k)
+                if(box_hits(g, k, bmin, bmax)) return false;
+        }
     return true;
 }
 

@@ -37,9 +37,10 @@ def _free(p, obs, margin):
     return not np.any(np.all((p >= lo) & (p <= hi), axis=1))
 
 
-def serving_queries(cfg, envs, n, seed, budget=0, min_dist=4.0, budget_range=None):
-    """n start/goal pairs at least `min_dist` apart, both clear of obstacles by
-    half a metre and of the workspace faces by 0.5, each in environment i % len(envs)."""
+def serving_queries(cfg, envs, n, seed, budget=0, min_dist=4.0, budget_range=None, clearance=0.2, tries=100000):
+    """n start/goal pairs at least `min_dist` apart, both `clearance` away from
+    every obstacle and 0.5 inside the workspace faces, each in environment
+    i % len(envs)."""
     rng = np.random.default_rng(seed)
     lo = np.array(cfg.ws_min, np.float64) + 0.5
     hi = np.array(cfg.ws_max, np.float64) - 0.5
@@ -47,10 +48,13 @@ def serving_queries(cfg, envs, n, seed, budget=0, min_dist=4.0, budget_range=Non
     S = 12 if cfg.model == kpax.MODEL_QUADCOPTER else 6
     for i in range(n):
         env = i % len(envs)
-        while True:
+        for _ in range(tries):
             a, b = rng.uniform(lo, hi), rng.uniform(lo, hi)
-            if np.linalg.norm(a - b) >= min_dist and _free(a, envs[env], 0.5) and _free(b, envs[env], 0.5):
+            if np.linalg.norm(a - b) >= min_dist and _free(a, envs[env], clearance) and \
+                    _free(b, envs[env], clearance):
                 break
+        else:
+            raise RuntimeError(f"no free start/goal pair in environment {env} after {tries} draws")
         st = np.zeros(S, np.float32)
         st[:3] = a
         bud = budget
