@@ -25,14 +25,14 @@ struct TreeView {
     int32_t* reg;
 };
 
+// Children of an iteration: one 96-byte record per child (six 16-byte
+// words), written by one lane with six vector stores when the child
+// finishes.  Words 0-11 state, 12-15 control, 16 integrated steps, 17
+// parent, 18 R1 region, 19 R2 cell, 20 admission flags (kValid |
+// kCandidate | kScoreAdmit).
+constexpr int kChildWords = 24;
 struct ChildView {
-    float* st;
-    float* ct;
-    int32_t* steps;
-    int32_t* parent;
-    int32_t* reg;
-    int32_t* r2;
-    uint8_t* adm;  // kValid | kCandidate | kScoreAdmit
+    uint4* rec;  // kChildWords / 4 uint4 per child
 };
 
 // Region state of slot q: counters and scores at q * R1, sub-region cells at

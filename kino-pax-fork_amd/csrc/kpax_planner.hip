@@ -377,16 +377,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
                     if(u01(admit_draw) < rv.score[ro + r1]) flags |= kScoreAdmit;
                 }
                 const uint32_t o = base + c;
+                uint32_t w[kChildWords];
 #pragma unroll
-                for(int d = 0; d < S; ++d) ch.st[(size_t)d * T + o] = x[d];
+                for(int d = 0; d < 12; ++d) w[d] = d < S ? __builtin_bit_cast(uint32_t, x[d < S ? d : 0]) : 0u;
 #pragma unroll
-                for(int d = 0; d < C; ++d) ch.ct[(size_t)d * T + o] = u[d];
-                ch.steps[o]  = i;  // steps integrated: n for a valid child
-                ch.parent[o] = (int32_t)parent;
-                ch.reg[o]    = r1;
-                ch.r2[o]     = r2;
-                ch.adm[o]    = (uint8_t)flags;
-                c            = kNone;
+                for(int d = 0; d < 4; ++d) w[12 + d] = d < C ? __builtin_bit_cast(uint32_t, u[d < C ? d : 0]) : 0u;
+                w[16] = (uint32_t)i;  // steps integrated: n for a valid child
+                w[17] = parent;
+                w[18] = (uint32_t)r1;
+                w[19] = (uint32_t)r2;
+                w[20] = flags;
+                w[21] = w[22] = w[23] = 0u;
+                uint4* rec = ch.rec + (size_t)o * (kChildWords / 4);
+#pragma unroll
+                for(int v = 0; v < kChildWords / 4; ++v)
+                    rec[v] = make_uint4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
+                c = kNone;
             }
         }
     }
@@ -485,11 +491,18 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
         for(int j = 0; j < 4; ++j)
         {
             const uint32_t c = cbase + j;
-            const uint32_t a = c < M ? ch.adm[base + c] : 0u;
+            uint4 meta       = make_uint4(0u, 0u, 0u, 0u);  // steps, parent, region, R2 cell
+            uint32_t a       = 0u;
             if(c < M)
             {
-                steps += (uint32_t)ch.steps[base + c];
-                const int r = ch.reg[base + c];
+                const uint4* rec = ch.rec + (base + c) * (kChildWords / 4);
+                meta             = rec[4];
+                a                = rec[5].x;
+            }
+            if(c < M)
+            {
+                steps += meta.x;
+                const int r = (int)meta.z;
                 if(local)
                 {
                     atomicAdd(&s_hist[r], 1);
@@ -506,8 +519,8 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
             bool first_in_cell = false;
             if(a & kCandidate)
             {
-                first_in_cell = r2s[ch.r2[base + c]] == cell_key(it.t, c);
-                if(first_in_cell) atomicAdd(&cov[ch.reg[base + c]], 1);
+                first_in_cell = r2s[meta.w] == cell_key(it.t, c);
+                if(first_in_cell) atomicAdd(&cov[meta.z], 1);
             }
             f[j] = (a & kValid) && ((a & kScoreAdmit) || first_in_cell) ? 1u : 0u;
             cnt += f[j];
@@ -529,14 +542,26 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
                 ctl[q].overflow = 1;  // cannot happen while M <= cap - ts; never write past the range
                 break;
             }
-            const size_t o = base + dst;
-            for(int dd = 0; dd < S; ++dd) tree.st[(size_t)dd * T + o] = ch.st[(size_t)dd * T + c];
-            for(int dd = 0; dd < C; ++dd) tree.ct[(size_t)dd * T + o] = ch.ct[(size_t)dd * T + c];
-            tree.steps[o]  = ch.steps[c];
-            tree.parent[o] = ch.parent[c];
-            tree.reg[o]    = ch.reg[c];
-            const float dx = ch.st[c] - d.goal[0], dy = ch.st[(size_t)T + c] - d.goal[1],
-                        dz = ch.st[2 * (size_t)T + c] - d.goal[2];
+            const size_t o   = base + dst;
+            const uint4* rec = ch.rec + c * (kChildWords / 4);
+            uint32_t w[20];
+#pragma unroll
+            for(int v = 0; v < 5; ++v)
+            {
+                const uint4 q4 = rec[v];
+                w[4 * v]       = q4.x;
+                w[4 * v + 1]   = q4.y;
+                w[4 * v + 2]   = q4.z;
+                w[4 * v + 3]   = q4.w;
+            }
+            for(int dd = 0; dd < S; ++dd) tree.st[(size_t)dd * T + o] = __builtin_bit_cast(float, w[dd]);
+            for(int dd = 0; dd < C; ++dd) tree.ct[(size_t)dd * T + o] = __builtin_bit_cast(float, w[12 + dd]);
+            tree.steps[o]  = (int32_t)w[16];
+            tree.parent[o] = (int32_t)w[17];
+            tree.reg[o]    = (int32_t)w[18];
+            const float dx = __builtin_bit_cast(float, w[0]) - d.goal[0],
+                        dy = __builtin_bit_cast(float, w[1]) - d.goal[1],
+                        dz = __builtin_bit_cast(float, w[2]) - d.goal[2];
             if(__builtin_sqrtf(dx * dx + dy * dy + dz * dz) < PP.goal_radius) atomicMin(&ctl[q].goal, (int32_t)dst);
             ++dst;
         }
@@ -1277,13 +1302,7 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->tree.steps, T));
     TRY(dalloc(h, &h->tree.parent, T));
     TRY(dalloc(h, &h->tree.reg, T));
-    TRY(dalloc(h, &h->ch.st, T * h->S));
-    TRY(dalloc(h, &h->ch.ct, T * h->C));
-    TRY(dalloc(h, &h->ch.steps, T));
-    TRY(dalloc(h, &h->ch.parent, T));
-    TRY(dalloc(h, &h->ch.reg, T));
-    TRY(dalloc(h, &h->ch.r2, T));
-    TRY(dalloc(h, &h->ch.adm, T));
+    TRY(dalloc(h, &h->ch.rec, T * (kChildWords / 4)));
     TRY(dalloc(h, &h->R, T));
     const size_t r1 = (size_t)h->RP.r1_count * Q, r2 = (size_t)h->R2 * Q;
     TRY(dalloc(h, &h->rv.valid, r1));
