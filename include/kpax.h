@@ -83,6 +83,14 @@ typedef struct kpax_config {
     int32_t verify;            /* 1: re-propagate every returned trajectory in FP64 and
                                   re-check it against all obstacles on the host          */
     int32_t device;            /* HIP device ordinal                                      */
+    /* Goal-progress pruning of PruneKPAX (src/planners/PruneKPAX.cu:466-501): an
+     * admitted child is kept with probability
+     *   clamp(exploration + goal_bias * clamp(progress / progress_scale + 1/2, 0, 1), eps, 1)
+     * where progress = parent's goal distance - child's goal distance.  0 = off. */
+    int32_t prune;
+    float   prune_exploration;     /* h_explorationBias_ (default 0.3)                     */
+    float   prune_goal_bias;       /* h_goalBias_ (default 0.7)                            */
+    float   prune_progress_scale;  /* h_progressScale_ (default 10)                        */
 } kpax_config;
 
 /* Trajectory returned by kpax_plan.  Replaces the public fields

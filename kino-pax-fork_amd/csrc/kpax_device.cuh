@@ -46,7 +46,13 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
     return make_uint4(c0, c1, c2, c3);
 }
 
-enum : uint32_t { PURPOSE_CONTROL = 0, PURPOSE_STEPS_ADMIT = 1, PURPOSE_READMIT = 2, PURPOSE_PROPOSAL = 3 };
+enum : uint32_t {
+    PURPOSE_CONTROL     = 0,
+    PURPOSE_STEPS_ADMIT = 1,
+    PURPOSE_READMIT     = 2,
+    PURPOSE_PROPOSAL    = 3,
+    PURPOSE_PRUNE       = 4
+};
 
 __device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
 
@@ -526,6 +532,24 @@ __device__ __forceinline__ float f_accept(uint32_t t, uint32_t tree_plus_new, ui
     const double x  = (double)tas;
     const double p5 = x * x * x * x * x;
     return (float)((double)((float)t * eps) * p5);
+}
+
+// Goal-progress pruning (prune_updateFrontier_kernel, PruneKPAX.cu:472-489):
+// keep probability from the parent's and the child's goal distances (xyz,
+// helper.cuh:138-152), compared with the child's own uniform draw.
+__device__ __forceinline__ bool prune_keep(const PlanParams& PP, const float* child, const float* parent,
+                                           const float* goal, uint32_t draw)
+{
+    const float cx = child[0] - goal[0], cy = child[1] - goal[1], cz = child[2] - goal[2];
+    const float px = parent[0] - goal[0], py = parent[1] - goal[1], pz = parent[2] - goal[2];
+    const float cd       = __builtin_sqrtf(cx * cx + cy * cy + cz * cz);
+    const float pd       = __builtin_sqrtf(px * px + py * py + pz * pz);
+    const float progress = pd - cd;
+    float np             = progress / PP.prune_scale + 0.5f;
+    np                   = np < 0.0f ? 0.0f : (np > 1.0f ? 1.0f : np);
+    float p              = PP.prune_expl + PP.prune_goal * np;
+    p                    = p < PP.eps ? PP.eps : (p > 1.0f ? 1.0f : p);
+    return u01(draw) < p;
 }
 
 // Fan-out of the next iteration (KPAX.cu:323-333), capped so the children

@@ -50,6 +50,9 @@ std::string validate_config(const kpax_config& c)
     if(c.sparse_repeat > 1024 || c.dense_repeat > 1024) return "repeat counts must be <= 1024";
     if(!(c.eps > 0.0f)) return "eps must be positive";
     if(c.device < 0) return "bad device";
+    if(c.prune && !(c.prune_progress_scale > 0.0f && std::isfinite(c.prune_exploration) &&
+                    std::isfinite(c.prune_goal_bias)))
+        return "pruning needs a positive progress scale and finite biases";
     return "";
 }
 
@@ -135,6 +138,10 @@ PlanParams make_plan_params(const kpax_config& c)
     p.sparse_valid_below = c.sparse_valid_below;
     p.eps                = c.eps;
     p.goal_radius        = c.goal_radius;
+    p.prune              = c.prune;
+    p.prune_expl         = c.prune_exploration;
+    p.prune_goal         = c.prune_goal_bias;
+    p.prune_scale        = c.prune_progress_scale;
     return p;
 }
 
@@ -352,6 +359,10 @@ int kpax_config_workload(int32_t workload, kpax_config* out)
     c.eps                = 1e-2f;  // EPSILON, config.h:263
     c.max_iter           = 300;  // MAX_ITER, config.h:212
     c.verify             = 1;
+    c.prune                = 0;      // PruneKPAX defaults, PruneKPAX.cu:34-36
+    c.prune_exploration    = 0.3f;
+    c.prune_goal_bias      = 0.7f;
+    c.prune_progress_scale = 10.0f;
     c.steps_min          = 5;
     c.steps_max          = 20;
     c.gravity            = -9.81f;  // GRAVITY, config.h:306

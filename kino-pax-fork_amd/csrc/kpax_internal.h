@@ -104,6 +104,8 @@ struct PlanParams {
     int32_t sparse_valid_below;
     float eps;
     float goal_radius;
+    int32_t prune;  // goal-progress pruning (PruneKPAX) on/off
+    float prune_expl, prune_goal, prune_scale;
 };
 
 int state_dim(int model);

@@ -523,6 +523,18 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
                 if(first_in_cell) atomicAdd(&cov[meta.z], 1);
             }
             f[j] = (a & kValid) && ((a & kScoreAdmit) || first_in_cell) ? 1u : 0u;
+            if(f[j] && PP.prune)
+            {
+                // goal-progress pruning (PruneKPAX): decided before the scan, so
+                // the kept children are appended without holes
+                const float4 cxyz = __builtin_bit_cast(float4, ch.rec[(base + c) * (kChildWords / 4)]);
+                const size_t pn   = base + meta.y;
+                const float child[3]  = {cxyz.x, cxyz.y, cxyz.z};
+                const float parent[3] = {tree.st[pn], tree.st[(size_t)T + pn], tree.st[2 * (size_t)T + pn]};
+                const uint4 dp        = philox(c, it.t, d.query, PURPOSE_PRUNE, (uint32_t)d.seed,
+                                               (uint32_t)(d.seed >> 32));
+                f[j] = prune_keep(PP, child, parent, d.goal, dp.x) ? 1u : 0u;
+            }
             cnt += f[j];
         }
         uint32_t agg, steps_tile;

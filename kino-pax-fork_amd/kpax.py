@@ -43,6 +43,8 @@ class Config(ctypes.Structure):
         ("fanout", ctypes.c_uint32), ("root_repeat", ctypes.c_uint32), ("sparse_repeat", ctypes.c_uint32),
         ("dense_repeat", ctypes.c_uint32), ("sparse_valid_below", ctypes.c_int32), ("eps", ctypes.c_float),
         ("verify", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("prune", ctypes.c_int32), ("prune_exploration", ctypes.c_float), ("prune_goal_bias", ctypes.c_float),
+        ("prune_progress_scale", ctypes.c_float),
     ]
 
     def copy(self):

@@ -623,6 +623,24 @@ struct Planner {
                     fresh = true;
                 }
                 adm[ci] = (csadm[ci] || fresh) ? 1 : 0;
+                if(adm[ci] && c.prune)
+                {
+                    // goal-progress pruning (prune_updateFrontier_kernel, PruneKPAX.cu:472-489);
+                    // decided before the append, so the kept children leave no holes
+                    const float* x1 = &cs[(size_t)ci * S];
+                    const float* x0 = &st[(size_t)cpar[ci] * S];
+                    const float cx = x1[0] - goal[0], cy = x1[1] - goal[1], cz = x1[2] - goal[2];
+                    const float px = x0[0] - goal[0], py = x0[1] - goal[1], pz = x0[2] - goal[2];
+                    const float cd = std::sqrt(cx * cx + cy * cy + cz * cz);
+                    const float pd = std::sqrt(px * px + py * py + pz * pz);
+                    const float progress = pd - cd;
+                    float np = progress / c.prune_progress_scale + 0.5f;
+                    np       = np < 0.0f ? 0.0f : (np > 1.0f ? 1.0f : np);
+                    float pk = c.prune_exploration + c.prune_goal_bias * np;
+                    pk       = pk < c.eps ? c.eps : (pk > 1.0f ? 1.0f : pk);
+                    const U4 dp = draw(seed, ci, t, query, 4u);
+                    adm[ci]     = u01(dp.v[0]) < pk ? 1 : 0;
+                }
             }
             // scores (Graph::updateVertices, Graph.cu:207-227)
             scores(c, R, valid.data(), total.data(), covered.data(), score.data(), &score_total);

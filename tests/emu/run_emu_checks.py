@@ -90,6 +90,16 @@ def main():
 
     batch_checks()
     mlp_checks()
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=1500, max_iter=8)
+    small(cfg)
+    cfg.prune = 1
+    with kpax.KPAX(cfg) as p:
+        r = p.plan(start, goal, obs, seed=21)
+        tree = p.tree()
+    o = oracle.plan(cfg, obs, start, goal, seed=21, threads=4)
+    check(r.tree_size == o["tree_size"] and np.array_equal(tree["parents"], o["parents"])
+          and np.array_equal(tree["states"].view(np.uint32), o["states"].view(np.uint32)),
+          f"pruned plan: tree {r.tree_size} vs oracle {o['tree_size']}")
 
 
 def mlp_checks():

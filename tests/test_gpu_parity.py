@@ -193,6 +193,22 @@ def test_plan_tree_bit_exact_vs_oracle(kpax, oracle, k, seed, kw):
     _compare_plans(kpax, oracle, k, seed, **kw)
 
 
+@pytest.mark.parametrize("k,kw", [(1, dict(max_tree=30000, max_iter=12)), (2, dict(max_tree=1 << 17, max_iter=25))])
+def test_plan_with_goal_progress_pruning_bit_exact(kpax, oracle, k, kw):
+    cfg, obs, start, goal = workload(k, seed=2, **kw)
+    cfg.prune = 1  # PruneKPAX defaults: exploration 0.3, goal bias 0.7, progress scale 10
+    with kpax.KPAX(cfg) as p:
+        r = p.plan(start, goal, obs, seed=11)
+        tree = p.tree()
+    o = oracle.plan(cfg, obs, start, goal, seed=11, threads=8)
+    print(f"pruned plan workload {k}: gpu {r.status} it {r.iterations} ts {r.tree_size}; oracle {o['status']} "
+          f"it {o['iterations']} ts {o['tree_size']}", flush=True)
+    assert (r.status, r.iterations, r.tree_size, r.expansions) == (o["status"], o["iterations"], o["tree_size"],
+                                                                   o["expansions"])
+    assert np.array_equal(tree["parents"], o["parents"])
+    assert np.array_equal(tree["states"].view(np.uint32), o["states"].view(np.uint32))
+
+
 def test_plan_quadcopter_full_size_finds_verified_path(kpax):
     cfg, obs, start, goal = workload(2, seed=0)
     with kpax.KPAX(cfg) as p:

@@ -480,3 +480,29 @@ def test_bf16_rounding_is_nearest_even():
     r = mlp_oracle.bf16(x)
     # halfway cases go to the even neighbour: 1 + 2^-8 -> 1, 1 + 3 2^-8 -> 1 + 2^-6
     assert list(r) == [1.0, 1.0, 1.0 + 2**-6, 1.0 + 2**-7, 1.0, -2.5]
+
+
+# ---- goal-progress pruning (PruneKPAX.cu:466-501) ----------------------------
+def test_pruning_with_certain_acceptance_is_the_plain_planner(oracle, kpax):
+    cfg, obs, start, goal = workload(1, seed=0, max_tree=20000, max_iter=8)
+    plain = oracle.plan(cfg, obs, start, goal, seed=3, threads=4)
+    cfg.prune, cfg.prune_exploration, cfg.prune_goal_bias = 1, 1.0, 0.0
+    pruned = oracle.plan(cfg, obs, start, goal, seed=3, threads=4)
+    assert pruned["tree_size"] == plain["tree_size"] and np.array_equal(pruned["parents"], plain["parents"])
+
+
+def test_pruning_keeps_goal_ward_children(oracle, kpax):
+    # exploration 0, goal bias 1, a small progress scale: only children that move
+    # toward the goal survive, so every kept edge ends nearer the goal than it began
+    cfg, obs, start, goal = workload(1, seed=0, max_tree=20000, max_iter=6, n_obs=0)
+    cfg.prune, cfg.prune_exploration, cfg.prune_goal_bias, cfg.prune_progress_scale = 1, 0.0, 1.0, 1e-4
+    o = oracle.plan(cfg, obs, start, goal, seed=5, threads=4)
+    st, par = o["states"], o["parents"]
+    d = np.linalg.norm(st[:3].T - np.asarray(goal[:3], np.float32), axis=1)
+    kept = np.arange(1, o["tree_size"])
+    closer = d[kept] < d[par[kept]]
+    # p = eps for children that move away; almost none of those survive
+    assert closer.mean() > 0.97, closer.mean()
+    plain = cfg.copy()
+    plain.prune = 0
+    assert o["tree_size"] < oracle.plan(plain, obs, start, goal, seed=5, threads=4)["tree_size"]
