@@ -206,6 +206,18 @@ typedef struct kpax_query {
  * Every out[i] must be released with kpax_result_free. */
 int kpax_plan_batch(kpax_handle* h, const kpax_query* queries, uint32_t n, kpax_result* out);
 
+/* kpax_plan plus the per-iteration dumps of KPAX::planBench
+ * (src/planners/KPAX.cu:245-300, writeDeviceVectorsToCSV :544-618): under
+ * `dir` (the reference's "Data"), Samples/Samples<bench_itr>/samples<itr>.csv,
+ * Parents/..., TotalCountPerVertex/..., ValidCountPerVertex/..., Frontier/...,
+ * FrontierRepeatCount/..., VertexScores/..., FrontierSize/..., TreeSize/...,
+ * ExpandedNodes/..., ControlPathToGoal/... and ExecutionTime/executionTime.csv,
+ * in the reference's CSV formats (helper.cuh:57-108), so its visualisers
+ * (viz/KGMT_3D*.m) read them.  Each iteration drains the stream first, as
+ * planBench does. */
+int kpax_plan_dump(kpax_handle* h, const float* start, const float* goal, uint64_t seed,
+                   const char* dir, int32_t bench_itr, kpax_result* out);
+
 /* Free the arrays of a result filled by kpax_plan or kpax_plan_batch. */
 void kpax_result_free(kpax_result* r);
 

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -139,6 +140,8 @@ struct kpax_handle {
     // profiling; KPAX_TRACE=1 prints every iteration record to stderr
     bool profiling = false;
     bool trace     = false;
+    // called after every iteration (with the stream drained) by kpax_plan_dump
+    std::function<int(uint64_t)> iter_hook;
     kpax_stats stats{};
     std::vector<void*> allocs;
 };

@@ -32,8 +32,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <functional>
+#include <iomanip>
 #include <map>
+#include <sstream>
 #include <new>
 #include <string>
 #include <thread>
@@ -966,7 +970,7 @@ int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
     HIPC(hipSetDevice(h->cfg.device));
     if(!h->graph_tried) capture_graphs(h);
     const bool prof  = h->profiling;
-    const bool graph = h->use_graph && !prof && !h->trace;
+    const bool graph = h->use_graph && !prof && !h->trace && !h->iter_hook;
     NodeRanges ranges(h->T);
     std::vector<int64_t> slot_query(Q, -1);
     std::vector<uint32_t> slot_base(Q, 0), slot_cap(Q, 0);
@@ -1132,6 +1136,11 @@ int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
                 std::fprintf(stderr, "kpax: slot %u after iteration %llu: next t=%u ts=%u W=%u M=%u fan=%u status=%d\n",
                              s, (unsigned long long)it, x.t, x.ts, x.W, x.M, x.fan, x.status);
             }
+        }
+        if(h->iter_hook)
+        {
+            HIPC(hipStreamSynchronize(h->stream));
+            if((rc = h->iter_hook(it)) != KPAX_OK) return rc;
         }
         // One iteration stays queued behind the one whose statuses are read,
         // so the GPU never waits for the host; a slot that finished is
@@ -1466,6 +1475,141 @@ int kpax_plan_batch(kpax_handle* h, const kpax_query* queries, uint32_t n, kpax_
     const int rc = run_queries(h, qs.data(), n, out);
     h->planned   = false;  // slot 0's tree is no longer one single plan's
     return rc;
+}
+
+// planBench (KPAX.cu:245-300) with writeDeviceVectorsToCSV (KPAX.cu:544-618):
+// the same directory layout and CSV formats (helper.cuh:57-108), written after
+// every iteration.  Rows past the tree are not written (the reference writes
+// its whole zero-filled capacity); the frontier flags and repeat counts are
+// those of the repeat list built for the next iteration.
+int kpax_plan_dump(kpax_handle* h, const float* start, const float* goal, uint64_t seed, const char* dir,
+                   int32_t bench_itr, kpax_result* out)
+{
+    if(!h || !start || !goal || !out || !dir) return fail(h, KPAX_INVALID_ARG, "null argument");
+    const std::string root = std::string(dir) + "/";
+    const std::string b    = std::to_string(bench_itr);
+    const char* sub[] = {"Samples/Samples", "Parents/Parents", "TotalCountPerVertex/TotalCountPerVertex",
+                         "ValidCountPerVertex/ValidCountPerVertex", "Frontier/Frontier",
+                         "FrontierRepeatCount/FrontierRepeatCount", "VertexScores/VertexScores",
+                         "FrontierSize/FrontierSize", "TreeSize/TreeSize", "ExpandedNodes/ExpandedNodes",
+                         "ControlPathToGoal/ControlPathToGoal"};
+    std::error_code ec;
+    for(const char* d : sub) std::filesystem::create_directories(root + d + b, ec);
+    std::filesystem::create_directories(root + "ExecutionTime", ec);
+    if(ec) return fail(h, KPAX_IO_ERROR, "cannot create " + root + ": " + ec.message());
+    const uint32_t cap = std::min<uint32_t>(h->cfg.max_tree, h->T);
+    const int S = h->S, C = h->C, D = S + C + 1;
+    const uint32_t R1 = (uint32_t)h->RP.r1_count;
+    auto vec_rows = [](const std::string& f, const std::vector<std::string>& rows) {
+        std::ofstream o(f, std::ios_base::app);
+        for(const auto& r : rows) o << r << "\n";
+        return (bool)o;
+    };
+    auto join = [](auto begin, auto end, int prec) {
+        std::ostringstream os;
+        os << std::fixed << std::setprecision(prec);
+        for(auto it = begin; it != end; ++it) os << (it == begin ? "" : ",") << *it;
+        return os.str();
+    };
+    auto value = [](const std::string& f, double v, bool is_float) {
+        std::ofstream o(f, std::ios_base::app);
+        if(is_float) o << std::fixed << std::setprecision(10);
+        if(is_float)
+            o << v << "\n";
+        else
+            o << (long long)v << "\n";
+        return (bool)o;
+    };
+    h->iter_hook = [&](uint64_t it) -> int {
+        Ctl c;
+        HIPC(hipMemcpy(&c, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+        if(c.cur.status != kRunning) return KPAX_OK;  // queued after the plan ended
+        const uint32_t itr = c.cur.t;  // h_itr_ of the iteration just run
+        const bool found   = c.goal != INT_MAX;
+        const uint32_t ts  = found ? c.cur.ts + c.Fn : c.nxt.ts;
+        const std::string n = std::to_string(itr);
+        std::vector<float> st((size_t)S * ts), ct((size_t)C * ts);
+        std::vector<int32_t> steps(ts), par(ts), valid(R1), total(R1);
+        std::vector<float> score(R1);
+        for(int d = 0; d < S; ++d)
+            HIPC(hipMemcpy(st.data() + (size_t)d * ts, h->tree.st + (size_t)d * h->T, (size_t)ts * 4,
+                           hipMemcpyDeviceToHost));
+        for(int d = 0; d < C; ++d)
+            HIPC(hipMemcpy(ct.data() + (size_t)d * ts, h->tree.ct + (size_t)d * h->T, (size_t)ts * 4,
+                           hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(steps.data(), h->tree.steps, (size_t)ts * 4, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(par.data(), h->tree.parent, (size_t)ts * 4, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(valid.data(), h->rv.valid, R1 * 4, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(total.data(), h->rv.total, R1 * 4, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(score.data(), h->rv.score, R1 * 4, hipMemcpyDeviceToHost));
+        // repeat counts of the next iteration's list (empty once the goal is found)
+        std::vector<int32_t> repeat(ts, 0);
+        if(!found)
+        {
+            const uint32_t wl = std::min<uint32_t>(c.nxt.W, cap);
+            std::vector<uint32_t> R(wl);
+            if(wl) HIPC(hipMemcpy(R.data(), h->R, (size_t)wl * 4, hipMemcpyDeviceToHost));
+            for(uint32_t k = 0; k < wl; ++k)
+                if(R[k] < ts) repeat[R[k]]++;
+        }
+        std::vector<std::string> rows(ts);
+        std::vector<float> row(D);
+        for(uint32_t i = 0; i < ts; ++i)
+        {
+            for(int d = 0; d < S; ++d) row[d] = st[(size_t)d * ts + i];
+            for(int d = 0; d < C; ++d) row[S + d] = ct[(size_t)d * ts + i];
+            row[S + C] = h->cfg.dt * (float)steps[i];
+            rows[i]    = join(row.begin(), row.end(), 6);
+        }
+        bool ok = vec_rows(root + "Samples/Samples" + b + "/samples" + n + ".csv", rows);
+        for(uint32_t i = 0; i < ts; ++i) rows[i] = std::to_string(par[i]);
+        ok = ok && vec_rows(root + "Parents/Parents" + b + "/parents" + n + ".csv", rows);
+        ok = ok && vec_rows(root + "TotalCountPerVertex/TotalCountPerVertex" + b + "/totalCountPerVertex.csv",
+                            {join(total.begin(), total.end(), 6)});
+        ok = ok && vec_rows(root + "ValidCountPerVertex/ValidCountPerVertex" + b + "/validCountPerVertex.csv",
+                            {join(valid.begin(), valid.end(), 6)});
+        std::vector<int32_t> flag(ts);
+        uint32_t fsize = 0;
+        for(uint32_t i = 0; i < ts; ++i)
+        {
+            flag[i] = repeat[i] > 0;
+            fsize += (uint32_t)flag[i];
+        }
+        ok = ok && vec_rows(root + "Frontier/Frontier" + b + "/frontier.csv", {join(flag.begin(), flag.end(), 6)});
+        ok = ok && vec_rows(root + "FrontierRepeatCount/FrontierRepeatCount" + b + "/frontierRepeatCount.csv",
+                            {join(repeat.begin(), repeat.end(), 6)});
+        ok = ok && vec_rows(root + "VertexScores/VertexScores" + b + "/vertexScores.csv",
+                            {join(score.begin(), score.end(), 6)});
+        ok = ok && value(root + "FrontierSize/FrontierSize" + b + "/frontierSize.csv", fsize, false);
+        ok = ok && value(root + "TreeSize/TreeSize" + b + "/treeSize.csv", ts, false);
+        ok = ok && value(root + "ExpandedNodes/ExpandedNodes" + b + "/expandedNodes.csv", c.cur.M, false);
+        if(!ok) return fail(h, KPAX_IO_ERROR, "cannot write under " + root);
+        (void)it;
+        return KPAX_OK;
+    };
+    QueryIn q{start, goal, seed, cap, 0};
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc  = run_queries(h, &q, 1, out);
+    h->iter_hook  = nullptr;
+    if(rc != KPAX_OK) return rc;
+    if(out->status == KPAX_INVALID_ARG) return fail(h, KPAX_INVALID_ARG, "start or goal is not finite");
+    h->last_tree = out->tree_size;
+    h->planned   = true;
+    if(out->n_rows)
+    {
+        // goal -> root rows, as d_controlPathToGoal_
+        std::vector<std::string> rows(out->n_rows);
+        for(uint32_t k = 0; k < out->n_rows; ++k)
+        {
+            const float* r = out->rows + (size_t)(out->n_rows - 1 - k) * D;
+            rows[k]        = join(r, r + D, 6);
+        }
+        if(!vec_rows(root + "ControlPathToGoal/ControlPathToGoal" + b + "/controlPathToGoal.csv", rows))
+            return fail(h, KPAX_IO_ERROR, "cannot write the path");
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if(!value(root + "ExecutionTime/executionTime.csv", secs, true)) return fail(h, KPAX_IO_ERROR, "cannot write");
+    return KPAX_OK;
 }
 
 void kpax_result_free(kpax_result* r)

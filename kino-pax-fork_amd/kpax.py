@@ -121,6 +121,7 @@ def lib():
         "kpax_create_batch": (i32, [P, u32, u64, P]),
         "kpax_set_environment": (i32, [P, u32, P, u32]),
         "kpax_plan_batch": (i32, [P, P, u32, P]),
+        "kpax_plan_dump": (i32, [P, P, P, u64, ctypes.c_char_p, i32, P]),
         "kpax_mlp_init_params": (i32, [u64, P]),
         "kpax_set_proposal": (i32, [P, P, ctypes.c_float]),
         "kpax_mlp_features": (i32, [P, u32, P, P, u32, P]),
@@ -428,6 +429,20 @@ class KPAX:
         g = _f32(np.asarray(goal, np.float32).ravel()[:3])
         r = _Result()
         self._check(lib().kpax_plan(self._h, _p(x0), _p(g), int(seed) & (2**64 - 1), ctypes.byref(r)), "plan")
+        out = _plan_result(r, self.S, self.C)
+        lib().kpax_result_free(ctypes.byref(r))
+        return out
+
+    def plan_dump(self, initial, goal, directory, seed=0, bench_itr=0):
+        """plan() plus KPAX::planBench's per-iteration CSV dumps under `directory`
+        (the reference's Data/ layout, read by its MATLAB visualisers)."""
+        if self._obs is None:
+            self.set_obstacles(np.zeros((0, 6), np.float32))
+        x0 = _f32(np.asarray(initial, np.float32).ravel()[: self.S])
+        g = _f32(np.asarray(goal, np.float32).ravel()[:3])
+        r = _Result()
+        self._check(lib().kpax_plan_dump(self._h, _p(x0), _p(g), int(seed) & (2**64 - 1), str(directory).encode(),
+                                         int(bench_itr), ctypes.byref(r)), "plan_dump")
         out = _plan_result(r, self.S, self.C)
         lib().kpax_result_free(ctypes.byref(r))
         return out

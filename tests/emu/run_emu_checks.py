@@ -100,6 +100,32 @@ def main():
     check(r.tree_size == o["tree_size"] and np.array_equal(tree["parents"], o["parents"])
           and np.array_equal(tree["states"].view(np.uint32), o["states"].view(np.uint32)),
           f"pruned plan: tree {r.tree_size} vs oracle {o['tree_size']}")
+    dump_checks()
+
+
+def dump_checks():
+    """planBench-compatible dump: one samples/parents file per iteration, the
+    last one equal to the returned tree."""
+    import tempfile
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=4096, max_iter=12)
+    small(cfg)
+    goal = np.array([1.6, 0.5, 1.0], np.float32)
+    with tempfile.TemporaryDirectory() as d, kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        r = p.plan_dump(start, goal, d, seed=77, bench_itr=3)
+        tree = p.tree()
+        last = np.loadtxt(os.path.join(d, "Samples", "Samples3", f"samples{r.iterations}.csv"), delimiter=",", ndmin=2)
+        par = np.loadtxt(os.path.join(d, "Parents", "Parents3", f"parents{r.iterations}.csv"), ndmin=1)
+        ts = np.loadtxt(os.path.join(d, "TreeSize", "TreeSize3", "treeSize.csv"), ndmin=1)
+        n_files = len(os.listdir(os.path.join(d, "Samples", "Samples3")))
+        ok = (last.shape == (r.tree_size, cfg.S + cfg.C + 1) and np.array_equal(par, tree["parents"])
+              and np.allclose(last[:, :3], tree["states"][:3].T, atol=1e-6) and ts[-1] == r.tree_size
+              and n_files == r.iterations and len(ts) == r.iterations)
+        if r.status == kpax.OK:
+            path = np.loadtxt(os.path.join(d, "ControlPathToGoal", "ControlPathToGoal3", "controlPathToGoal.csv"),
+                              delimiter=",", ndmin=2)
+            ok = ok and path.shape[0] == r.rows.shape[0] and np.allclose(path[::-1], r.rows, atol=1e-6)
+    check(ok and r.status == kpax.OK, f"planBench dump: {n_files} iterations, tree {r.tree_size}, status {r.status}")
 
 
 def mlp_checks():

@@ -219,3 +219,20 @@ def test_plan_quadcopter_full_size_finds_verified_path(kpax):
         assert np.array_equal(r.rows[0, :12], start)
         ok, dev = kpax.verify_path(cfg, obs, r.rows, r.steps, list(goal))
         assert ok and dev < 1e-3
+
+
+def test_plan_dump_matches_the_tree(kpax, tmp_path):
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=1 << 16, max_iter=30)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        plain = p.plan(start, goal, seed=8)
+        r = p.plan_dump(start, goal, tmp_path, seed=8, bench_itr=0)
+        tree = p.tree()
+    assert (r.status, r.tree_size, r.iterations, r.expansions) == (plain.status, plain.tree_size, plain.iterations,
+                                                                   plain.expansions)
+    d = tmp_path
+    last = np.loadtxt(d / "Samples" / "Samples0" / f"samples{r.iterations}.csv", delimiter=",", ndmin=2)
+    par = np.loadtxt(d / "Parents" / "Parents0" / f"parents{r.iterations}.csv", ndmin=1)
+    assert last.shape == (r.tree_size, 17) and np.array_equal(par, tree["parents"])
+    assert np.allclose(last[:, :12], tree["states"].T, atol=5e-6 * (1 + np.abs(tree["states"].T)))
+    assert len(np.loadtxt(d / "TreeSize" / "TreeSize0" / "treeSize.csv", ndmin=1)) == r.iterations
