@@ -13,14 +13,16 @@ passes do identical work).
 Also measured, each on the same GPU and stated beside the headline:
   batched  256 queries over 32 environments (16 forests of 100-600 columns,
            16 mazes of 2,000 boxes), budgets log-uniform in [16k, 1M] nodes,
-           in one kpax_plan_batch call, against the same queries planned one
-           at a time; every query's result is identical in both forms, so
-           the success rates match by construction
+           start-goal distance from 1 m up to 2 m (16k) .. 9 m (1M) with the
+           log of the budget, in one kpax_plan_batch call, against the same
+           queries planned one at a time; every query's result is identical
+           in both forms, so the success rates match by construction
   proposal the learned control proposal (50/50 mixture with uniform) against
-           the plain sampler on held-out forests and queries at a 2^16 budget
+           the plain sampler on held-out forests and queries, at 2^18-node
+           budgets (where both solve every query) and at 2^16
 Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-proposal]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-proposal] [--batched-only]
 """
 import argparse
 import json
@@ -49,6 +51,15 @@ PARAMS = os.path.join(ROOT, "kino-pax-fork_amd", "data", "proposal_forest.npy")
 # (4(S+C+1)), parent index (4), admission flag (1), two region counters (8).
 def bytes_per_expansion(S, C):
     return 8 * S + 4 * C + 17
+
+
+# HBM bytes per expansion that k_expand<quadcopter> moves on this workload,
+# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (average per
+# dispatch over the expansions per dispatch; FETCH scaled by 2, WRITE by 1,
+# the factors that make a 1 GiB device copy read and write 1 GiB): 156 B read
+# (parents are read in longest-edge-first order, a cache line per state word)
+# and 95 B written (the 80-byte child and its share of partial lines).
+MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 156 + 95
 
 
 # FP32 operations per integration step as written in the kernel (add, sub,
@@ -83,7 +94,8 @@ def serving_workload(cfg, n_queries=256, n_env=32, seed=0):
             envs.append(kpax.generate_environment(2, 500 + e, int(rng.integers(100, 601)), cfg.ws_min, cfg.ws_max))
         else:
             envs.append(kpax.generate_environment(3, 500 + e, 2000, cfg.ws_min, cfg.ws_max))
-    qs = proposal.serving_queries(cfg, envs, n_queries, seed=7, min_dist=3.0, budget_range=(16384, 1 << 20))
+    qs = proposal.serving_queries(cfg, envs, n_queries, seed=7, min_dist=1.0, budget_range=(16384, 1 << 20),
+                                  max_dist=proposal.budget_distance)
     return envs, qs
 
 
@@ -127,20 +139,31 @@ def learned_proposal(cfg):
     if not os.path.exists(PARAMS):
         return {"skipped": f"{PARAMS} not found"}
     params = np.load(PARAMS)
-    c = cfg.copy()
-    c.max_tree = 1 << 16
-    c.max_iter = 100
-    envs = proposal.forest_environments(c, 8, seed=2)
-    qs = proposal.serving_queries(c, envs, 96, seed=4)
-    with kpax.KPAX(c) as p:
-        for e, o in enumerate(envs):
-            p.set_environment(e, o)
-        plain = proposal.evaluate(p, qs, None)
-        learned = proposal.evaluate(p, qs, params, 0.5)
     strip = lambda d: {k: v for k, v in d.items() if k != "results"}  # noqa: E731
-    return {"workload": "96 held-out queries (start/goal >= 4 m apart) over 8 held-out forests, 2^16-node budget, "
-                        "max_iter 100, single-query form",
-            "plain": strip(plain), "learned_mix_0.5": strip(learned)}
+    out = {"workload": "96 held-out queries (start/goal >= 4 m apart) over 8 held-out forests of 100-600 columns, "
+                       "max_iter 100, single-query form; parameters trained by train_proposal.py on 10,000 "
+                       "plain-planner runs in other forests"}
+    for budget in (1 << 18, 1 << 16):
+        c = cfg.copy()
+        c.max_tree = budget
+        c.max_iter = 100
+        envs = proposal.forest_environments(c, 8, seed=2)
+        qs = proposal.serving_queries(c, envs, 96, seed=4)
+        with kpax.KPAX(c) as p:
+            for e, o in enumerate(envs):
+                p.set_environment(e, o)
+            plain = proposal.evaluate(p, qs, None)
+            learned = proposal.evaluate(p, qs, params, 0.5)
+        both = [(a, b) for a, b in zip(plain["results"], learned["results"]) if a.status == 0 and b.status == 0]
+        out[f"budget_{budget}"] = {
+            "plain": strip(plain), "learned_mix_0.5": strip(learned),
+            "on_queries_both_solve": {
+                "n": len(both),
+                "mean_iterations_plain": float(np.mean([a.iterations for a, _ in both])) if both else None,
+                "mean_iterations_learned": float(np.mean([b.iterations for _, b in both])) if both else None,
+                "mean_expansions_plain": float(np.mean([a.expansions for a, _ in both])) if both else None,
+                "mean_expansions_learned": float(np.mean([b.expansions for _, b in both])) if both else None}}
+    return out
 
 
 def main():
@@ -154,6 +177,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-proposal", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched-serving section (for profiling it alone); prints its JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the byte counters)")
     args = ap.parse_args()
@@ -173,6 +198,9 @@ def main():
         sys.exit(2)
 
     cfg, obs, start, goal = workload(args.workload, seed=args.env_seed)
+    if args.batched_only:
+        print(json.dumps({"batched": batched(cfg)}), flush=True)
+        return
     planner = kpax.KPAX(cfg)
     planner.set_obstacles(obs)
     seeds = [1000 + i for i in range(args.warmup + args.steps)]
@@ -237,7 +265,10 @@ def main():
         "graph_launches": graph_launches,
         "phase_ms_per_plan": {k: v / max(1, st["plans"]) for k, v in st["phase_ms"].items()},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_gbs / HBM_PEAK_GBS,
+                     "traffic": per_launch_exp * MEASURED_EXPAND_TRAFFIC_PER_EXPANSION,
+                     "traffic_basis": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE bytes per expansion x expansions "
+                                      "per launch",
                      "kernel": "k_expand<quadcopter>", "bytes_per_expansion": bytes_per_expansion(S, C),
                      "expansions_per_launch": per_launch_exp, "avg_launch_ms": per_launch_ms},
         "roofline_valu": {"bound": "valu_fp32", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS,

@@ -249,6 +249,11 @@ typedef struct kpax_launch {
     uint32_t expand_chunk;     /* children a wave takes per work ticket (multiple of 64, <= 4096) */
     uint32_t scan_blocks;      /* work-groups of the append and frontier scans                   */
     int32_t  use_graph;        /* 1: replay each iteration as one captured HIP graph             */
+    int32_t  order_children;   /* 1: hand out children longest drawn edge first                  */
+    uint32_t refill_lanes;     /* idle lanes (1..64) before a wave writes out finished children
+                                  and takes new ones; 1 services every lane as soon as it idles */
+    uint32_t scan_tile;        /* items per append/frontier scan tile: 1024 (256-thread blocks)
+                                  or 4096 (1024-thread blocks; fewer look-back steps)           */
 } kpax_launch;
 
 /* Current launch shape of the handle. */
@@ -259,7 +264,8 @@ int kpax_get_launch(kpax_handle* h, kpax_launch* out);
 int kpax_set_launch(kpax_handle* h, const kpax_launch* launch);
 
 /* Time the n sample queries (as kpax_plan_batch) under candidate launch
- * shapes, one knob at a time, keeping the fastest of `repeats` runs of each;
+ * shapes (expansion work-groups, children per ticket, scan work-groups, graph
+ * replay, child ordering), one knob at a time, keeping the fastest of `repeats` runs of each;
  * the best shape is applied to the handle and written to *best.  Every
  * candidate must reproduce the first run's results, or the call fails with
  * KPAX_DEVICE_ERROR. */

@@ -552,6 +552,15 @@ __device__ __forceinline__ bool prune_keep(const PlanParams& PP, const float* ch
     return u01(draw) < p;
 }
 
+// Integration steps of an edge from its Philox word: steps_min + floor(u *
+// span), u in [0, 1).
+__device__ __forceinline__ int draw_steps(const ModelParams& P, uint32_t word)
+{
+    int kk = (int)(u01(word) * (float)P.span);
+    if(kk > P.span - 1) kk = P.span - 1;
+    return P.steps_min + kk;
+}
+
 // Fan-out of the next iteration (KPAX.cu:323-333), capped so the children
 // never exceed the remaining tree capacity.
 __device__ __forceinline__ Iter make_iter(uint32_t t, uint32_t ts, uint32_t W, uint32_t cap, const PlanParams& PP)

@@ -26,14 +26,16 @@ struct TreeView {
     int32_t* reg;
 };
 
-// Children of an iteration: one 96-byte record per child (six 16-byte
-// words), written by one lane with six vector stores when the child
-// finishes.  Words 0-11 state, 12-15 control, 16 integrated steps, 17
-// parent, 18 R1 region, 19 R2 cell, 20 admission flags (kValid |
-// kCandidate | kScoreAdmit).
-constexpr int kChildWords = 24;
+// Children of an iteration, written by the lane that made each one when it
+// finishes: a 64-byte record (words 0-11 state, 12-15 control; four vector
+// stores) and a 16-byte meta word {integrated steps | admission flags << 24,
+// parent, R1 region, R2 cell}.  The append kernel reads every child's meta
+// word and the record only of the children it admits.
+constexpr int kChildWords = 16;
+constexpr uint32_t kMetaFlagShift = 24;  // integrated steps < 2^24 (an edge has at most 100000)
 struct ChildView {
-    uint4* rec;  // kChildWords / 4 uint4 per child
+    uint4* rec;   // kChildWords / 4 uint4 per child
+    uint4* meta;  // one per child
 };
 
 // Region state of slot q: counters and scores at q * R1, sub-region cells at
@@ -59,7 +61,7 @@ struct Env {
     std::vector<float> host;
 };
 
-constexpr int kTile       = 1024;  // items per scan tile (256 threads x 4)
+constexpr int kTile       = 1024;  // items per scan tile (256 threads x 4); the planner's smallest tile
 constexpr int kLdsRegions = 4096;  // R1 counts up to this size are summed in LDS
 constexpr int kMaxEnvs    = 64;    // environments of one handle (LDS table in batched expansion)
 
@@ -105,6 +107,11 @@ struct kpax_handle {
     void* expand_args       = nullptr;
     uint32_t expand_blocks  = 1;
     uint32_t expand_chunk   = 64;
+    bool order_children     = true;     // hand out children longest drawn edge first
+    uint32_t refill_lanes   = 24;       // idle lanes that trigger a wave's service pass
+    uint32_t scan_tile      = 4096;     // items per append/frontier tile: 1024 (256 threads) or 4096 (1024)
+    uint32_t* order         = nullptr;  // T positions -> flattened child
+    uint32_t* order_hist    = nullptr;  // buckets x order blocks, scanned in place
     uint32_t scan_blocks    = 1;
     uint32_t path_cap       = 0;        // rows per slot
     float* path_rows        = nullptr;  // Qmax x path_cap x (S + C + 1), goal -> root

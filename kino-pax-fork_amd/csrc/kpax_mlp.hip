@@ -52,29 +52,27 @@ __device__ __forceinline__ void wave_lds_sync()
 __global__ __launch_bounds__(256) void k_occupancy(FeatParams F, const float4* lo, const float4* hi, uint32_t n,
                                                    uint32_t* bits)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // one 32-cell word per thread
-    if(i >= kOccDim * kOccDim * kOccDim / 32) return;
-    uint32_t w = 0;
-    for(int j = 0; j < 32; ++j)
+    // one cell per lane; a wave's 64 verdicts are two bitmap words
+    constexpr int kCells = kOccDim * kOccDim * kOccDim;
+    const int cidx       = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit             = false;
+    if(cidx < kCells)
     {
-        const int cidx = i * 32 + j;
         const int a = cidx / (kOccDim * kOccDim), b = (cidx / kOccDim) % kOccDim, c = cidx % kOccDim;
         const float cl[3] = {F.occ_mn[0] + (float)a * F.occ_cell[0], F.occ_mn[1] + (float)b * F.occ_cell[1],
                              F.occ_mn[2] + (float)c * F.occ_cell[2]};
         const float ch[3] = {F.occ_mn[0] + (float)(a + 1) * F.occ_cell[0],
                              F.occ_mn[1] + (float)(b + 1) * F.occ_cell[1],
                              F.occ_mn[2] + (float)(c + 1) * F.occ_cell[2]};
-        for(uint32_t o = 0; o < n; ++o)
+        for(uint32_t o = 0; o < n && !hit; ++o)
         {
             const float4 p = lo[o], q = hi[o];
-            if(p.x < ch[0] && q.x > cl[0] && p.y < ch[1] && q.y > cl[1] && p.z < ch[2] && q.z > cl[2])
-            {
-                w |= 1u << j;
-                break;
-            }
+            hit = p.x < ch[0] && q.x > cl[0] && p.y < ch[1] && q.y > cl[1] && p.z < ch[2] && q.z > cl[2];
         }
     }
-    bits[i] = w;
+    const uint64_t m = __ballot(hit);
+    const int lane   = threadIdx.x & 63;
+    if(cidx < kCells && (lane == 0 || lane == 32)) bits[cidx / 32] = (uint32_t)(lane == 0 ? m : m >> 32);
 }
 
 // fp32 parameters (flat layout of kpax_mlp.cuh) -> bf16 device layouts.
@@ -665,7 +663,8 @@ int build_occupancy(kpax_handle* h, uint32_t env)
     Env& e = h->envs[env];
     if(!e.occ) HIPC(hipMalloc(&e.occ, kOccDim * kOccDim * kOccDim / 8));
     const FeatParams F = make_feat_params(h->cfg);
-    hipLaunchKernelGGL(k_occupancy, dim3(1), dim3(128), 0, h->stream, F, e.lo, e.hi, e.n_obs, e.occ);
+    hipLaunchKernelGGL(k_occupancy, dim3(kOccDim * kOccDim * kOccDim / 256), dim3(256), 0, h->stream, F, e.lo, e.hi,
+                       e.n_obs, e.occ);
     HIPC(hipGetLastError());
     return KPAX_OK;
 }

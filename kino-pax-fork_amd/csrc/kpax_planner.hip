@@ -139,6 +139,8 @@ struct ExpandArgs {
     GridParams GP;
     uint32_t T, Q, R2;
     uint32_t chunk;  // children per work ticket (multiple of 64)
+    const uint32_t* order;  // processing order of the children (null: index order)
+    uint32_t refill;        // idle lanes that trigger a wave's service pass (1..64)
     float mix;       // probability of the learned proposal (0: uniform controls only)
     const float* prop;
     SlotUniform s0;
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
                                                   uint32_t* atile_off, uint32_t* ftile_off, int32_t* slot_status,
                                                   uint32_t Q, uint64_t* st_append, uint64_t* st_frontier,
                                                   uint32_t n_status, PlanParams PP, const GridView* envs,
-                                                  SlotUniform* s0, uint32_t* ptile_off)
+                                                  SlotUniform* s0, uint32_t* ptile_off, uint32_t tile)
 {
     __shared__ uint32_t lds[16];
     __shared__ uint32_t s_epoch;
@@ -177,8 +179,8 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
             c->Fn       = 0;
             c->f_accept = f_accept(n.t, n.ts, qd[q].cap, PP.eps);  // kept when no child is admitted
             m           = n.M;
-            at          = (n.M + kTile - 1) / kTile;
-            ft          = (uint32_t)(((uint64_t)n.ts + n.M + kTile - 1) / kTile);
+            at          = (n.M + tile - 1) / tile;
+            ft          = (uint32_t)(((uint64_t)n.ts + n.M + tile - 1) / tile);
             pt          = (n.M + 31) / 32;
             run         = 1;
         }
@@ -284,98 +286,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
     int i = 0, n = 0;
     float x[S], u[C];
     uint32_t q_next = 0, q_end = 0;  // the wave's current chunk [q_next, q_end)
-    bool drained        = false;
+    bool drained = false;
+    bool live = false, pend = false, valid = false;  // integrating / finished, not yet written / its verdict
+    const uint32_t refill = A->refill;
     for(;;)
     {
-        if(!drained)
+        // Service pass, once `refill` lanes are idle (or all are): write out the
+        // finished children, then hand the idle lanes new ones.  Deferring it
+        // lets the wave run the write-out and the draws for several lanes at
+        // once instead of on nearly every trip.
+        const uint64_t idle = __ballot(!live);
+        if(idle == ~0ull || (uint32_t)__popcll(idle) >= refill)
         {
-            uint64_t need = __ballot(c == kNone);
-            while(need)
-            {
-                if(q_next >= q_end)
-                {
-                    const uint32_t chunk = A->chunk;
-                    uint32_t b           = 0;
-                    if(lane == 0) b = atomicAdd(&A->bc->tile_expand, chunk);
-                    b = __shfl(b, 0, 64);
-                    if(b >= total)
-                    {
-                        drained = true;
-                        break;
-                    }
-                    q_next = b;
-                    q_end  = total - b > chunk ? b + chunk : total;
-                }
-                const uint32_t avail = q_end - q_next;
-                const uint32_t rank  = (uint32_t)__popcll(need & below);
-                if(c == kNone && rank < avail)
-                {
-                    const uint32_t k = q_next + rank;
-                    uint32_t ci      = k;
-                    if constexpr(MULTI)
-                    {
-                        const uint32_t s = segment_of(A->child_off, A->Q, k);
-                        ci               = k - A->child_off[s];
-                        t                = A->ctl[s].cur.t;
-                        fan              = A->ctl[s].cur.fan;
-                        const QueryDev& d = A->qd[s];
-                        base             = d.base;
-                        k0               = (uint32_t)d.seed;
-                        k1               = (uint32_t)(d.seed >> 32);
-                        qword            = d.query;
-                        const GridView& g = A->envs[d.env];
-                        G.start          = g.start;
-                        G.items          = g.items;
-                        G.lo             = g.lo;
-                        G.hi             = g.hi;
-                        slot             = s;
-                    }
-                    c      = ci;
-                    parent            = R[base + c / fan];
-                    const uint32_t pn = base + parent;
-#pragma unroll
-                    for(int d = 0; d < S; ++d) x[d] = tree.st[(size_t)d * T + pn];
-                    const uint4 dc        = philox(c, t, qword, PURPOSE_CONTROL, k0, k1);
-                    const uint4 ds        = philox(c, t, qword, PURPOSE_STEPS_ADMIT, k0, k1);
-                    const uint32_t dcw[4] = {dc.x, dc.y, dc.z, dc.w};
-#pragma unroll
-                    for(int j = 0; j < C; ++j) u[j] = P.ctrl_min[j] + u01(dcw[j]) * P.ctrl_rng[j];
-                    if(A->mix > 0.0f && u01(ds.z) < A->mix)
-                    {
-                        // the parent's Gaussian instead (learned proposal)
-                        const uint4 dp = philox(c, t, qword, PURPOSE_PROPOSAL, k0, k1);
-                        float mean[C], sg[C];
-#pragma unroll
-                        for(int j = 0; j < C; ++j)
-                        {
-                            mean[j] = A->prop[(size_t)j * T + pn];
-                            sg[j]   = A->prop[(size_t)(4 + j) * T + pn];
-                        }
-                        propose_control<C>(mean, sg, dp, P, u);
-                    }
-                    int kk = (int)(u01(ds.x) * (float)P.span);
-                    if(kk > P.span - 1) kk = P.span - 1;
-                    n          = P.steps_min + kk;
-                    i          = 0;
-                    admit_draw = ds.y;
-                }
-                const uint32_t wanted = (uint32_t)__popcll(need);
-                q_next += wanted < avail ? wanted : avail;
-                need = __ballot(c == kNone);
-            }
-        }
-        if(__ballot(c != kNone) == 0) break;
-        if(c != kNone)
-        {
-            const bool ok = step<MODEL>(P, G, x, u);
-            ++i;
-            if(!ok || i == n)
+            if(pend)
             {
                 int r1, r2;
                 locate(RP, x, &r1, &r2);
                 const size_t ro = MULTI ? (size_t)slot * R1 : 0;
                 uint32_t flags  = 0;
-                if(ok)
+                if(valid)
                 {
                     flags = kValid | claim_cell(rv.r2_state + (MULTI ? (size_t)slot * A->R2 : 0), r2, t, c);
                     if(u01(admit_draw) < rv.score[ro + r1]) flags |= kScoreAdmit;
@@ -386,20 +315,206 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
                 for(int d = 0; d < 12; ++d) w[d] = d < S ? __builtin_bit_cast(uint32_t, x[d < S ? d : 0]) : 0u;
 #pragma unroll
                 for(int d = 0; d < 4; ++d) w[12 + d] = d < C ? __builtin_bit_cast(uint32_t, u[d < C ? d : 0]) : 0u;
-                w[16] = (uint32_t)i;  // steps integrated: n for a valid child
-                w[17] = parent;
-                w[18] = (uint32_t)r1;
-                w[19] = (uint32_t)r2;
-                w[20] = flags;
-                w[21] = w[22] = w[23] = 0u;
                 uint4* rec = ch.rec + (size_t)o * (kChildWords / 4);
 #pragma unroll
                 for(int v = 0; v < kChildWords / 4; ++v)
                     rec[v] = make_uint4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
-                c = kNone;
+                // steps integrated: n for a valid child
+                ch.meta[o] = make_uint4((uint32_t)i | (flags << kMetaFlagShift), parent, (uint32_t)r1, (uint32_t)r2);
+                c    = kNone;
+                pend = false;
+            }
+            if(!drained)
+            {
+                uint64_t need = __ballot(c == kNone);
+                while(need)
+                {
+                    if(q_next >= q_end)
+                    {
+                        const uint32_t chunk = A->chunk;
+                        uint32_t b           = 0;
+                        if(lane == 0) b = atomicAdd(&A->bc->tile_expand, chunk);
+                        b = __shfl(b, 0, 64);
+                        if(b >= total)
+                        {
+                            drained = true;
+                            break;
+                        }
+                        q_next = b;
+                        q_end  = total - b > chunk ? b + chunk : total;
+                    }
+                    const uint32_t avail = q_end - q_next;
+                    const uint32_t rank  = (uint32_t)__popcll(need & below);
+                    if(c == kNone && rank < avail)
+                    {
+                        const uint32_t k = A->order ? A->order[q_next + rank] : q_next + rank;
+                        uint32_t ci      = k;
+                        if constexpr(MULTI)
+                        {
+                            const uint32_t s = segment_of(A->child_off, A->Q, k);
+                            ci               = k - A->child_off[s];
+                            t                = A->ctl[s].cur.t;
+                            fan              = A->ctl[s].cur.fan;
+                            const QueryDev& d = A->qd[s];
+                            base             = d.base;
+                            k0               = (uint32_t)d.seed;
+                            k1               = (uint32_t)(d.seed >> 32);
+                            qword            = d.query;
+                            const GridView& g = A->envs[d.env];
+                            G.start          = g.start;
+                            G.items          = g.items;
+                            G.lo             = g.lo;
+                            G.hi             = g.hi;
+                            slot             = s;
+                        }
+                        c                 = ci;
+                        parent            = R[base + c / fan];
+                        const uint32_t pn = base + parent;
+#pragma unroll
+                        for(int d = 0; d < S; ++d) x[d] = tree.st[(size_t)d * T + pn];
+                        const uint4 dc        = philox(c, t, qword, PURPOSE_CONTROL, k0, k1);
+                        const uint4 ds        = philox(c, t, qword, PURPOSE_STEPS_ADMIT, k0, k1);
+                        const uint32_t dcw[4] = {dc.x, dc.y, dc.z, dc.w};
+#pragma unroll
+                        for(int j = 0; j < C; ++j) u[j] = P.ctrl_min[j] + u01(dcw[j]) * P.ctrl_rng[j];
+                        if(A->mix > 0.0f && u01(ds.z) < A->mix)
+                        {
+                            // the parent's Gaussian instead (learned proposal)
+                            const uint4 dp = philox(c, t, qword, PURPOSE_PROPOSAL, k0, k1);
+                            float mean[C], sg[C];
+#pragma unroll
+                            for(int j = 0; j < C; ++j)
+                            {
+                                mean[j] = A->prop[(size_t)j * T + pn];
+                                sg[j]   = A->prop[(size_t)(4 + j) * T + pn];
+                            }
+                            propose_control<C>(mean, sg, dp, P, u);
+                        }
+                        n          = draw_steps(P, ds.x);
+                        i          = 0;
+                        admit_draw = ds.y;
+                        live       = true;
+                    }
+                    const uint32_t wanted = (uint32_t)__popcll(need);
+                    q_next += wanted < avail ? wanted : avail;
+                    need = __ballot(c == kNone);
+                }
+            }
+            if(__ballot(live) == 0) break;  // drained, and every child written out
+        }
+        if(live)
+        {
+            valid = step<MODEL>(P, G, x, u);
+            ++i;
+            if(!valid || i == n)
+            {
+                live = false;
+                pend = true;
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Processing order of the children: a counting sort by drawn step count,
+// longest first, so the last children a launch starts are short ones and the
+// launch does not wait on a long edge begun at its end.  Results do not depend
+// on the order (every child is a function of its index and of the tree
+// before the iteration).  Three launches: bucket counts per block, one
+// exclusive scan over (bucket, block), scatter.
+// ---------------------------------------------------------------------------
+constexpr int kOrderBlocks = 256, kOrderBuckets = 32;
+static_assert(kOrderBlocks * kOrderBuckets <= 8192, "k_order_scan covers 1024 x 8 counts");
+
+__device__ __forceinline__ uint32_t order_bucket(const ModelParams& P, int n)
+{
+    const uint32_t back = (uint32_t)(P.steps_min + P.span - 1 - n);  // 0 for the longest edge
+    return P.span <= kOrderBuckets ? back : (uint32_t)((uint64_t)back * kOrderBuckets / (uint32_t)P.span);
+}
+
+template<bool MULTI>
+__device__ __forceinline__ uint32_t order_total(const ExpandArgs* A)
+{
+    if constexpr(MULTI)
+        return A->bc->children;
+    else
+        return A->s0.cur.status == kRunning ? A->s0.cur.M : 0u;
+}
+
+template<bool MULTI>
+__device__ __forceinline__ uint32_t child_bucket(const ExpandArgs* A, uint32_t k)
+{
+    uint32_t c = k, t;
+    uint64_t seed;
+    uint32_t query;
+    if constexpr(MULTI)
+    {
+        const uint32_t s = segment_of(A->child_off, A->Q, k);
+        c                = k - A->child_off[s];
+        t                = A->ctl[s].cur.t;
+        seed             = A->qd[s].seed;
+        query            = A->qd[s].query;
+    }
+    else
+    {
+        t     = A->s0.cur.t;
+        seed  = A->s0.qd.seed;
+        query = A->s0.qd.query;
+    }
+    const uint4 ds = philox(c, t, query, PURPOSE_STEPS_ADMIT, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return order_bucket(A->P, draw_steps(A->P, ds.x));
+}
+
+template<bool MULTI>
+__global__ __launch_bounds__(256) void k_order_count(const ExpandArgs* __restrict__ A, uint32_t* hist)
+{
+    __shared__ uint32_t h[kOrderBuckets];
+    if(threadIdx.x < kOrderBuckets) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t total = order_total<MULTI>(A);
+    const uint32_t per   = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t b0 = blockIdx.x * per, b1 = b0 + per < total ? b0 + per : total;
+    for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x) atomicAdd(&h[child_bucket<MULTI>(A, k)], 1u);
+    __syncthreads();
+    if(threadIdx.x < kOrderBuckets) hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of the (bucket-major) counts, in place; one block of 1024,
+// eight consecutive counts per thread (n <= 8192).
+__global__ __launch_bounds__(1024) void k_order_scan(uint32_t* hist, uint32_t n)
+{
+    __shared__ uint32_t lds[16];
+    const uint32_t i0 = threadIdx.x * 8;
+    uint32_t v[8], sum = 0;
+#pragma unroll
+    for(int j = 0; j < 8; ++j)
+    {
+        v[j] = i0 + j < n ? hist[i0 + j] : 0u;
+        sum += v[j];
+    }
+    uint32_t agg;
+    uint32_t run = block_excl_scan<16>(sum, &agg, lds);
+#pragma unroll
+    for(int j = 0; j < 8; ++j)
+        if(i0 + j < n)
+        {
+            hist[i0 + j] = run;
+            run += v[j];
+        }
+}
+
+template<bool MULTI>
+__global__ __launch_bounds__(256) void k_order_scatter(const ExpandArgs* __restrict__ A, const uint32_t* hist,
+                                                       uint32_t* order)
+{
+    __shared__ uint32_t pos[kOrderBuckets];
+    if(threadIdx.x < kOrderBuckets) pos[threadIdx.x] = hist[threadIdx.x * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const uint32_t total = order_total<MULTI>(A);
+    const uint32_t per   = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t b0 = blockIdx.x * per, b1 = b0 + per < total ? b0 + per : total;
+    for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x)
+        order[atomicAdd(&pos[child_bucket<MULTI>(A, k)], 1u)] = k;
 }
 
 // ---------------------------------------------------------------------------
@@ -460,12 +575,14 @@ __global__ __launch_bounds__(1024) void k_score(Ctl* ctl, RegionView rv, RegionP
 // regions it touched reach HBM: siblings land in the same few regions, and
 // per-child device-scope atomics would serialise on them.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, BatchCtl* bc, const uint32_t* atile_off,
-                                                uint32_t Q, ChildView ch, TreeView tree, RegionView rv,
-                                                uint64_t* status, PlanParams PP, int S, int C, uint32_t T,
-                                                uint32_t R1, uint32_t R2)
+template<int NT>
+__global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, BatchCtl* bc, const uint32_t* atile_off,
+                                               uint32_t Q, ChildView ch, TreeView tree, RegionView rv,
+                                               uint64_t* status, PlanParams PP, int S, int C, uint32_t T,
+                                               uint32_t R1, uint32_t R2)
 {
-    __shared__ uint32_t lds[4];
+    constexpr uint32_t kT = 4 * NT;  // children per tile
+    __shared__ uint32_t lds[NT / 64];
     __shared__ uint32_t s_tile, s_pre;
     HIP_DYNAMIC_SHARED(int32_t, s_hist)
     const uint32_t ntiles = bc->atiles, epoch = bc->epoch;
@@ -489,23 +606,22 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
         int32_t* cov          = rv.covered + (size_t)q * R1;
         int32_t* tot          = rv.total + (size_t)q * R1;
         int32_t* val          = rv.valid + (size_t)q * R1;
-        const uint32_t cbase  = lt * kTile + threadIdx.x * 4;
+        const uint32_t cbase  = lt * kT + threadIdx.x * 4;
         uint32_t f[4], cnt = 0, steps = 0;
 #pragma unroll
         for(int j = 0; j < 4; ++j)
         {
             const uint32_t c = cbase + j;
-            uint4 meta       = make_uint4(0u, 0u, 0u, 0u);  // steps, parent, region, R2 cell
+            uint4 meta       = make_uint4(0u, 0u, 0u, 0u);  // steps | flags, parent, region, R2 cell
             uint32_t a       = 0u;
             if(c < M)
             {
-                const uint4* rec = ch.rec + (base + c) * (kChildWords / 4);
-                meta             = rec[4];
-                a                = rec[5].x;
+                meta = ch.meta[base + c];
+                a    = meta.x >> kMetaFlagShift;
             }
             if(c < M)
             {
-                steps += meta.x;
+                steps += meta.x & ((1u << kMetaFlagShift) - 1u);
                 const int r = (int)meta.z;
                 if(local)
                 {
@@ -542,10 +658,10 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
             cnt += f[j];
         }
         uint32_t agg, steps_tile;
-        (void)block_excl_scan_256(steps, &steps_tile, lds);
+        (void)block_excl_scan<NT / 64>(steps, &steps_tile, lds);
         if(threadIdx.x == 0 && steps_tile)
             atomicAdd((unsigned long long*)&ctl[q].steps_done, (unsigned long long)steps_tile);
-        const uint32_t loc = block_excl_scan_256(cnt, &agg, lds);
+        const uint32_t loc = block_excl_scan<NT / 64>(cnt, &agg, lds);
         const uint32_t pre = tile_prefix(status, tile, agg, first, epoch, &s_pre);
         uint32_t dst       = ts + pre + loc;
 #pragma unroll
@@ -560,9 +676,10 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
             }
             const size_t o   = base + dst;
             const uint4* rec = ch.rec + c * (kChildWords / 4);
-            uint32_t w[20];
+            const uint4 m4   = ch.meta[c];
+            uint32_t w[kChildWords];
 #pragma unroll
-            for(int v = 0; v < 5; ++v)
+            for(int v = 0; v < kChildWords / 4; ++v)
             {
                 const uint4 q4 = rec[v];
                 w[4 * v]       = q4.x;
@@ -572,9 +689,9 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
             }
             for(int dd = 0; dd < S; ++dd) tree.st[(size_t)dd * T + o] = __builtin_bit_cast(float, w[dd]);
             for(int dd = 0; dd < C; ++dd) tree.ct[(size_t)dd * T + o] = __builtin_bit_cast(float, w[12 + dd]);
-            tree.steps[o]  = (int32_t)w[16];
-            tree.parent[o] = (int32_t)w[17];
-            tree.reg[o]    = (int32_t)w[18];
+            tree.steps[o]  = (int32_t)(m4.x & ((1u << kMetaFlagShift) - 1u));
+            tree.parent[o] = (int32_t)m4.y;
+            tree.reg[o]    = (int32_t)m4.z;
             const float dx = __builtin_bit_cast(float, w[0]) - d.goal[0],
                         dy = __builtin_bit_cast(float, w[1]) - d.goal[1],
                         dz = __builtin_bit_cast(float, w[2]) - d.goal[2];
@@ -607,12 +724,14 @@ __global__ __launch_bounds__(256) void k_append(Ctl* ctl, const QueryDev* qd, Ba
 // run-length expanded into the repeat list (Planner.cu:52-80).  A slot whose
 // goal was reached ends here instead.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_frontier(Ctl* ctl, const QueryDev* qd, BatchCtl* bc,
-                                                  const uint32_t* ftile_off, int32_t* slot_status, uint32_t Q,
-                                                  TreeView tree, uint32_t* R, RegionView rv, uint64_t* status,
-                                                  PlanParams PP, uint32_t R1)
+template<int NT>
+__global__ __launch_bounds__(NT) void k_frontier(Ctl* ctl, const QueryDev* qd, BatchCtl* bc,
+                                                 const uint32_t* ftile_off, int32_t* slot_status, uint32_t Q,
+                                                 TreeView tree, uint32_t* R, RegionView rv, uint64_t* status,
+                                                 PlanParams PP, uint32_t R1)
 {
-    __shared__ uint32_t lds[4];
+    constexpr uint32_t kT = 4 * NT;  // nodes per tile
+    __shared__ uint32_t lds[NT / 64];
     __shared__ uint32_t s_tile, s_pre;
     const uint32_t ntiles = bc->ftiles, epoch = bc->epoch;
     for(;;)
@@ -637,7 +756,7 @@ __global__ __launch_bounds__(256) void k_frontier(Ctl* ctl, const QueryDev* qd, 
             }
             continue;
         }
-        const uint32_t tsn = it.ts + Fn, nt = (tsn + kTile - 1) / kTile;
+        const uint32_t tsn = it.ts + Fn, nt = (tsn + kT - 1) / kT;
         if(lt >= nt) continue;  // the slot's tile budget assumed every child admitted
         const QueryDev d       = qd[q];
         const float fa         = ctl[q].f_accept;
@@ -645,7 +764,7 @@ __global__ __launch_bounds__(256) void k_frontier(Ctl* ctl, const QueryDev* qd, 
         const uint32_t k0      = (uint32_t)d.seed, k1 = (uint32_t)(d.seed >> 32);
         const float* score     = rv.score + (size_t)q * R1;
         const int32_t* valid   = rv.valid + (size_t)q * R1;
-        const uint32_t nbase   = lt * kTile + threadIdx.x * 4;
+        const uint32_t nbase   = lt * kT + threadIdx.x * 4;
         uint32_t w[4], cnt = 0;
 #pragma unroll
         for(int j = 0; j < 4; ++j)
@@ -666,7 +785,7 @@ __global__ __launch_bounds__(256) void k_frontier(Ctl* ctl, const QueryDev* qd, 
             cnt += w[j];
         }
         uint32_t agg;
-        const uint32_t loc = block_excl_scan_256(cnt, &agg, lds);
+        const uint32_t loc = block_excl_scan<NT / 64>(cnt, &agg, lds);
         const uint32_t pre = tile_prefix(status, tile, agg, first, epoch, &s_pre);
         uint32_t pos       = pre + loc;
 #pragma unroll
@@ -786,6 +905,8 @@ int upload_expand_args(kpax_handle* h)
     a.Q         = h->Qmax;
     a.R2        = h->R2;
     a.chunk     = h->expand_chunk;
+    a.order     = h->order_children ? h->order : nullptr;
+    a.refill    = h->refill_lanes;
     a.mix       = h->mix;
     a.prop      = h->prop;
     memset(&a.s0, 0, sizeof(a.s0));
@@ -804,8 +925,30 @@ void launch_expand_model(kpax_handle* h)
         hipLaunchKernelGGL((k_expand<MODEL, true>), dim3(h->expand_blocks), dim3(256), 0, h->stream, A);
 }
 
+void launch_order(kpax_handle* h)
+{
+    const ExpandArgs* A = (const ExpandArgs*)h->expand_args;
+    if(h->Qmax == 1)
+    {
+        hipLaunchKernelGGL((k_order_count<false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
+        hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, h->stream, h->order_hist,
+                           (uint32_t)(kOrderBlocks * kOrderBuckets));
+        hipLaunchKernelGGL((k_order_scatter<false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist,
+                           h->order);
+    }
+    else
+    {
+        hipLaunchKernelGGL((k_order_count<true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
+        hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, h->stream, h->order_hist,
+                           (uint32_t)(kOrderBlocks * kOrderBuckets));
+        hipLaunchKernelGGL((k_order_scatter<true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist,
+                           h->order);
+    }
+}
+
 void launch_expand(kpax_handle* h)
 {
+    if(h->order_children) launch_order(h);
     switch(h->cfg.model)
     {
         case KPAX_MODEL_DOUBLE_INTEGRATOR: launch_expand_model<KPAX_MODEL_DOUBLE_INTEGRATOR>(h); break;
@@ -822,13 +965,18 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
     const uint32_t Q = h->Qmax, R1 = (uint32_t)h->RP.r1_count;
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc, h->child_off, h->atile_off,
                        h->ftile_off, h->slot_status, Q, h->st_append, h->st_frontier, h->n_status, h->PP, h->envs_dev,
-                       &((ExpandArgs*)h->expand_args)->s0, h->ptile_off);
+                       &((ExpandArgs*)h->expand_args)->s0, h->ptile_off, h->scan_tile);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
     launch_expand(h);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][1], h->stream));
-    hipLaunchKernelGGL(k_append, dim3(h->scan_blocks), dim3(256), append_lds_bytes(h), h->stream, h->ctl, h->qd, h->bc,
-                       h->atile_off, Q,
-                       h->ch, h->tree, h->rv, h->st_append, h->PP, h->S, h->C, h->T, R1, h->R2);
+    if(h->scan_tile == 4 * 1024)
+        hipLaunchKernelGGL(k_append<1024>, dim3(h->scan_blocks), dim3(1024), append_lds_bytes(h), h->stream, h->ctl,
+                           h->qd, h->bc, h->atile_off, Q, h->ch, h->tree, h->rv, h->st_append, h->PP, h->S, h->C, h->T,
+                           R1, h->R2);
+    else
+        hipLaunchKernelGGL(k_append<256>, dim3(h->scan_blocks), dim3(256), append_lds_bytes(h), h->stream, h->ctl,
+                           h->qd, h->bc, h->atile_off, Q, h->ch, h->tree, h->rv, h->st_append, h->PP, h->S, h->C, h->T,
+                           R1, h->R2);
     if(h->mix > 0.0f)
     {
         const int rc = launch_propose(h, 0);
@@ -837,8 +985,12 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][2], h->stream));
     hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][3], h->stream));
-    hipLaunchKernelGGL(k_frontier, dim3(h->scan_blocks), dim3(256), 0, h->stream, h->ctl, h->qd, h->bc, h->ftile_off,
-                       h->slot_status, Q, h->tree, h->R, h->rv, h->st_frontier, h->PP, R1);
+    if(h->scan_tile == 4 * 1024)
+        hipLaunchKernelGGL(k_frontier<1024>, dim3(h->scan_blocks), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc,
+                           h->ftile_off, h->slot_status, Q, h->tree, h->R, h->rv, h->st_frontier, h->PP, R1);
+    else
+        hipLaunchKernelGGL(k_frontier<256>, dim3(h->scan_blocks), dim3(256), 0, h->stream, h->ctl, h->qd, h->bc,
+                           h->ftile_off, h->slot_status, Q, h->tree, h->R, h->rv, h->st_frontier, h->PP, R1);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(h->h_status + (size_t)p * (Q + 1), h->slot_status, (Q + 1) * sizeof(int32_t),
@@ -1230,13 +1382,17 @@ namespace {
 int apply_launch(kpax_handle* h, const kpax_launch& l)
 {
     if(l.expand_blocks < 1 || l.expand_blocks > (1u << 20) || l.expand_chunk < 64 || l.expand_chunk > 4096 ||
-       l.expand_chunk % 64 != 0 || l.scan_blocks < 1 || l.scan_blocks > (1u << 20))
+       l.expand_chunk % 64 != 0 || l.scan_blocks < 1 || l.scan_blocks > (1u << 20) || l.refill_lanes < 1 ||
+       l.refill_lanes > 64 || (l.scan_tile != 1024 && l.scan_tile != 4096))
         return fail(h, KPAX_INVALID_ARG, "launch shape out of range");
     HIPC(hipStreamSynchronize(h->stream));
     h->expand_blocks = l.expand_blocks;
     h->expand_chunk  = l.expand_chunk;
     h->scan_blocks   = l.scan_blocks;
     h->use_graph     = l.use_graph != 0;
+    h->order_children = l.order_children != 0;
+    h->refill_lanes   = l.refill_lanes;
+    h->scan_tile      = l.scan_tile;
     drop_graphs(h);
     const int rc = upload_expand_args(h);
     if(rc == KPAX_OK && !l.use_graph) h->graph_tried = true;  // stay on direct launches
@@ -1250,6 +1406,9 @@ kpax_launch current_launch(const kpax_handle* h)
     l.expand_chunk  = h->expand_chunk;
     l.scan_blocks   = h->scan_blocks;
     l.use_graph     = h->use_graph ? 1 : 0;
+    l.order_children = h->order_children ? 1 : 0;
+    l.refill_lanes   = h->refill_lanes;
+    l.scan_tile      = h->scan_tile;
     return l;
 }
 
@@ -1317,13 +1476,15 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     h->expand_chunk  = 64;
     const uint32_t Q = h->Qmax;
     const size_t T   = h->T;
-    h->scan_blocks   = std::min<uint32_t>(blocks_for(T, kTile) + Q, 4 * h->cu_count);
+    // tiles of 4096 items run as 1024-thread blocks, two resident per CU
+    h->scan_blocks   = std::min<uint32_t>(blocks_for(T, h->scan_tile) + Q, 2 * h->cu_count);
     TRY(dalloc(h, &h->tree.st, T * h->S));
     TRY(dalloc(h, &h->tree.ct, T * h->C));
     TRY(dalloc(h, &h->tree.steps, T));
     TRY(dalloc(h, &h->tree.parent, T));
     TRY(dalloc(h, &h->tree.reg, T));
     TRY(dalloc(h, &h->ch.rec, T * (kChildWords / 4)));
+    TRY(dalloc(h, &h->ch.meta, T));
     TRY(dalloc(h, &h->R, T));
     const size_t r1 = (size_t)h->RP.r1_count * Q, r2 = (size_t)h->R2 * Q;
     TRY(dalloc(h, &h->rv.valid, r1));
@@ -1338,6 +1499,12 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->atile_off, Q + 1));
     TRY(dalloc(h, &h->ftile_off, Q + 1));
     TRY(dalloc(h, &h->ptile_off, Q + 1));
+    TRY(dalloc(h, &h->order, T));
+    TRY(dalloc(h, &h->order_hist, (size_t)kOrderBlocks * kOrderBuckets));
+    {
+        const char* env       = std::getenv("KPAX_ORDER");
+        h->order_children = !(env && env[0] == '0');
+    }
     TRY(dalloc(h, &h->slot_status, Q + 1));
     h->n_status = (uint32_t)(T / kTile + Q + 2);
     TRY(dalloc(h, &h->st_append, h->n_status));
@@ -1730,19 +1897,25 @@ int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_
     cur.expand_blocks = occ * cu;
     double cur_ms     = 0.0;
     int rc            = run(cur, &cur_ms, ref);
-    std::vector<std::vector<kpax_launch>> knobs(4);
-    for(uint32_t k = 1; k <= occ; ++k) knobs[0].push_back(kpax_launch{k * cu, 0, 0, 0});
-    for(uint32_t c : {64u, 128u, 256u, 512u}) knobs[1].push_back(kpax_launch{0, c, 0, 0});
-    for(uint32_t m : {1u, 2u, 4u, 8u}) knobs[2].push_back(kpax_launch{0, 0, m * cu, 0});
-    for(int g : {1, 0}) knobs[3].push_back(kpax_launch{0, 0, 0, g});
-    for(int k = 0; rc == KPAX_OK && k < 4; ++k)
+    std::vector<std::vector<kpax_launch>> knobs(7);
+    for(uint32_t k = 1; k <= occ; ++k) knobs[0].push_back(kpax_launch{k * cu, 0, 0, 0, 0, 0, 0});
+    for(uint32_t c : {64u, 128u, 256u, 512u}) knobs[1].push_back(kpax_launch{0, c, 0, 0, 0, 0, 0});
+    for(uint32_t t : {4096u, 1024u}) knobs[2].push_back(kpax_launch{0, 0, 0, 0, 0, 0, t});
+    for(uint32_t m : {1u, 2u, 4u}) knobs[3].push_back(kpax_launch{0, 0, m * cu, 0, 0, 0, 0});
+    for(int g : {1, 0}) knobs[4].push_back(kpax_launch{0, 0, 0, g, 0, 0, 0});
+    for(int o : {1, 0}) knobs[5].push_back(kpax_launch{0, 0, 0, 0, o, 0, 0});
+    for(uint32_t r : {1u, 8u, 16u, 24u, 32u}) knobs[6].push_back(kpax_launch{0, 0, 0, 0, 0, r, 0});
+    for(int k = 0; rc == KPAX_OK && k < 7; ++k)
         for(const kpax_launch& v : knobs[k])
         {
             kpax_launch cand = cur;
             if(k == 0) cand.expand_blocks = v.expand_blocks;
             if(k == 1) cand.expand_chunk = v.expand_chunk;
-            if(k == 2) cand.scan_blocks = v.scan_blocks;
-            if(k == 3) cand.use_graph = v.use_graph;
+            if(k == 2) cand.scan_tile = v.scan_tile;
+            if(k == 3) cand.scan_blocks = v.scan_blocks;
+            if(k == 4) cand.use_graph = v.use_graph;
+            if(k == 5) cand.order_children = v.order_children;
+            if(k == 6) cand.refill_lanes = v.refill_lanes;
             if(memcmp(&cand, &cur, sizeof(cand)) == 0) continue;
             double ms = 0.0;
             if((rc = run(cand, &ms, got)) != KPAX_OK) break;

@@ -37,10 +37,20 @@ def _free(p, obs, margin):
     return not np.any(np.all((p >= lo) & (p <= hi), axis=1))
 
 
-def serving_queries(cfg, envs, n, seed, budget=0, min_dist=4.0, budget_range=None, clearance=0.2, tries=100000):
-    """n start/goal pairs at least `min_dist` apart, both `clearance` away from
-    every obstacle and 0.5 inside the workspace faces, each in environment
-    i % len(envs)."""
+def budget_distance(budget, lo_budget=16384, hi_budget=1 << 20, near=2.0, far=9.0):
+    """Largest start-goal distance a query of this tree budget asks for: `near`
+    metres at `lo_budget` nodes up to `far` at `hi_budget`, linear in the
+    log of the budget."""
+    f = np.clip(np.log(budget / lo_budget) / np.log(hi_budget / lo_budget), 0.0, 1.0)
+    return near + (far - near) * f
+
+
+def serving_queries(cfg, envs, n, seed, budget=0, min_dist=4.0, budget_range=None, clearance=0.2, tries=100000,
+                    max_dist=None):
+    """n start/goal pairs `min_dist` to `max_dist` apart (max_dist a number or
+    a function of the query's budget), both `clearance` away from every
+    obstacle and 0.5 inside the workspace faces, each in environment
+    i % len(envs).  With `budget_range`, budgets are log-uniform in it."""
     rng = np.random.default_rng(seed)
     lo = np.array(cfg.ws_min, np.float64) + 0.5
     hi = np.array(cfg.ws_max, np.float64) - 0.5
@@ -48,18 +58,19 @@ def serving_queries(cfg, envs, n, seed, budget=0, min_dist=4.0, budget_range=Non
     S = 12 if cfg.model == kpax.MODEL_QUADCOPTER else 6
     for i in range(n):
         env = i % len(envs)
+        bud = budget
+        if budget_range is not None:
+            bud = int(np.exp(rng.uniform(np.log(budget_range[0]), np.log(budget_range[1]))))
+        dmax = np.inf if max_dist is None else (max_dist(bud) if callable(max_dist) else max_dist)
         for _ in range(tries):
             a, b = rng.uniform(lo, hi), rng.uniform(lo, hi)
-            if np.linalg.norm(a - b) >= min_dist and _free(a, envs[env], clearance) and \
-                    _free(b, envs[env], clearance):
+            d = np.linalg.norm(a - b)
+            if min_dist <= d <= dmax and _free(a, envs[env], clearance) and _free(b, envs[env], clearance):
                 break
         else:
             raise RuntimeError(f"no free start/goal pair in environment {env} after {tries} draws")
         st = np.zeros(S, np.float32)
         st[:3] = a
-        bud = budget
-        if budget_range is not None:
-            bud = int(np.exp(rng.uniform(np.log(budget_range[0]), np.log(budget_range[1]))))
         out.append(kpax.Query(st, b.astype(np.float32), seed=int(seed) * 100003 + i, budget=bud, env=env))
     return out
 

@@ -6,6 +6,7 @@ out-of-bounds access a kernel makes aborts with a report.  Exits 0 when all
 outputs equal the oracle's."""
 import os
 import sys
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -18,11 +19,14 @@ import oracle  # noqa: E402
 from workloads import random_states, workload  # noqa: E402
 
 
+T0 = time.perf_counter()
+
+
 def check(cond, what):
     if not cond:
         print(f"FAIL: {what}", flush=True)
         sys.exit(1)
-    print(f"ok: {what}", flush=True)
+    print(f"ok: {what} ({time.perf_counter() - T0:.1f} s)", flush=True)
 
 
 def small(cfg):
@@ -32,7 +36,7 @@ def small(cfg):
     return cfg
 
 
-def main():
+def ops_checks():
     rng = np.random.default_rng(7)
     for k in (1, 2, 3):
         cfg, obs, start, goal = workload(k, seed=1, max_tree=1024)
@@ -67,6 +71,9 @@ def main():
             w = rng.integers(0, 4, n).astype(np.uint32)
             out, total = p.repeat_expand(w)
             check(total == w.sum() and np.array_equal(out, np.repeat(np.arange(n), w)), f"repeat_expand n={n}")
+
+
+def plan_checks():
     # whole plans: budget exhausted, goal found, max_iter reached
     cases = ((1, 0, dict(max_tree=1500, max_iter=6), None),
              (2, 0, dict(max_tree=4096, max_iter=12), [1.6, 0.5, 1.0]),
@@ -88,8 +95,8 @@ def main():
             check(r.status == kpax.OK and r.verified == 1 and r.goal_index == o["goal_index"],
                   f"plan workload {k} reaches the goal with a verified path")
 
-    batch_checks()
-    mlp_checks()
+
+def prune_checks():
     cfg, obs, start, goal = workload(2, seed=0, max_tree=1500, max_iter=8)
     small(cfg)
     cfg.prune = 1
@@ -100,7 +107,6 @@ def main():
     check(r.tree_size == o["tree_size"] and np.array_equal(tree["parents"], o["parents"])
           and np.array_equal(tree["states"].view(np.uint32), o["states"].view(np.uint32)),
           f"pruned plan: tree {r.tree_size} vs oracle {o['tree_size']}")
-    dump_checks()
 
 
 def dump_checks():
@@ -213,11 +219,15 @@ def batch_checks():
         p.set_environment(0, obs)
         p.set_environment(1, obs1)
         res = p.plan_batch(qs)
-        p.set_launch(expand_blocks=3, expand_chunk=192, scan_blocks=2)
+        p.set_launch(expand_blocks=3, expand_chunk=192, scan_blocks=2, order_children=0, refill_lanes=1,
+                     scan_tile=1024)
         res2 = p.plan_batch(qs)
-    check(all(a.status == b.status and a.tree_size == b.tree_size and a.expansions == b.expansions
-              and a.integrated_steps == b.integrated_steps and np.array_equal(a.rows, b.rows)
-              for a, b in zip(res, res2)), "another launch shape gives the same batch results")
+        p.set_launch(refill_lanes=64)
+        res3 = p.plan_batch(qs)
+    for other, what in ((res2, "another launch shape"), (res3, "servicing a wave only when all its lanes idle")):
+        check(all(a.status == b.status and a.tree_size == b.tree_size and a.expansions == b.expansions
+                  and a.integrated_steps == b.integrated_steps and np.array_equal(a.rows, b.rows)
+                  for a, b in zip(res, other)), f"{what} gives the same batch results")
     for i, (q, r) in enumerate(zip(qs, res)):
         c = cfg.copy()
         c.max_tree = q.budget
@@ -232,5 +242,11 @@ def batch_checks():
           "batch holds both found and unfound queries")
 
 
+SECTIONS = {"ops": ops_checks, "plans": plan_checks, "batch": batch_checks, "mlp": mlp_checks,
+            "prune": prune_checks, "dump": dump_checks}
+
+
 if __name__ == "__main__":
-    main()
+    # sections named on the command line, or all of them
+    for name in sys.argv[1:] or list(SECTIONS):
+        SECTIONS[name]()

@@ -29,9 +29,21 @@ def emu_lib(tmp_path_factory):
 
 
 def test_kernels_under_asan_match_oracle(emu_lib):
+    """The check sections run as concurrent child processes (each emulated
+    work-group is a host thread; the sections share nothing)."""
     asan = subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
     env = dict(os.environ, KPAX_LIB=emu_lib, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
-    r = subprocess.run([sys.executable, os.path.join(EMU, "run_emu_checks.py")], capture_output=True, text=True,
-                       env=env, timeout=1800)
-    print(r.stdout[-4000:])
-    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-4000:])
+    groups = [("ops", "plans", "prune", "dump"), ("batch",), ("mlp",)]
+    procs = [subprocess.Popen([sys.executable, os.path.join(EMU, "run_emu_checks.py"), *g], stdout=subprocess.PIPE,
+                              stderr=subprocess.PIPE, text=True, env=env) for g in groups]
+    failed = []
+    for g, p in zip(groups, procs):
+        try:
+            out, err = p.communicate(timeout=1800)
+        except subprocess.TimeoutExpired:
+            p.kill()
+            out, err = p.communicate()
+        print(out[-4000:])
+        if p.returncode != 0:
+            failed.append((g, out[-2000:], err[-4000:]))
+    assert not failed, failed

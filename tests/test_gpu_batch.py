@@ -152,3 +152,32 @@ def test_autotune_keeps_results_and_applies_the_shape(kpax):
         after = p.plan_batch(qs)
     for a, b in zip(before, after):
         assert _same(a, b)
+
+
+def test_launch_knobs_leave_results_unchanged(kpax):
+    """A wave writes out finished children and takes new ones once
+    `refill_lanes` of its lanes are idle, and the append and frontier scans
+    run over tiles of `scan_tile` items; every child is a function of its
+    index and the scans are exact, so both knobs change only speed.  Checked
+    on a single plan's tree and on a batch."""
+    cfg, obs, start, goal = workload(2, seed=0, max_iter=25)
+    qs = [kpax.Query(start, goal, seed=70 + i, budget=50000 + 9000 * i) for i in range(6)]
+    shapes = [(1, 1024), (24, 4096), (64, 1024), (16, 4096)]
+    out = {}
+    for refill, tile in shapes:
+        with kpax.KPAX(cfg, max_queries=3, node_capacity=300000) as p:
+            p.set_obstacles(obs)
+            p.set_launch(refill_lanes=refill, scan_tile=tile)
+            assert p.launch()["refill_lanes"] == refill and p.launch()["scan_tile"] == tile
+            res = p.plan_batch(qs)
+            one = p.plan(start, goal, seed=5)
+            tree = p.tree()
+        out[(refill, tile)] = (res, one, tree)
+    ref_res, ref_one, ref_tree = out[shapes[0]]
+    for shape in shapes[1:]:
+        res, one, tree = out[shape]
+        for a, b in zip(ref_res, res):
+            assert _same(a, b), shape
+        assert _same(ref_one, one)
+        for k in ref_tree:
+            assert np.array_equal(ref_tree[k].view(np.uint8), tree[k].view(np.uint8)), (shape, k)

@@ -21,10 +21,35 @@ def test_library_exports_every_declared_symbol(kpax):
     assert not missing, missing
 
 
-def test_abi_struct_sizes_match_header(kpax):
-    # kpax_config: 29 scalar slots + arrays; compare against the C layout via offsets of the last field
-    assert ctypes.sizeof(kpax.Config) == kpax.Config.device.offset + 4
-    assert kpax.Config.max_tree.offset % 4 == 0
+def test_abi_struct_layouts_match_header(kpax, tmp_path):
+    """Every ctypes mirror of a struct in include/kpax.h has the C compiler's
+    size and field offsets (a C program built against the header prints them)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    mirrors = {"kpax_config": kpax.Config, "kpax_result": kpax._Result, "kpax_stats": kpax._Stats,
+               "kpax_launch": kpax._Launch, "kpax_query": kpax._Query}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "kpax.h"', 'int main(void) {']
+    for cname, cls in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, cls in mirrors.items():
+        assert ctypes.sizeof(cls) == got[(cname, "size")], cname
+        for f in cls._fields_:
+            assert getattr(cls, f[0]).offset == got[(cname, f[0])], (cname, f[0])
 
 
 @pytest.mark.parametrize("k,model,S,C,N,R1,R2,G,dt", [
