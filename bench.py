@@ -22,7 +22,7 @@ Also measured, each on the same GPU and stated beside the headline:
            budgets (where both solve every query) and at 2^16
 Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-proposal] [--batched-only]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-proposal] [--batched-only] [--proposal-only]
 """
 import argparse
 import json
@@ -56,10 +56,10 @@ def bytes_per_expansion(S, C):
 # HBM bytes per expansion that k_expand<quadcopter> moves on this workload,
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (average per
 # dispatch over the expansions per dispatch; FETCH scaled by 2, WRITE by 1,
-# the factors that make a 1 GiB device copy read and write 1 GiB): 156 B read
+# the factors that make a 1 GiB device copy read and write 1 GiB): 163 B read
 # (parents are read in longest-edge-first order, a cache line per state word)
-# and 95 B written (the 80-byte child and its share of partial lines).
-MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 156 + 95
+# and 96 B written (the 80-byte child and its share of partial lines).
+MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 163 + 96
 
 
 # FP32 operations per integration step as written in the kernel (add, sub,
@@ -179,6 +179,8 @@ def main():
     ap.add_argument("--no-proposal", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched-serving section (for profiling it alone); prints its JSON object")
+    ap.add_argument("--proposal-only", action="store_true",
+                    help="only the learned-proposal section (for profiling it alone); prints its JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the byte counters)")
     args = ap.parse_args()
@@ -200,6 +202,9 @@ def main():
     cfg, obs, start, goal = workload(args.workload, seed=args.env_seed)
     if args.batched_only:
         print(json.dumps({"batched": batched(cfg)}), flush=True)
+        return
+    if args.proposal_only:
+        print(json.dumps({"learned_proposal": learned_proposal(cfg)}), flush=True)
         return
     planner = kpax.KPAX(cfg)
     planner.set_obstacles(obs)

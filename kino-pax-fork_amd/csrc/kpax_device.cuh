@@ -271,47 +271,91 @@ __device__ __forceinline__ bool step_dubins(const ModelParams& P, const GridView
     return ws_and_collision(P, G, p0, p1);
 }
 
-// 12D quadcopter ode: statePropagator.cu:325-370.
-__device__ __forceinline__ void quad_ode(float* out, const float* x, const float* hh, float delta, const float* u,
-                                         const ModelParams& k)
+// 12D quadcopter ode: statePropagator.cu:325-370.  The reference's terms,
+// each sum of products evaluated with fused multiply-adds (one rounding per
+// fma; the host restatement calls std::fma on the same operands in the same
+// nesting, so the two agree bit for bit).
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// sin and cos of a + d from those of a: the angle-sum identities, with sin d
+// and cos d from their Taylor series through d^11 and d^10 when |d| <= 1
+// (truncation below 3e-9), else sincos_f32 of the rounded sum a + d.  The
+// RK4 stages move the Euler angles by d = delta * rate, small except near
+// gimbal lock, so the stages after the first need no range reduction.
+__device__ __forceinline__ void sincos_shift(float sa, float ca, float a, float d, float* s, float* c)
 {
-    const float phi = x[3] + delta * hh[3], theta = x[4] + delta * hh[4], psi = x[5] + delta * hh[5];
-    const float uu = x[6] + delta * hh[6], v = x[7] + delta * hh[7], w = x[8] + delta * hh[8];
-    const float p = x[9] + delta * hh[9], q = x[10] + delta * hh[10], r = x[11] + delta * hh[11];
-    float sf, cf, st, ct, ss, cs;
-    sincos_f32(phi, &sf, &cf);
-    sincos_f32(theta, &st, &ct);
-    sincos_f32(psi, &ss, &cs);
-    const float tt = st / ct;
-    out[0]         = ct * cs * uu + (sf * st * cs - cf * ss) * v + (cf * st * cs + sf * ss) * w;
-    out[1]         = ct * ss * uu + (sf * st * ss + cf * cs) * v + (cf * st * ss - sf * cs) * w;
-    out[2]         = -st * uu + sf * ct * v + cf * ct * w;
-    out[3]         = p + (q * sf + r * cf) * tt;
-    out[4]         = q * cf - r * sf;
-    out[5]         = (q * sf + r * cf) / ct;
-    const float XYZ = k.negnu * __builtin_sqrtf(uu * uu + v * v + w * w);
-    out[6]          = (r * v - q * w) - k.g * st + k.mi * XYZ * uu;
-    out[7]          = (p * w - r * uu) + k.g * ct * sf + k.mi * XYZ * v;
-    out[8]          = (q * uu - p * v) + k.g * ct * cf + k.mi * XYZ * w + k.mi * u[0];
-    const float LMN = k.negmu * __builtin_sqrtf(p * p + q * q + r * r);
-    out[9]          = k.k1x * q * r + k.ixi * LMN * p + k.ixi * u[1];
-    out[10]         = k.k1y * p * r + k.iyi * LMN * q + k.iyi * u[2];
-    out[11]         = k.k1z * p * q + k.izi * LMN * r + k.izi * u[3];
+    if(__builtin_fabsf(d) <= 1.0f)
+    {
+        const float d2 = d * d;
+        const float ps =
+          fma_(d2, fma_(d2, fma_(d2, fma_(d2, -2.5052108e-8f, 2.7557319e-6f), -1.9841270e-4f), 8.3333333e-3f),
+               -1.6666667e-1f);
+        const float pc =
+          fma_(d2, fma_(d2, fma_(d2, fma_(d2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f), 4.1666667e-2f), -0.5f);
+        const float sd = fma_(d * d2, ps, d);
+        const float cd = fma_(d2, pc, 1.0f);
+        *s             = fma_(sa, cd, ca * sd);
+        *c             = fma_(ca, cd, -(sa * sd));
+    }
+    else
+        sincos_f32(a + d, s, c);
 }
 
-// Quadcopter RK4: statePropagatorSpatialHash.cu:256-296.
+// sn/cn: sin and cos of the stage's roll, pitch and yaw.
+__device__ __forceinline__ void quad_ode(float* out, const float* x, const float* hh, float delta, const float* u,
+                                         const ModelParams& k, const float* sn, const float* cn)
+{
+    const float uu = fma_(delta, hh[6], x[6]), v = fma_(delta, hh[7], x[7]), w = fma_(delta, hh[8], x[8]);
+    const float p = fma_(delta, hh[9], x[9]), q = fma_(delta, hh[10], x[10]), r = fma_(delta, hh[11], x[11]);
+    const float sf = sn[0], cf = cn[0], st = sn[1], ct = cn[1], ss = sn[2], cs = cn[2];
+    const float tt   = st / ct;
+    const float sfst = sf * st, cfst = cf * st;
+    out[0]           = fma_(ct * cs, uu, fma_(fma_(sfst, cs, -(cf * ss)), v, fma_(cfst, cs, sf * ss) * w));
+    out[1]           = fma_(ct * ss, uu, fma_(fma_(sfst, ss, cf * cs), v, fma_(cfst, ss, -(sf * cs)) * w));
+    out[2]           = fma_(-st, uu, fma_(sf * ct, v, cf * ct * w));
+    const float qr   = fma_(q, sf, r * cf);
+    out[3]           = fma_(qr, tt, p);
+    out[4]           = fma_(q, cf, -(r * sf));
+    out[5]           = qr / ct;
+    const float mX   = k.mi * (k.negnu * __builtin_sqrtf(fma_(uu, uu, fma_(v, v, w * w))));
+    const float gct  = k.g * ct;
+    out[6]           = fma_(mX, uu, fma_(-k.g, st, fma_(r, v, -(q * w))));
+    out[7]           = fma_(mX, v, fma_(gct, sf, fma_(p, w, -(r * uu))));
+    out[8]           = fma_(k.mi, u[0], fma_(mX, w, fma_(gct, cf, fma_(q, uu, -(p * v)))));
+    const float LMN  = k.negmu * __builtin_sqrtf(fma_(p, p, fma_(q, q, r * r)));
+    out[9]           = fma_(k.ixi, u[1], fma_(k.ixi * LMN, p, k.k1x * q * r));
+    out[10]          = fma_(k.iyi, u[2], fma_(k.iyi * LMN, q, k.k1y * p * r));
+    out[11]          = fma_(k.izi, u[3], fma_(k.izi * LMN, r, k.k1z * p * q));
+}
+
+// Quadcopter RK4: statePropagatorSpatialHash.cu:256-296.  The stage sum
+// k1 + 2 k2 + 2 k3 + k4 is accumulated as each stage finishes (the same
+// operations in the same order), so only two stage vectors are live.
 __device__ __forceinline__ bool step_quad(const ModelParams& P, const GridView& G, float* h0, const float* u)
 {
     const float dt = P.dt, half = P.half, d6 = P.d6;
-    float k1[12], k2[12], k3[12], k4[12];
+    float ka[12], kb[12], acc[12];
     const float zero[12] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     const float p0[3] = {h0[0], h0[1], h0[2]};
-    quad_ode(k1, h0, zero, 1.0f, u, P);
-    quad_ode(k2, h0, k1, half, u, P);
-    quad_ode(k3, h0, k2, half, u, P);
-    quad_ode(k4, h0, k3, dt, u, P);
+    float sb[3], cb[3], sn[3], cn[3];
 #pragma unroll
-    for(int j = 0; j < 12; ++j) h0[j] += d6 * (k1[j] + 2.0f * k2[j] + 2.0f * k3[j] + k4[j]);
+    for(int i = 0; i < 3; ++i) sincos_f32(h0[3 + i], &sb[i], &cb[i]);
+    quad_ode(ka, h0, zero, 1.0f, u, P, sb, cb);  // k1
+#pragma unroll
+    for(int i = 0; i < 3; ++i) sincos_shift(sb[i], cb[i], h0[3 + i], half * ka[3 + i], &sn[i], &cn[i]);
+    quad_ode(kb, h0, ka, half, u, P, sn, cn);  // k2
+#pragma unroll
+    for(int j = 0; j < 12; ++j) acc[j] = fma_(2.0f, kb[j], ka[j]);
+#pragma unroll
+    for(int i = 0; i < 3; ++i) sincos_shift(sb[i], cb[i], h0[3 + i], half * kb[3 + i], &sn[i], &cn[i]);
+    quad_ode(ka, h0, kb, half, u, P, sn, cn);  // k3
+#pragma unroll
+    for(int j = 0; j < 12; ++j) acc[j] = fma_(2.0f, ka[j], acc[j]);
+#pragma unroll
+    for(int i = 0; i < 3; ++i) sincos_shift(sb[i], cb[i], h0[3 + i], dt * ka[3 + i], &sn[i], &cn[i]);
+    quad_ode(kb, h0, ka, dt, u, P, sn, cn);  // k4
+#pragma unroll
+    for(int j = 0; j < 12; ++j) h0[j] = fma_(d6, acc[j] + kb[j], h0[j]);
     if(!inb(h0[6], P.vmin, P.vmax) || !inb(h0[7], P.vmin, P.vmax) || !inb(h0[8], P.vmin, P.vmax)) return false;
     const float p1[3] = {h0[0], h0[1], h0[2]};
     return ws_and_collision(P, G, p0, p1);

@@ -96,7 +96,6 @@ struct kpax_handle {
     uint32_t* child_off     = nullptr;  // Qmax + 1 offsets, written by the advance kernel
     uint32_t* atile_off     = nullptr;
     uint32_t* ftile_off     = nullptr;
-    uint32_t* ptile_off     = nullptr;
     int32_t* slot_status    = nullptr;  // Qmax statuses + the running count
     uint64_t* st_append     = nullptr;
     uint64_t* st_frontier   = nullptr;
@@ -129,7 +128,8 @@ struct kpax_handle {
     int32_t* h_path_steps   = nullptr;
     int32_t* h_path_idx     = nullptr;
     hipEvent_t it_ev[2]     = {nullptr, nullptr};
-    hipEvent_t ph_ev[2][6]  = {};       // phase boundaries of iterations of each parity (profiling)
+    hipEvent_t ph_ev[2][6]  = {};       // phase boundaries of iterations of each parity (profiling):
+                                        // after advance, expand, append, propose, score, frontier
     hipEvent_t init_ev[2]   = {nullptr, nullptr};
     // one captured iteration per status-buffer parity, replayed while profiling is off
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};

@@ -64,8 +64,7 @@ struct BatchCtl {
     uint32_t running;         // slots running in this iteration
     uint32_t tile_expand, tile_append, tile_frontier;  // work tickets
     uint32_t epoch;           // tag of this iteration's scan status words
-    uint32_t ptiles;          // 32-row tiles of the proposal pass (an upper bound)
-    uint32_t pad[3];
+    uint32_t pad[4];
 };
 
 // Largest number of query slots of one handle (the advance kernel gives each

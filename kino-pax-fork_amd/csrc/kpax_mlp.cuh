@@ -53,12 +53,18 @@ namespace dev {
 // ---------------------------------------------------------------------------
 // bf16 as raw 16-bit patterns (round to nearest even; NaN stays NaN)
 // ---------------------------------------------------------------------------
+// fp32 -> bf16, round to nearest even (NaN stays NaN): one conversion
+// instruction on gfx950, the same rounding written out for the host build.
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
+#ifndef KPAX_HOST_EMULATION
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+#else
     uint32_t u = __builtin_bit_cast(uint32_t, f);
     if((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
+#endif
 }
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
@@ -81,11 +87,19 @@ __device__ __forceinline__ f32x16 mfma_32x32x16(uint4 a, uint4 b, f32x16 c)
 
 __device__ __forceinline__ int mfma_row(int i, int lane) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); }
 
-// GELU, tanh form.
+// GELU, tanh form: 0.5 x (1 + tanh u) written as x / (1 + e^(-2u)), with the hardware's
+// base-2 exponential and reciprocal (about 1 ulp each; the result is rounded
+// to bf16 right after, and the training gradient uses the tanh form).
 __device__ __forceinline__ float gelu(float x)
 {
     const float k = 0.7978845608028654f;  // sqrt(2 / pi)
-    return 0.5f * x * (1.0f + tanhf(k * (x + 0.044715f * x * x * x)));
+    const float u = k * (x + 0.044715f * x * x * x);
+#ifndef KPAX_HOST_EMULATION
+    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * u);  // e^(-2u) = 2^(-2u log2 e)
+    return x * __builtin_amdgcn_rcpf(1.0f + e);
+#else
+    return x / (1.0f + exp2f(-2.8853900817779268f * u));
+#endif
 }
 
 __device__ __forceinline__ float gelu_grad(float x)

@@ -292,8 +292,6 @@ __global__ __launch_bounds__(256) void k_propose_sample(ModelParams P, const flo
 struct ProposeArgs {
     const Ctl* ctl;
     const QueryDev* qd;
-    const BatchCtl* bc;
-    const uint32_t* ptile_off;
     const GridView* envs;
     const float* tree_st;
     float* prop;
@@ -303,11 +301,59 @@ struct ProposeArgs {
     uint32_t n_init;
 };
 
+// Per-slot table of a proposal pass, in LDS after the network's buffers:
+// tile offsets (Q + 1), first node and node count of each slot.
+constexpr size_t kProposeLdsBytes = kMlpLdsBytes + (3 * (size_t)kMaxSlots + 1) * sizeof(uint32_t);
+static_assert(kProposeLdsBytes <= 160 * 1024, "proposal pass exceeds the LDS of a CU");
+
+// Gaussian of every node appended this iteration (or of every root being
+// initialised, n_init > 0): each block first lays out the slots' 32-row tiles
+// from the counts the append kernel left (ctl[q].Fn), so a block with no tile
+// exits before loading the weights and the tile loop reads only LDS.
 __global__ __launch_bounds__(256) void k_propose(const MlpDev* Wd, ProposeArgs A)
 {
     HIP_DYNAMIC_SHARED(char, lds)
-    const uint32_t ntiles = A.n_init ? A.n_init : A.bc->ptiles;
-    if(ntiles == 0) return;
+    __shared__ uint32_t s_scan[4];
+    uint32_t* s_off  = (uint32_t*)(lds + kMlpLdsBytes);
+    uint32_t* s_node = s_off + kMaxSlots + 1;
+    uint32_t* s_nv   = s_node + kMaxSlots;
+    const uint32_t n = A.n_init ? A.n_init : A.Q;  // table entries (n <= kMaxSlots = 4 x 256)
+    uint32_t cnt[4], mine = 0;
+#pragma unroll
+    for(int j = 0; j < 4; ++j)
+    {
+        const uint32_t k = threadIdx.x * 4 + j;
+        uint32_t nv = 0, node0 = 0;
+        if(k < n)
+        {
+            if(A.n_init)
+            {
+                nv    = 1;
+                node0 = A.qd[A.init_slots[k]].base;
+            }
+            else if(A.ctl[k].cur.status == kRunning)
+            {
+                nv    = A.ctl[k].Fn;
+                node0 = A.qd[k].base + A.ctl[k].cur.ts;
+            }
+            s_node[k] = node0;
+            s_nv[k]   = nv;
+        }
+        cnt[j] = (nv + 31) / 32;
+        mine += cnt[j];
+    }
+    uint32_t ntiles;
+    uint32_t run = block_excl_scan<4>(mine, &ntiles, s_scan);
+#pragma unroll
+    for(int j = 0; j < 4; ++j)
+    {
+        const uint32_t k = threadIdx.x * 4 + j;
+        if(k < n) s_off[k] = run;
+        run += cnt[j];
+    }
+    if(threadIdx.x == 0) s_off[n] = ntiles;
+    __syncthreads();
+    if(blockIdx.x * kMlpWaves >= ntiles) return;  // the same for every thread of the block
     load_weights_lds(Wd, lds);
     const MlpDev& W = *(const MlpDev*)lds;
     uint16_t(*xa)[kLdH];
@@ -316,22 +362,12 @@ __global__ __launch_bounds__(256) void k_propose(const MlpDev* Wd, ProposeArgs A
     const int lane = threadIdx.x & 63, r = lane & 31;
     for(uint32_t tile = blockIdx.x * kMlpWaves + (threadIdx.x >> 6); tile < ntiles; tile += gridDim.x * kMlpWaves)
     {
-        uint32_t q, node0, nv;
-        if(A.n_init)
-        {
-            q     = A.init_slots[tile];
-            node0 = A.qd[q].base;
-            nv    = 1;
-        }
-        else
-        {
-            q                 = segment_of(A.ptile_off, A.Q, tile);
-            const uint32_t lt = tile - A.ptile_off[q];
-            const uint32_t Fn = A.ctl[q].Fn;
-            nv                = Fn > lt * 32 ? (Fn - lt * 32 < 32 ? Fn - lt * 32 : 32u) : 0u;
-            node0             = A.qd[q].base + A.ctl[q].cur.ts + lt * 32;
-        }
-        if(nv == 0) continue;
+        const uint32_t k     = segment_of(s_off, n, tile);
+        const uint32_t lt    = tile - s_off[k];
+        const uint32_t left  = s_nv[k] - lt * 32;
+        const uint32_t nv    = left < 32 ? left : 32u;
+        const uint32_t node0 = s_node[k] + lt * 32;
+        const uint32_t q     = A.n_init ? A.init_slots[k] : k;
         if(lane < 32)
         {
             float f[kMlpIn];
@@ -343,8 +379,8 @@ __global__ __launch_bounds__(256) void k_propose(const MlpDev* Wd, ProposeArgs A
                 features(A.F, A.envs[d.env].occ, x, d.goal, f);
             }
             else
-                for(int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
-            for(int k = 0; k < kMlpIn; ++k) xa[r][k] = f2bf(f[k]);
+                for(int kk = 0; kk < kMlpIn; ++kk) f[kk] = 0.0f;
+            for(int kk = 0; kk < kMlpIn; ++kk) xa[r][kk] = f2bf(f[kk]);
         }
         wave_lds_sync();
         float o[16];
@@ -676,8 +712,6 @@ int launch_propose(kpax_handle* h, uint32_t n_init)
     ProposeArgs A;
     A.ctl        = h->ctl;
     A.qd         = h->qd;
-    A.bc         = h->bc;
-    A.ptile_off  = h->ptile_off;
     A.envs       = h->envs_dev;
     A.tree_st    = h->tree.st;
     A.prop       = h->prop;
@@ -688,7 +722,7 @@ int launch_propose(kpax_handle* h, uint32_t n_init)
     A.n_init     = n_init;
     const uint32_t blocks = n_init ? std::max<uint32_t>(1, (n_init + kMlpWaves - 1) / kMlpWaves)
                                    : (uint32_t)h->cu_count;
-    hipLaunchKernelGGL(k_propose, dim3(blocks), dim3(256), kMlpLdsBytes, h->stream, (const MlpDev*)h->mlp, A);
+    hipLaunchKernelGGL(k_propose, dim3(blocks), dim3(256), kProposeLdsBytes, h->stream, (const MlpDev*)h->mlp, A);
     HIPC(hipGetLastError());
     return KPAX_OK;
 }

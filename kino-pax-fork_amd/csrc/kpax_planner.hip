@@ -156,12 +156,12 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
                                                   uint32_t* atile_off, uint32_t* ftile_off, int32_t* slot_status,
                                                   uint32_t Q, uint64_t* st_append, uint64_t* st_frontier,
                                                   uint32_t n_status, PlanParams PP, const GridView* envs,
-                                                  SlotUniform* s0, uint32_t* ptile_off, uint32_t tile)
+                                                  SlotUniform* s0, uint32_t tile)
 {
     __shared__ uint32_t lds[16];
     __shared__ uint32_t s_epoch;
     const uint32_t q = threadIdx.x;
-    uint32_t m = 0, at = 0, ft = 0, pt = 0, run = 0;
+    uint32_t m = 0, at = 0, ft = 0, run = 0;
     if(q < Q)
     {
         Ctl* c       = &ctl[q];
@@ -181,30 +181,25 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
             m           = n.M;
             at          = (n.M + tile - 1) / tile;
             ft          = (uint32_t)(((uint64_t)n.ts + n.M + tile - 1) / tile);
-            pt          = (n.M + 31) / 32;
             run         = 1;
         }
     }
-    uint32_t tm, ta, tf, tp, tr;
+    uint32_t tm, ta, tf, tr;
     const uint32_t em = block_excl_scan<16>(m, &tm, lds);
     const uint32_t ea = block_excl_scan<16>(at, &ta, lds);
     const uint32_t ef = block_excl_scan<16>(ft, &tf, lds);
-    const uint32_t ep = block_excl_scan<16>(pt, &tp, lds);
     (void)block_excl_scan<16>(run, &tr, lds);
     if(q < Q)
     {
         child_off[q] = em;
         atile_off[q] = ea;
         ftile_off[q] = ef;
-        ptile_off[q] = ep;
     }
     if(q == 0)
     {
         child_off[Q]     = tm;
         atile_off[Q]     = ta;
         ftile_off[Q]     = tf;
-        ptile_off[Q]     = tp;
-        bc->ptiles       = tp;
         bc->children     = tm;
         bc->atiles       = ta;
         bc->ftiles       = tf;
@@ -420,11 +415,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
 // longest first, so the last children a launch starts are short ones and the
 // launch does not wait on a long edge begun at its end.  Results do not depend
 // on the order (every child is a function of its index and of the tree
-// before the iteration).  Three launches: bucket counts per block, one
-// exclusive scan over (bucket, block), scatter.
+// before the iteration).  Two launches: bucket counts per block, then the
+// scatter, each of whose blocks sums the counts it needs.
 // ---------------------------------------------------------------------------
 constexpr int kOrderBlocks = 256, kOrderBuckets = 32;
-static_assert(kOrderBlocks * kOrderBuckets <= 8192, "k_order_scan covers 1024 x 8 counts");
+static_assert(256 % kOrderBuckets == 0, "k_order_scatter splits its 256 threads evenly over the buckets");
 
 __device__ __forceinline__ uint32_t order_bucket(const ModelParams& P, int n)
 {
@@ -479,36 +474,41 @@ __global__ __launch_bounds__(256) void k_order_count(const ExpandArgs* __restric
     if(threadIdx.x < kOrderBuckets) hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the (bucket-major) counts, in place; one block of 1024,
-// eight consecutive counts per thread (n <= 8192).
-__global__ __launch_bounds__(1024) void k_order_scan(uint32_t* hist, uint32_t n)
-{
-    __shared__ uint32_t lds[16];
-    const uint32_t i0 = threadIdx.x * 8;
-    uint32_t v[8], sum = 0;
-#pragma unroll
-    for(int j = 0; j < 8; ++j)
-    {
-        v[j] = i0 + j < n ? hist[i0 + j] : 0u;
-        sum += v[j];
-    }
-    uint32_t agg;
-    uint32_t run = block_excl_scan<16>(sum, &agg, lds);
-#pragma unroll
-    for(int j = 0; j < 8; ++j)
-        if(i0 + j < n)
-        {
-            hist[i0 + j] = run;
-            run += v[j];
-        }
-}
+// Each block finds its own first position per bucket from the raw counts:
+// the children of all earlier buckets plus those of this bucket counted by
+// earlier blocks.  Thread (bucket b, part j) sums an eighth of b's block
+// counts; one thread per bucket then adds up the parts.
+constexpr int kOrderParts = 256 / kOrderBuckets;
 
 template<bool MULTI>
 __global__ __launch_bounds__(256) void k_order_scatter(const ExpandArgs* __restrict__ A, const uint32_t* hist,
                                                        uint32_t* order)
 {
     __shared__ uint32_t pos[kOrderBuckets];
-    if(threadIdx.x < kOrderBuckets) pos[threadIdx.x] = hist[threadIdx.x * gridDim.x + blockIdx.x];
+    __shared__ uint32_t s_all[kOrderBuckets][kOrderParts], s_before[kOrderBuckets][kOrderParts];
+    {
+        const uint32_t b = threadIdx.x / kOrderParts, part = threadIdx.x % kOrderParts;
+        const uint32_t per = (gridDim.x + kOrderParts - 1) / kOrderParts;
+        const uint32_t j0 = part * per, j1 = j0 + per < gridDim.x ? j0 + per : gridDim.x;
+        uint32_t all = 0, before = 0;
+        for(uint32_t j = j0; j < j1; ++j)
+        {
+            const uint32_t v = hist[b * gridDim.x + j];
+            all += v;
+            before += j < blockIdx.x ? v : 0u;
+        }
+        s_all[b][part]    = all;
+        s_before[b][part] = before;
+    }
+    __syncthreads();
+    if(threadIdx.x < kOrderBuckets)
+    {
+        uint32_t p = 0;
+        for(uint32_t b = 0; b < threadIdx.x; ++b)
+            for(int j = 0; j < kOrderParts; ++j) p += s_all[b][j];
+        for(int j = 0; j < kOrderParts; ++j) p += s_before[threadIdx.x][j];
+        pos[threadIdx.x] = p;
+    }
     __syncthreads();
     const uint32_t total = order_total<MULTI>(A);
     const uint32_t per   = (total + gridDim.x - 1) / gridDim.x;
@@ -931,16 +931,12 @@ void launch_order(kpax_handle* h)
     if(h->Qmax == 1)
     {
         hipLaunchKernelGGL((k_order_count<false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
-        hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, h->stream, h->order_hist,
-                           (uint32_t)(kOrderBlocks * kOrderBuckets));
         hipLaunchKernelGGL((k_order_scatter<false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist,
                            h->order);
     }
     else
     {
         hipLaunchKernelGGL((k_order_count<true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
-        hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, h->stream, h->order_hist,
-                           (uint32_t)(kOrderBlocks * kOrderBuckets));
         hipLaunchKernelGGL((k_order_scatter<true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist,
                            h->order);
     }
@@ -965,7 +961,7 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
     const uint32_t Q = h->Qmax, R1 = (uint32_t)h->RP.r1_count;
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc, h->child_off, h->atile_off,
                        h->ftile_off, h->slot_status, Q, h->st_append, h->st_frontier, h->n_status, h->PP, h->envs_dev,
-                       &((ExpandArgs*)h->expand_args)->s0, h->ptile_off, h->scan_tile);
+                       &((ExpandArgs*)h->expand_args)->s0, h->scan_tile);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
     launch_expand(h);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][1], h->stream));
@@ -977,21 +973,22 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
         hipLaunchKernelGGL(k_append<256>, dim3(h->scan_blocks), dim3(256), append_lds_bytes(h), h->stream, h->ctl,
                            h->qd, h->bc, h->atile_off, Q, h->ch, h->tree, h->rv, h->st_append, h->PP, h->S, h->C, h->T,
                            R1, h->R2);
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][2], h->stream));
     if(h->mix > 0.0f)
     {
         const int rc = launch_propose(h, 0);
         if(rc != KPAX_OK) return rc;
     }
-    if(ev) HIPC(hipEventRecord(h->ph_ev[p][2], h->stream));
-    hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][3], h->stream));
+    hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
     if(h->scan_tile == 4 * 1024)
         hipLaunchKernelGGL(k_frontier<1024>, dim3(h->scan_blocks), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc,
                            h->ftile_off, h->slot_status, Q, h->tree, h->R, h->rv, h->st_frontier, h->PP, R1);
     else
         hipLaunchKernelGGL(k_frontier<256>, dim3(h->scan_blocks), dim3(256), 0, h->stream, h->ctl, h->qd, h->bc,
                            h->ftile_off, h->slot_status, Q, h->tree, h->R, h->rv, h->st_frontier, h->PP, R1);
-    if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][5], h->stream));
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(h->h_status + (size_t)p * (Q + 1), h->slot_status, (Q + 1) * sizeof(int32_t),
                         hipMemcpyDeviceToHost, h->stream));
@@ -1082,13 +1079,18 @@ struct QueryIn {
 void account_iteration(kpax_handle* h, int p)
 {
     if(h->h_status[(size_t)p * (h->Qmax + 1) + h->Qmax] <= 0) return;  // no slot was running
+    // event pairs -> phases expand, append, propose, score, frontier
+    constexpr int kPhase[5] = {0, 1, 5, 2, 3};
     float ms = 0.0f;
-    for(int k = 0; k < 4; ++k)
+    for(int k = 0; k < 5; ++k)
+    {
+        if(kPhase[k] == 5 && h->mix <= 0.0f) continue;
         if(hipEventElapsedTime(&ms, h->ph_ev[p][k], h->ph_ev[p][k + 1]) == hipSuccess)
         {
-            h->stats.phase_ms[k] += ms;
-            h->stats.phase_launches[k] += 1;
+            h->stats.phase_ms[kPhase[k]] += ms;
+            h->stats.phase_launches[kPhase[k]] += 1;
         }
+    }
 }
 
 // Runs queries through the handle's slots until every one has finished.
@@ -1498,7 +1500,6 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->child_off, Q + 1));
     TRY(dalloc(h, &h->atile_off, Q + 1));
     TRY(dalloc(h, &h->ftile_off, Q + 1));
-    TRY(dalloc(h, &h->ptile_off, Q + 1));
     TRY(dalloc(h, &h->order, T));
     TRY(dalloc(h, &h->order_hist, (size_t)kOrderBlocks * kOrderBuckets));
     {

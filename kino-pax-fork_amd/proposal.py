@@ -112,15 +112,25 @@ def train(X, Y, epochs=20, batch=4096, lr=2e-3, seed=0, init_seed=1):
         return t.params(), losses
 
 
-def evaluate(planner, queries, params=None, mix=0.5):
+def evaluate(planner, queries, params=None, mix=0.5, phases=True):
     """Plan `queries` one at a time (the single-query form) with the plain
     sampler (params None) or the mixture; success = goal reached within the
-    budget and the FP64 re-check passed."""
+    budget and the FP64 re-check passed.  With `phases`, a second pass over
+    the same queries with HIP events on gives the device time per phase."""
     planner.set_proposal(params, mix if params is not None else 0.0)
     res = planner.plan_batch(queries[:1])  # warm-up
     t0 = time.perf_counter()
     res = [planner.plan_batch([q])[0] for q in queries]
     wall = time.perf_counter() - t0
+    phase_ms = None
+    if phases:
+        planner.set_profiling(True)
+        planner.reset_stats()
+        for q in queries:
+            planner.plan_batch([q])
+        st = planner.stats()
+        planner.set_profiling(False)
+        phase_ms = {k: v / max(1, st["plans"]) for k, v in st["phase_ms"].items()}
     planner.set_proposal(None)
     ok = [r for r in res if r.status == kpax.OK and r.verified == 1]
     return {
@@ -129,5 +139,6 @@ def evaluate(planner, queries, params=None, mix=0.5):
         "p50_ms_found": statistics.median(r.plan_ms for r in ok) if ok else None,
         "mean_iterations_found": statistics.mean(r.iterations for r in ok) if ok else None,
         "mean_expansions": statistics.mean(r.expansions for r in res),
+        "phase_ms_per_plan": phase_ms,
         "results": res,
     }

@@ -142,6 +142,28 @@ template<> struct Math<float> {
     static void sincos(float x, float* s, float* c) { sincos_f32(x, s, c); }
     static float tan_from(float s, float c, float) { return s / c; }
     static float sqrt(float x) { return std::sqrt(x); }
+    static float fma(float a, float b, float c) { return std::fma(a, b, c); }
+    // sin/cos of a + d from those of a (the kernel's sincos_shift): angle-sum
+    // identities with Taylor series of sin d and cos d through d^11 / d^10
+    // for |d| <= 1, else sincos_f32 of the rounded sum.
+    static void sincos_shift(float sa, float ca, float a, float d, float* s, float* c)
+    {
+        if(std::fabs(d) <= 1.0f)
+        {
+            const float d2 = d * d;
+            const float ps = fma(
+              d2, fma(d2, fma(d2, fma(d2, -2.5052108e-8f, 2.7557319e-6f), -1.9841270e-4f), 8.3333333e-3f),
+              -1.6666667e-1f);
+            const float pc =
+              fma(d2, fma(d2, fma(d2, fma(d2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f), 4.1666667e-2f), -0.5f);
+            const float sd = fma(d * d2, ps, d);
+            const float cd = fma(d2, pc, 1.0f);
+            *s             = fma(sa, cd, ca * sd);
+            *c             = fma(ca, cd, -(sa * sd));
+        }
+        else
+            sincos_f32(a + d, s, c);
+    }
 };
 template<> struct Math<double> {
     static void sincos(double x, double* s, double* c)
@@ -151,6 +173,8 @@ template<> struct Math<double> {
     }
     static double tan_from(double, double, double x) { return std::tan(x); }
     static double sqrt(double x) { return std::sqrt(x); }
+    static double fma(double a, double b, double c) { return std::fma(a, b, c); }
+    static void sincos_shift(double, double, double a, double d, double* s, double* c) { sincos(a + d, s, c); }
 };
 
 // Float truncation toward zero clamped to [0, len-1]; NaN maps to 0.  Equal
@@ -339,32 +363,37 @@ QuadK<T> quad_constants(const kpax_config& c)
     return k;
 }
 
-// ode(), src/statePropagator/statePropagator.cu:325-370.
+// ode(), src/statePropagator/statePropagator.cu:325-370: the reference's
+// terms, each sum of products as nested fused multiply-adds (one rounding per
+// fma), the same nesting as the kernel's quad_ode.
+// sn/cn: sin and cos of the stage's roll, pitch and yaw (theta is the pitch,
+// for the double path's tan).
 template<class T>
-void quad_ode(T* out, const T* x, const T* hh, T delta, const T* u, const QuadK<T>& k)
+void quad_ode(T* out, const T* x, const T* hh, T delta, const T* u, const QuadK<T>& k, const T* sn, const T* cn,
+              T theta)
 {
-    const T phi = x[3] + delta * hh[3], theta = x[4] + delta * hh[4], psi = x[5] + delta * hh[5];
-    const T uu = x[6] + delta * hh[6], v = x[7] + delta * hh[7], w = x[8] + delta * hh[8];
-    const T p = x[9] + delta * hh[9], q = x[10] + delta * hh[10], r = x[11] + delta * hh[11];
-    T sf, cf, st, ct, ss, cs;
-    Math<T>::sincos(phi, &sf, &cf);
-    Math<T>::sincos(theta, &st, &ct);
-    Math<T>::sincos(psi, &ss, &cs);
-    const T tt = Math<T>::tan_from(st, ct, theta);
-    out[0] = ct * cs * uu + (sf * st * cs - cf * ss) * v + (cf * st * cs + sf * ss) * w;
-    out[1] = ct * ss * uu + (sf * st * ss + cf * cs) * v + (cf * st * ss - sf * cs) * w;
-    out[2] = -st * uu + sf * ct * v + cf * ct * w;
-    out[3] = p + (q * sf + r * cf) * tt;
-    out[4] = q * cf - r * sf;
-    out[5] = (q * sf + r * cf) / ct;
-    const T XYZ = k.negnu * Math<T>::sqrt(uu * uu + v * v + w * w);
-    out[6]      = (r * v - q * w) - k.g * st + k.mi * XYZ * uu;
-    out[7]      = (p * w - r * uu) + k.g * ct * sf + k.mi * XYZ * v;
-    out[8]      = (q * uu - p * v) + k.g * ct * cf + k.mi * XYZ * w + k.mi * u[0];
-    const T LMN = k.negmu * Math<T>::sqrt(p * p + q * q + r * r);
-    out[9]      = k.k1x * q * r + k.ixi * LMN * p + k.ixi * u[1];
-    out[10]     = k.k1y * p * r + k.iyi * LMN * q + k.iyi * u[2];
-    out[11]     = k.k1z * p * q + k.izi * LMN * r + k.izi * u[3];
+    auto fma      = [](T a, T b, T c) { return Math<T>::fma(a, b, c); };
+    const T uu    = fma(delta, hh[6], x[6]), v = fma(delta, hh[7], x[7]), w = fma(delta, hh[8], x[8]);
+    const T p     = fma(delta, hh[9], x[9]), q = fma(delta, hh[10], x[10]), r = fma(delta, hh[11], x[11]);
+    const T sf = sn[0], cf = cn[0], st = sn[1], ct = cn[1], ss = sn[2], cs = cn[2];
+    const T tt   = Math<T>::tan_from(st, ct, theta);
+    const T sfst = sf * st, cfst = cf * st;
+    out[0]       = fma(ct * cs, uu, fma(fma(sfst, cs, -(cf * ss)), v, fma(cfst, cs, sf * ss) * w));
+    out[1]       = fma(ct * ss, uu, fma(fma(sfst, ss, cf * cs), v, fma(cfst, ss, -(sf * cs)) * w));
+    out[2]       = fma(-st, uu, fma(sf * ct, v, cf * ct * w));
+    const T qr   = fma(q, sf, r * cf);
+    out[3]       = fma(qr, tt, p);
+    out[4]       = fma(q, cf, -(r * sf));
+    out[5]       = qr / ct;
+    const T mX   = k.mi * (k.negnu * Math<T>::sqrt(fma(uu, uu, fma(v, v, w * w))));
+    const T gct  = k.g * ct;
+    out[6]       = fma(mX, uu, fma(-k.g, st, fma(r, v, -(q * w))));
+    out[7]       = fma(mX, v, fma(gct, sf, fma(p, w, -(r * uu))));
+    out[8]       = fma(k.mi, u[0], fma(mX, w, fma(gct, cf, fma(q, uu, -(p * v)))));
+    const T LMN  = k.negmu * Math<T>::sqrt(fma(p, p, fma(q, q, r * r)));
+    out[9]       = fma(k.ixi, u[1], fma(k.ixi * LMN, p, k.k1x * q * r));
+    out[10]      = fma(k.iyi, u[2], fma(k.iyi * LMN, q, k.k1y * p * r));
+    out[11]      = fma(k.izi, u[3], fma(k.izi * LMN, r, k.k1z * p * q));
 }
 
 // Quadcopter RK4, statePropagatorSpatialHash.cu:234-308.
@@ -382,11 +411,18 @@ int propagate_quad(const Env& e, const T* x0, const T* u, int steps, T* x1)
     for(int i = 0; i < steps; ++i)
     {
         const T p0[3] = {h0[0], h0[1], h0[2]};
-        quad_ode<T>(k1, h0, zero, T(1), u, k);
-        quad_ode<T>(k2, h0, k1, half, u, k);
-        quad_ode<T>(k3, h0, k2, half, u, k);
-        quad_ode<T>(k4, h0, k3, dt, u, k);
-        for(int j = 0; j < 12; ++j) h0[j] += d6 * (k1[j] + T(2) * k2[j] + T(2) * k3[j] + k4[j]);
+        // stage angles: the state's own, then shifted by delta * rate
+        T sb[3], cb[3], sn[3], cn[3];
+        for(int j = 0; j < 3; ++j) Math<T>::sincos(h0[3 + j], &sb[j], &cb[j]);
+        quad_ode<T>(k1, h0, zero, T(1), u, k, sb, cb, h0[4]);
+        for(int j = 0; j < 3; ++j) Math<T>::sincos_shift(sb[j], cb[j], h0[3 + j], half * k1[3 + j], &sn[j], &cn[j]);
+        quad_ode<T>(k2, h0, k1, half, u, k, sn, cn, h0[4] + half * k1[4]);
+        for(int j = 0; j < 3; ++j) Math<T>::sincos_shift(sb[j], cb[j], h0[3 + j], half * k2[3 + j], &sn[j], &cn[j]);
+        quad_ode<T>(k3, h0, k2, half, u, k, sn, cn, h0[4] + half * k2[4]);
+        for(int j = 0; j < 3; ++j) Math<T>::sincos_shift(sb[j], cb[j], h0[3 + j], dt * k3[3 + j], &sn[j], &cn[j]);
+        quad_ode<T>(k4, h0, k3, dt, u, k, sn, cn, h0[4] + dt * k3[4]);
+        for(int j = 0; j < 12; ++j)
+            h0[j] = Math<T>::fma(d6, Math<T>::fma(T(2), k3[j], Math<T>::fma(T(2), k2[j], k1[j])) + k4[j], h0[j]);
         if(!in_box(h0[6], vmin, vmax) || !in_box(h0[7], vmin, vmax) || !in_box(h0[8], vmin, vmax))
         {
             fail = i + 1;
@@ -708,6 +744,16 @@ void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out)
     U4 c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
     U4 r = philox4x32_10(c, key[0], key[1]);
     for(int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+
+void oracle_sincos_shift(const float* a, const float* d, uint32_t n, float* s, float* c)
+{
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        float sa, ca;
+        sincos_f32(a[i], &sa, &ca);
+        Math<float>::sincos_shift(sa, ca, a[i], d[i], &s[i], &c[i]);
+    }
 }
 
 void oracle_sincos_f32(const float* x, uint32_t n, float* s, float* c)

@@ -24,6 +24,7 @@ def lib():
         sig = {
             "oracle_philox": (None, [P, P, P]),
             "oracle_sincos_f32": (None, [P, u32, P, P]),
+            "oracle_sincos_shift": (None, [P, P, u32, P, P]),
             "oracle_propagate": (i32, [P, P, u32, P, P, P, u32, i32, P, P, P]),
             "oracle_collide": (None, [P, u32, P, P, u32, P]),
             "oracle_grid_counts": (None, [P, P, u32, P]),
@@ -64,6 +65,17 @@ def sincos_f32(x):
     s = np.zeros_like(x)
     c = np.zeros_like(x)
     lib().oracle_sincos_f32(_p(x), x.size, _p(s), _p(c))
+    return s, c
+
+
+def sincos_shift(a, d):
+    """sin and cos of a + d as the quadcopter's RK4 stages compute them: from
+    sincos_f32(a) by the angle-sum identities (|d| <= 1) or sincos_f32(a + d)."""
+    a = np.ascontiguousarray(a, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    s = np.zeros_like(a)
+    c = np.zeros_like(a)
+    lib().oracle_sincos_shift(_p(a), _p(d), a.size, _p(s), _p(c))
     return s, c
 
 

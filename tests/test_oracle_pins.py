@@ -57,6 +57,27 @@ def test_sincos_within_two_ulp(oracle):
     assert s[x.size - 8] == 0 and c[x.size - 8] == 1
 
 
+def test_sincos_shift_matches_the_exact_shifted_angle(oracle, rng):
+    """The RK4 stages' sin/cos of a + d: angle-sum identities for |d| <= 1
+    (no rounding of the argument: within 4 ulp of 1 of the exact value),
+    sincos_f32 of the float sum beyond (its argument rounding, half an ulp of
+    a + d, plus 2 ulp).  A sign or coefficient slip in either series or in
+    the identities is off by orders of magnitude."""
+    a = rng.uniform(-8, 8, 300000).astype(np.float32)
+    d = np.concatenate([rng.uniform(-1, 1, 200000), rng.uniform(-3, 3, 100000)]).astype(np.float32)
+    a[:5] = [0, 0, 1.5707964, -3.1415927, 7.9]
+    d[:5] = [0, 1.0, -1.0, 1e-20, -0.999]
+    s, c = oracle.sincos_shift(a, d)
+    exact = a.astype(np.float64) + d.astype(np.float64)
+    small = np.abs(d) <= 1
+    arg_err = 0.5 * np.spacing(np.abs(exact).astype(np.float32)).astype(np.float64)
+    bound = np.where(small, 4 * 2.0 ** -24, arg_err + 2 * 2.0 ** -24)
+    for got, want in ((s, np.sin(exact)), (c, np.cos(exact))):
+        err = np.abs(got.astype(np.float64) - want)
+        assert np.all(err <= bound), (err / bound).max()
+    assert small.sum() > 200000 and (~small).sum() > 30000
+
+
 # --- double integrator ------------------------------------------------------
 def _di_closed_form(x0, u, n, dt, textbook):
     p, v, a = x0[:3], x0[3:6], u
