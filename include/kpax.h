@@ -370,6 +370,16 @@ int kpax_scores(kpax_handle* h, const int32_t* valid_count, const int32_t* total
 int kpax_repeat_expand(kpax_handle* h, const uint32_t* weights, uint32_t n, uint32_t* out,
                        uint32_t out_cap, uint32_t* total);
 
+/* Stream compaction of a flag array: out lists every index i with
+ * flags[i] != 0, in increasing i; *count = their number (out must hold
+ * out_cap entries; at most out_cap are written).  Replaces the frontier mask
+ * scan + findInd (KPAX.cu:310-321 with its +1 fix-up for a set last element,
+ * Planner.cu:52-58) and the next-frontier compaction (KPAX.cu:523-525, which
+ * drops a set last element; here it is kept).  The repeat-expansion above
+ * with weights 0 / 1, on the same kernel. */
+int kpax_scan_compact(kpax_handle* h, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t out_cap,
+                      uint32_t* count);
+
 /* Copy the collision grid built by kpax_set_obstacles: cell_start holds
  * G + 1 offsets (cells x fastest, gridToIndex of spatialHash.cuh:57-60),
  * items holds the obstacle ids of every cell in ascending order.  Replaces

@@ -100,18 +100,23 @@ __device__ __forceinline__ bool inb(float v, float lo, float hi) { return v >= l
 // Uniform grid in CSR form: start[G+1], items[K] (obstacle ids ascending per
 // cell), obstacle boxes as two float4 arrays.  Built once per environment.
 // ---------------------------------------------------------------------------
+// The CSR items hold the obstacles' boxes themselves (two float4 per item,
+// copied from the obstacle list after the build), so testing an item is one
+// pair of independent loads instead of an index load and then the box.
 struct GridView {
     GridParams gp;
     const int32_t* start;
-    const int32_t* items;
-    const float4* lo;
-    const float4* hi;
+    const float4* box;
     const uint32_t* occ;  // occupancy bitmap (learned proposal's features)
 };
 
-__device__ __forceinline__ int cell_of(float x, float mn, float cell, int D)
+// Cell of coordinate x: a multiply by cells per metre instead of a division
+// by the cell size.  The grid build and every query use this same monotone
+// map, so a box and every obstacle it overlaps share a cell whatever the
+// rounding.
+__device__ __forceinline__ int cell_of(float x, float mn, float inv, int D)
 {
-    return clamp_index((x - mn) / cell, D);
+    return clamp_index((x - mn) * inv, D);
 }
 
 // Segment box vs all obstacles in the cells it overlaps
@@ -120,9 +125,8 @@ __device__ __forceinline__ int cell_of(float x, float mn, float cell, int D)
 // least one cell with it, so the verdict equals the brute-force one.
 __device__ __forceinline__ bool box_hits(const GridView& g, int k, const float bmin[3], const float bmax[3])
 {
-    const int o    = g.items[k];
-    const float4 a = g.lo[o];
-    const float4 b = g.hi[o];
+    const float4 a = g.box[2 * (size_t)k];
+    const float4 b = g.box[2 * (size_t)k + 1];
     const bool sep = bmax[0] <= a.x || b.x <= bmin[0] || bmax[1] <= a.y || b.y <= bmin[1] || bmax[2] <= a.z ||
                      b.z <= bmin[2];
     return !sep;
@@ -134,8 +138,8 @@ __device__ __forceinline__ bool seg_free(const GridView& g, const float bmin[3],
 #pragma unroll
     for(int d = 0; d < 3; ++d)
     {
-        lo[d] = cell_of(bmin[d], g.gp.mn[d], g.gp.cell[d], g.gp.D[d]);
-        hi[d] = cell_of(bmax[d], g.gp.mn[d], g.gp.cell[d], g.gp.D[d]);
+        lo[d] = cell_of(bmin[d], g.gp.mn[d], g.gp.inv[d], g.gp.D[d]);
+        hi[d] = cell_of(bmax[d], g.gp.mn[d], g.gp.inv[d], g.gp.D[d]);
     }
     if(hi[0] - lo[0] <= 1 && hi[1] - lo[1] <= 1 && hi[2] - lo[2] <= 1)
     {
@@ -308,7 +312,8 @@ __device__ __forceinline__ void quad_ode(float* out, const float* x, const float
     const float uu = fma_(delta, hh[6], x[6]), v = fma_(delta, hh[7], x[7]), w = fma_(delta, hh[8], x[8]);
     const float p = fma_(delta, hh[9], x[9]), q = fma_(delta, hh[10], x[10]), r = fma_(delta, hh[11], x[11]);
     const float sf = sn[0], cf = cn[0], st = sn[1], ct = cn[1], ss = sn[2], cs = cn[2];
-    const float tt   = st / ct;
+    const float sec  = 1.0f / ct;  // one division for tan theta and psi-dot
+    const float tt   = st * sec;
     const float sfst = sf * st, cfst = cf * st;
     out[0]           = fma_(ct * cs, uu, fma_(fma_(sfst, cs, -(cf * ss)), v, fma_(cfst, cs, sf * ss) * w));
     out[1]           = fma_(ct * ss, uu, fma_(fma_(sfst, ss, cf * cs), v, fma_(cfst, ss, -(sf * cs)) * w));
@@ -316,7 +321,7 @@ __device__ __forceinline__ void quad_ode(float* out, const float* x, const float
     const float qr   = fma_(q, sf, r * cf);
     out[3]           = fma_(qr, tt, p);
     out[4]           = fma_(q, cf, -(r * sf));
-    out[5]           = qr / ct;
+    out[5]           = qr * sec;
     const float mX   = k.mi * (k.negnu * __builtin_sqrtf(fma_(uu, uu, fma_(v, v, w * w))));
     const float gct  = k.g * ct;
     out[6]           = fma_(mX, uu, fma_(-k.g, st, fma_(r, v, -(q * w))));

@@ -55,6 +55,7 @@ struct Env {
     int32_t* items  = nullptr;
     float4* lo      = nullptr;
     float4* hi      = nullptr;
+    float4* box     = nullptr;  // per CSR item its obstacle's box: box[2k] min corner, box[2k + 1] max corner
     uint32_t* occ   = nullptr;  // occupancy bitmap of the learned proposal's features
     uint32_t n_obs  = 0;
     uint32_t k      = 0;
@@ -107,7 +108,7 @@ struct kpax_handle {
     uint32_t expand_blocks  = 1;
     uint32_t expand_chunk   = 64;
     bool order_children     = true;     // hand out children longest drawn edge first
-    uint32_t refill_lanes   = 24;       // idle lanes that trigger a wave's service pass
+    uint32_t refill_lanes   = 32;       // idle lanes that trigger a wave's service pass
     uint32_t scan_tile      = 4096;     // items per append/frontier tile: 1024 (256 threads) or 4096 (1024)
     uint32_t* order         = nullptr;  // T positions -> flattened child
     uint32_t* order_hist    = nullptr;  // buckets x order blocks, scanned in place
@@ -185,9 +186,7 @@ inline dev::GridView grid_view(const kpax_handle* h, uint32_t env)
     dev::GridView g;
     g.gp    = h->GPm;
     g.start = h->envs[env].start;
-    g.items = h->envs[env].items;
-    g.lo    = h->envs[env].lo;
-    g.hi    = h->envs[env].hi;
+    g.box   = h->envs[env].box;
     g.occ   = h->envs[env].occ;
     return g;
 }

@@ -123,6 +123,7 @@ GridParams make_grid_params(const kpax_config& c)
         g.D[d]    = c.grid_dims[d];
         g.mn[d]   = c.ws_min[d];
         g.cell[d] = (c.ws_max[d] - c.ws_min[d]) / (float)c.grid_dims[d];
+        g.inv[d]  = (float)c.grid_dims[d] / (c.ws_max[d] - c.ws_min[d]);
     }
     return g;
 }

@@ -94,6 +94,7 @@ struct RegionParams {
 struct GridParams {
     int32_t D[3];
     float mn[3], cell[3];
+    float inv[3];  // cells per metre, D / extent: cell index = (x - mn) * inv
 };
 
 // Planner parameters that are not per-model or per-query.

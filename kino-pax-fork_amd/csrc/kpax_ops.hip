@@ -158,8 +158,8 @@ __device__ __forceinline__ void obstacle_cells(const GridParams& gp, float4 a, f
 #pragma unroll
     for(int d = 0; d < 3; ++d)
     {
-        lo[d] = cell_of(av[d], gp.mn[d], gp.cell[d], gp.D[d]);
-        hi[d] = cell_of(bv[d], gp.mn[d], gp.cell[d], gp.D[d]);
+        lo[d] = cell_of(av[d], gp.mn[d], gp.inv[d], gp.D[d]);
+        hi[d] = cell_of(bv[d], gp.mn[d], gp.inv[d], gp.D[d]);
     }
 }
 
@@ -208,6 +208,16 @@ __global__ void k_grid_sort(const int32_t* start, int32_t* items, uint32_t G)
     }
 }
 
+// The boxes of the sorted CSR items, in item order.
+__global__ void k_grid_boxes(const int32_t* items, const float4* lo, const float4* hi, uint32_t K, float4* box)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k >= K) return;
+    const int32_t o = items[k];
+    box[2 * (size_t)k]     = lo[o];
+    box[2 * (size_t)k + 1] = hi[o];
+}
+
 }  // namespace
 
 namespace kpax {
@@ -216,6 +226,7 @@ void free_env(Env& e)
 {
     if(e.start) (void)hipFree(e.start);
     if(e.items) (void)hipFree(e.items);
+    if(e.box) (void)hipFree(e.box);
     if(e.lo) (void)hipFree(e.lo);
     if(e.hi) (void)hipFree(e.hi);
     if(e.occ) (void)hipFree(e.occ);
@@ -270,6 +281,10 @@ int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n)
         hipLaunchKernelGGL(k_grid_fill, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->GPm, e.lo, e.hi, n,
                            e.start, cursor, e.items);
     hipLaunchKernelGGL(k_grid_sort, dim3(blocks_for(G, 256)), dim3(256), 0, h->stream, e.start, e.items, G);
+    HIPC(hipMalloc(&e.box, 2 * std::max<size_t>(K, 1) * sizeof(float4)));
+    if(K)
+        hipLaunchKernelGGL(k_grid_boxes, dim3(blocks_for(K, 256)), dim3(256), 0, h->stream, e.items, e.lo, e.hi,
+                           (uint32_t)K, e.box);
     HIPC(hipGetLastError());
     {
         const int rc = build_occupancy(h, env);
@@ -455,6 +470,15 @@ int kpax_repeat_expand(kpax_handle* h, const uint32_t* weights, uint32_t n, uint
     const uint32_t k = std::min<uint32_t>(*total, out_cap);
     if(k) HIPC(hipMemcpy(out, dout, (size_t)k * 4, hipMemcpyDeviceToHost));
     return KPAX_OK;
+}
+
+int kpax_scan_compact(kpax_handle* h, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t out_cap,
+                      uint32_t* count)
+{
+    if(!h || !count || (n && !flags) || (out_cap && !out)) return fail(h, KPAX_INVALID_ARG, "null argument");
+    std::vector<uint32_t> w(n);
+    for(uint32_t i = 0; i < n; ++i) w[i] = flags[i] ? 1u : 0u;
+    return kpax_repeat_expand(h, w.data(), n, out, out_cap, count);
 }
 
 }  // extern "C"

@@ -357,9 +357,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
                             qword            = d.query;
                             const GridView& g = A->envs[d.env];
                             G.start          = g.start;
-                            G.items          = g.items;
-                            G.lo             = g.lo;
-                            G.hi             = g.hi;
+                            G.box            = g.box;
                             slot             = s;
                         }
                         c                 = ci;

@@ -625,6 +625,16 @@ class KPAX:
         self._check(lib().kpax_scores(self._h, _p(v), _p(t), _p(c), _p(out)), "scores")
         return out
 
+    def scan_compact(self, flags, cap=None):
+        """Indices of the set flags, increasing (kpax_scan_compact)."""
+        f = np.ascontiguousarray(np.asarray(flags) != 0, np.uint8)
+        cap = int(f.sum()) if cap is None else int(cap)
+        out = np.zeros(max(cap, 1), np.uint32)
+        count = ctypes.c_uint32()
+        self._check(lib().kpax_scan_compact(self._h, _p(f), f.shape[0], _p(out), cap, ctypes.byref(count)),
+                    "scan_compact")
+        return out[: min(cap, count.value)], count.value
+
     def repeat_expand(self, weights, cap=None):
         w = np.ascontiguousarray(weights, np.uint32)
         cap = int(w.sum(dtype=np.uint64)) if cap is None else int(cap)

@@ -140,7 +140,7 @@ inline void sincos_f32(float x, float* s, float* c)
 template<class T> struct Math;
 template<> struct Math<float> {
     static void sincos(float x, float* s, float* c) { sincos_f32(x, s, c); }
-    static float tan_from(float s, float c, float) { return s / c; }
+    static float tan_from(float s, float sec, float) { return s * sec; }
     static float sqrt(float x) { return std::sqrt(x); }
     static float fma(float a, float b, float c) { return std::fma(a, b, c); }
     // sin/cos of a + d from those of a (the kernel's sincos_shift): angle-sum
@@ -376,7 +376,8 @@ void quad_ode(T* out, const T* x, const T* hh, T delta, const T* u, const QuadK<
     const T uu    = fma(delta, hh[6], x[6]), v = fma(delta, hh[7], x[7]), w = fma(delta, hh[8], x[8]);
     const T p     = fma(delta, hh[9], x[9]), q = fma(delta, hh[10], x[10]), r = fma(delta, hh[11], x[11]);
     const T sf = sn[0], cf = cn[0], st = sn[1], ct = cn[1], ss = sn[2], cs = cn[2];
-    const T tt   = Math<T>::tan_from(st, ct, theta);
+    const T sec  = T(1) / ct;  // the kernel's single division (exact in the double path's sense)
+    const T tt   = Math<T>::tan_from(st, sec, theta);
     const T sfst = sf * st, cfst = cf * st;
     out[0]       = fma(ct * cs, uu, fma(fma(sfst, cs, -(cf * ss)), v, fma(cfst, cs, sf * ss) * w));
     out[1]       = fma(ct * ss, uu, fma(fma(sfst, ss, cf * cs), v, fma(cfst, ss, -(sf * cs)) * w));
@@ -384,7 +385,7 @@ void quad_ode(T* out, const T* x, const T* hh, T delta, const T* u, const QuadK<
     const T qr   = fma(q, sf, r * cf);
     out[3]       = fma(qr, tt, p);
     out[4]       = fma(q, cf, -(r * sf));
-    out[5]       = qr / ct;
+    out[5]       = qr * sec;
     const T mX   = k.mi * (k.negnu * Math<T>::sqrt(fma(uu, uu, fma(v, v, w * w))));
     const T gct  = k.g * ct;
     out[6]       = fma(mX, uu, fma(-k.g, st, fma(r, v, -(q * w))));
@@ -817,20 +818,23 @@ void oracle_collide(const float* obs, uint32_t n_obs, const float* bbmin_soa, co
 
 // Uniform grid restated from spatialHash.cu:9-54 (count per cell; the CSR
 // start offsets are the exclusive scan of the counts).  Cells are indexed
-// x fastest (gridToIndex, spatialHash.cuh:57-60).
+// x fastest (gridToIndex, spatialHash.cuh:57-60).  The cell of a coordinate
+// is (x - min) times cells per metre, the kernels' form (the reference
+// divides by the cell size); any monotone map keeps the verdicts equal to
+// brute force.
 void oracle_grid_counts(const kpax_config* cfg, const float* obs, uint32_t n_obs, int32_t* counts)
 {
     const int* D = cfg->grid_dims;
-    float cell[3];
-    for(int d = 0; d < 3; ++d) cell[d] = (cfg->ws_max[d] - cfg->ws_min[d]) / (float)D[d];
+    float inv[3];  // cells per metre, as the kernels' grid
+    for(int d = 0; d < 3; ++d) inv[d] = (float)D[d] / (cfg->ws_max[d] - cfg->ws_min[d]);
     std::fill(counts, counts + (size_t)D[0] * D[1] * D[2], 0);
     for(uint32_t i = 0; i < n_obs; ++i)
     {
         int lo[3], hi[3];
         for(int d = 0; d < 3; ++d)
         {
-            lo[d] = clamp_index((obs[6 * i + d] - cfg->ws_min[d]) / cell[d], D[d]);
-            hi[d] = clamp_index((obs[6 * i + 3 + d] - cfg->ws_min[d]) / cell[d], D[d]);
+            lo[d] = clamp_index((obs[6 * i + d] - cfg->ws_min[d]) * inv[d], D[d]);
+            hi[d] = clamp_index((obs[6 * i + 3 + d] - cfg->ws_min[d]) * inv[d], D[d]);
         }
         for(int z = lo[2]; z <= hi[2]; ++z)
             for(int y = lo[1]; y <= hi[1]; ++y)

@@ -155,6 +155,20 @@ def test_repeat_expand_zeros_and_cap(kpax):
         assert total == 15000 and np.array_equal(out, np.repeat(np.arange(34, dtype=np.uint32), 3)[:100])
 
 
+@pytest.mark.parametrize("n", [1, 1024, 1025, 70000])
+def test_scan_compact(kpax, rng, n):
+    """Frontier compaction: the indices of the set flags in increasing order,
+    the last element included when set (the reference's next-frontier
+    compaction drops it, KPAX.cu:523-525); all-clear and all-set masks."""
+    with kpax.KPAX(kpax.config_workload(1)) as p:
+        for flags in (rng.uniform(0, 1, n) < 0.4, np.zeros(n, bool), np.ones(n, bool)):
+            flags = flags.copy()
+            flags[-1] = True if flags.any() else flags[-1]
+            out, count = p.scan_compact(flags)
+            want = np.flatnonzero(flags).astype(np.uint32)
+            assert count == want.size and np.array_equal(out, want)
+
+
 def _compare_plans(kpax, oracle, k, seed, **kw):
     cfg, obs, start, goal = workload(k, seed=seed, **kw)
     with kpax.KPAX(cfg) as p:
