@@ -5,22 +5,25 @@
 // admitted query owns a slot and a contiguous node range [base, base + cap)
 // of the tree, children and repeat-list arrays (SoA, column d of node i at
 // d * T + i), plus its own region counters.  One planner iteration advances
-// every running query with five launches on the handle's stream and no host
-// round trip between them (the reference makes at least eight blocking
-// device->host transfers per iteration of its single query, KPAX.cu:303-543):
+// every running query with a handful of launches on the handle's stream and
+// no host round trip between them (the reference makes at least eight
+// blocking device->host transfers per iteration of its single query,
+// KPAX.cu:303-543):
 //   k_advance   1 block   publish each slot's iteration record; scan the
 //                         slots' child / tile counts into offsets
+//   k_order_*   2 passes  children sorted longest drawn edge first (optional)
 //   k_expand    hot       one lane per child: Philox controls, RK4 with checks,
 //                         region counters, admission (propagateFrontier_kernel1/2)
 //   k_append    scan      admitted children -> tree in slot order, goal test
 //                         (updateFrontier_kernel, first half)
+//   k_propose   (learned proposal only) the network on the appended nodes
 //   k_score     1 block per slot: region scores (Graph::updateVertices)
 //   k_frontier  scan      re-admission + repeat counts -> next repeat list
 //                         (updateFrontier_kernel second half + findInd/repeatInd)
 // Work of all slots is flattened: children, append tiles and frontier tiles
 // are numbered across slots by the advance kernel's offsets, and each item
-// finds its slot by binary search.  With profiling off the five launches and
-// the status copy are replayed as one captured HIP graph.
+// finds its slot by binary search.  With profiling off the launches and the
+// status copy are replayed as one captured HIP graph.
 
 #include <hip/hip_runtime.h>
 
