@@ -18,8 +18,9 @@ Also measured, each on the same GPU and stated beside the headline:
            queries planned one at a time; every query's result is identical
            in both forms, so the success rates match by construction
   proposal the learned control proposal (50/50 mixture with uniform) against
-           the plain sampler on held-out forests and queries, at 2^18-node
-           budgets (where both solve every query) and at 2^16
+           the plain sampler on held-out forests and queries, at 2^18-, 2^17-
+           and 2^16-node budgets, then at budgets log-uniform in [2^15, 2^19]
+           with and without the budget policy (uniform sampling below 2^17)
 Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-proposal] [--batched-only] [--proposal-only]
@@ -142,8 +143,9 @@ def learned_proposal(cfg):
     strip = lambda d: {k: v for k, v in d.items() if k != "results"}  # noqa: E731
     out = {"workload": "96 held-out queries (start/goal >= 4 m apart) over 8 held-out forests of 100-600 columns, "
                        "max_iter 100, single-query form; parameters trained by train_proposal.py on 10,000 "
-                       "plain-planner runs in other forests"}
-    for budget in (1 << 18, 1 << 16):
+                       "plain-planner runs in other forests; per budget, then budgets log-uniform in "
+                       "[2^15, 2^19] with and without the budget policy (uniform sampling below 2^17)"}
+    for budget in (1 << 18, 1 << 17, 1 << 16):
         c = cfg.copy()
         c.max_tree = budget
         c.max_iter = 100
@@ -163,6 +165,20 @@ def learned_proposal(cfg):
                 "mean_iterations_learned": float(np.mean([b.iterations for _, b in both])) if both else None,
                 "mean_expansions_plain": float(np.mean([a.expansions for a, _ in both])) if both else None,
                 "mean_expansions_learned": float(np.mean([b.expansions for _, b in both])) if both else None}}
+    # budgets log-uniform in [2^15, 2^19]: the mixture for every query, and
+    # with the budget policy (uniform sampling below 2^17)
+    c = cfg.copy()
+    c.max_tree = 1 << 19
+    c.max_iter = 100
+    envs = proposal.forest_environments(c, 8, seed=2)
+    qs = proposal.serving_queries(c, envs, 96, seed=5, budget_range=(1 << 15, 1 << 19))
+    with kpax.KPAX(c) as p:
+        for e, o in enumerate(envs):
+            p.set_environment(e, o)
+        out["mixed_budgets_32k_512k"] = {
+            "plain": strip(proposal.evaluate(p, qs, None)),
+            "learned_mix_0.5": strip(proposal.evaluate(p, qs, params, 0.5)),
+            "learned_mix_0.5_from_128k": strip(proposal.evaluate(p, qs, params, 0.5, min_budget=1 << 17))}
     return out
 
 

@@ -294,6 +294,14 @@ int kpax_mlp_init_params(uint64_t seed, float* params);
  * whose results are then unchanged bit for bit. */
 int kpax_set_proposal(kpax_handle* h, const float* params, float mix);
 
+/* Budget policy of the proposal: queries whose tree budget (kpax_query.budget,
+ * or max_tree for kpax_plan) is below min_budget sample uniformly, exactly as
+ * the plain planner; the others use the mixture.  0 (the default) applies it
+ * to every query.  Takes effect for queries admitted after the call.  (With
+ * the forest-trained network, held-out queries at 2^16-node budgets lose
+ * success under any mixture weight, while at 2^17 and above the mixture wins.) */
+int kpax_set_proposal_min_budget(kpax_handle* h, uint32_t min_budget);
+
 /* Features of n states (S x n SoA) with their goals (3 x n) in environment
  * `env`: 3 scaled positions, 3 angles / pi, 3 body velocities / vel bound,
  * 3 rates / 4, 3 goal-relative positions, 64 occupancy bits of the 4x4x4

@@ -140,6 +140,7 @@ struct kpax_handle {
     void* mlp               = nullptr;  // bf16 network layouts
     float* prop             = nullptr;  // per node: 4 means then 4 sigmas, column stride T
     float mix               = 0.0f;
+    uint32_t prop_min_budget = 0;       // queries with a smaller tree budget sample uniformly
     // environments
     std::vector<kpax::Env> envs;
     // last single-query plan (for kpax_get_tree / kpax_get_regions)

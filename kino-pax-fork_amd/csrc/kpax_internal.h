@@ -53,7 +53,8 @@ struct QueryDev {
     uint32_t query;  // Philox query word
     uint64_t seed;
     float goal[4];   // goal ball centre (xyz)
-    uint32_t pad[2];
+    float mix;       // probability of the learned proposal for this query (0: uniform controls only)
+    uint32_t pad;
 };
 
 // Shared record of one iteration over all query slots (written by the

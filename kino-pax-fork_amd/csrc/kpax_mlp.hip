@@ -328,10 +328,11 @@ __global__ __launch_bounds__(256) void k_propose(const MlpDev* Wd, ProposeArgs A
         {
             if(A.n_init)
             {
-                nv    = 1;
-                node0 = A.qd[A.init_slots[k]].base;
+                const QueryDev& d = A.qd[A.init_slots[k]];
+                nv                = d.mix > 0.0f ? 1u : 0u;  // queries under the budget policy sample uniformly
+                node0             = d.base;
             }
-            else if(A.ctl[k].cur.status == kRunning)
+            else if(A.ctl[k].cur.status == kRunning && A.qd[k].mix > 0.0f)
             {
                 nv    = A.ctl[k].Fn;
                 node0 = A.qd[k].base + A.ctl[k].cur.ts;
@@ -832,6 +833,13 @@ int batch_gradients(kpax_trainer* t, uint32_t B, float* loss_part)
 }  // namespace
 
 extern "C" {
+
+int kpax_set_proposal_min_budget(kpax_handle* h, uint32_t min_budget)
+{
+    if(!h) return KPAX_INVALID_ARG;
+    h->prop_min_budget = min_budget;
+    return KPAX_OK;
+}
 
 int kpax_set_proposal(kpax_handle* h, const float* params, float mix)
 {

@@ -144,7 +144,6 @@ struct ExpandArgs {
     uint32_t chunk;  // children per work ticket (multiple of 64)
     const uint32_t* order;  // processing order of the children (null: index order)
     uint32_t refill;        // idle lanes that trigger a wave's service pass (1..64)
-    float mix;       // probability of the learned proposal (0: uniform controls only)
     const float* prop;
     SlotUniform s0;
 };
@@ -373,7 +372,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
                         const uint32_t dcw[4] = {dc.x, dc.y, dc.z, dc.w};
 #pragma unroll
                         for(int j = 0; j < C; ++j) u[j] = P.ctrl_min[j] + u01(dcw[j]) * P.ctrl_rng[j];
-                        if(A->mix > 0.0f && u01(ds.z) < A->mix)
+                        // this query's weight of the learned proposal (0: uniform only)
+                        const float mixq = MULTI ? A->qd[slot].mix : A->s0.qd.mix;
+                        if(mixq > 0.0f && u01(ds.z) < mixq)
                         {
                             // the parent's Gaussian instead (learned proposal)
                             const uint4 dp = philox(c, t, qword, PURPOSE_PROPOSAL, k0, k1);
@@ -908,7 +909,6 @@ int upload_expand_args(kpax_handle* h)
     a.chunk     = h->expand_chunk;
     a.order     = h->order_children ? h->order : nullptr;
     a.refill    = h->refill_lanes;
-    a.mix       = h->mix;
     a.prop      = h->prop;
     memset(&a.s0, 0, sizeof(a.s0));
     a.s0.cur.status = kNotStarted;
@@ -1155,6 +1155,7 @@ int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
             d.env   = qs[qi].env;
             d.query = 0;  // the same draws as a plan of this query alone
             d.seed  = qs[qi].seed;
+            d.mix   = d.cap >= h->prop_min_budget ? h->mix : 0.0f;  // proposal policy by tree budget
             for(int j = 0; j < 3; ++j) d.goal[j] = qs[qi].goal[j];
             h->h_init_slots[n_init] = s;
             for(int j = 0; j < 16; ++j) h->h_init_starts[16 * (size_t)n_init + j] = j < S ? qs[qi].start[j] : 0.0f;

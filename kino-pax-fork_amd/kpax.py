@@ -478,11 +478,13 @@ class KPAX:
         return out
 
     # ---- learned control proposal ---------------------------------------------
-    def set_proposal(self, params, mix=0.5):
+    def set_proposal(self, params, mix=0.5, min_budget=0):
         """Draw each child's control from its parent's learned Gaussian with
-        probability `mix` (params None or mix 0: the plain uniform sampler)."""
+        probability `mix` (params None or mix 0: the plain uniform sampler);
+        queries with a tree budget below `min_budget` sample uniformly."""
         p = None if params is None else _f32(params, (MLP_PARAMS,))
         self._check(lib().kpax_set_proposal(self._h, _p(p), float(mix if p is not None else 0.0)), "set_proposal")
+        self._check(lib().kpax_set_proposal_min_budget(self._h, int(min_budget)), "set_proposal_min_budget")
 
     def mlp_features(self, env, states, goals):
         """Features (n, 80) of states (S, n) with goals (3, n) in environment `env`."""

@@ -112,12 +112,13 @@ def train(X, Y, epochs=20, batch=4096, lr=2e-3, seed=0, init_seed=1):
         return t.params(), losses
 
 
-def evaluate(planner, queries, params=None, mix=0.5, phases=True):
+def evaluate(planner, queries, params=None, mix=0.5, phases=True, min_budget=0):
     """Plan `queries` one at a time (the single-query form) with the plain
-    sampler (params None) or the mixture; success = goal reached within the
-    budget and the FP64 re-check passed.  With `phases`, a second pass over
-    the same queries with HIP events on gives the device time per phase."""
-    planner.set_proposal(params, mix if params is not None else 0.0)
+    sampler (params None) or the mixture (off for queries whose budget is
+    below `min_budget`); success = goal reached within the budget and the FP64
+    re-check passed.  With `phases`, a second pass over the same queries with
+    HIP events on gives the device time per phase."""
+    planner.set_proposal(params, mix if params is not None else 0.0, min_budget=min_budget)
     res = planner.plan_batch(queries[:1])  # warm-up
     t0 = time.perf_counter()
     res = [planner.plan_batch([q])[0] for q in queries]

@@ -147,3 +147,28 @@ def test_batch_with_proposal_matches_single_plans(kpax):
         assert (r.status, r.tree_size, r.expansions, r.iterations) == (o.status, o.tree_size, o.expansions,
                                                                        o.iterations)
         assert np.array_equal(r.rows.view(np.uint32), o.rows.view(np.uint32))
+
+
+def test_proposal_budget_policy(kpax):
+    """Queries under the proposal's minimum budget sample uniformly (the plain
+    planner's results bit for bit); the others use the mixture, in the same
+    batch."""
+    cfg, obs, start, goal = workload(2, seed=0, max_iter=40)
+    params = kpax.mlp_init_params(9)
+    budgets = [30000, 80000, 45000, 95000]
+    qs = [kpax.Query(start, goal, seed=90 + i, budget=b) for i, b in enumerate(budgets)]
+    with kpax.KPAX(cfg, max_queries=2, node_capacity=200000) as p:
+        p.set_obstacles(obs)
+        p.set_proposal(params, 0.5, min_budget=60000)
+        res = p.plan_batch(qs)
+    for q, r in zip(qs, res):
+        c = cfg.copy()
+        c.max_tree = q.budget
+        with kpax.KPAX(c) as s:
+            s.set_obstacles(obs)
+            if q.budget >= 60000:
+                s.set_proposal(params, 0.5)
+            o = s.plan(q.start, q.goal, seed=q.seed)
+        assert (r.status, r.tree_size, r.expansions, r.iterations) == (o.status, o.tree_size, o.expansions,
+                                                                       o.iterations), q.budget
+        assert np.array_equal(r.rows.view(np.uint32), o.rows.view(np.uint32))
