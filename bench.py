@@ -69,6 +69,21 @@ MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 186 + 95
 FLOP_PER_STEP = {kpax.MODEL_DOUBLE_INTEGRATOR: 60, kpax.MODEL_DUBINS_AIRPLANE: 67, kpax.MODEL_QUADCOPTER: 624}
 
 
+MODEL_NAME = {kpax.MODEL_DOUBLE_INTEGRATOR: ("6D double integrator", "double_integrator"),
+              kpax.MODEL_DUBINS_AIRPLANE: ("6D Dubins airplane", "dubins"),
+              kpax.MODEL_QUADCOPTER: ("12D quadcopter", "quadcopter")}
+ENV_NAME = {1: "random boxes", 2: "column forest", 3: "box maze"}
+
+
+def describe(k, cfg, n_obs):
+    """The workload of `bench.py --workload k` in words, from its config."""
+    ext = lambda a, b: f"[{a:g},{b:g}]"  # noqa: E731
+    ws = "x".join(ext(a, b) for a, b in zip(cfg.ws_min, cfg.ws_max))
+    l1, l2, g = cfg.r1_len[0], cfg.r2_len[0], cfg.grid_dims[0]
+    return (f"{MODEL_NAME[cfg.model][0]}, {n_obs}-obstacle {ENV_NAME[k]}, {ws}, RK4 dt={cfg.dt:g}, "
+            f"2^{int(cfg.max_tree).bit_length() - 1}-node tree cap, {l1}^3 regions x {l2}^3 sub-regions, {g}^3 grid")
+
+
 def cpu_baseline(k, env_seed, plans, seeds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # checker only: times the CPU restatement of the same planner
@@ -114,6 +129,22 @@ def batched(cfg):
         wall = time.perf_counter() - t0
         st = p.stats()
         launch = p.launch()
+        # the same batch with the learned proposal (queries from 2^17 nodes)
+        prop = None
+        if os.path.exists(PARAMS):
+            p.set_proposal(np.load(PARAMS), 0.5, min_budget=1 << 17)
+            p.plan_batch(qs[:4])
+            t0 = time.perf_counter()
+            res_p = p.plan_batch(qs)
+            wall_p = time.perf_counter() - t0
+            p.set_proposal(None)
+            ok_p = [r for r in res_p if r.status == kpax.OK and r.verified == 1]
+            lat_p = [r.plan_ms for r in res_p]
+            prop = {"plans_per_sec": len(qs) / wall_p, "wall_s": wall_p, "success_rate": len(ok_p) / len(qs),
+                    "p50_latency_ms": statistics.median(lat_p), "p99_latency_ms": float(np.percentile(lat_p, 99)),
+                    "queries_with_proposal": sum(q.budget >= (1 << 17) for q in qs),
+                    "note": "mixture 0.5 for queries with budgets >= 2^17 (forest-trained network; half of the "
+                            "environments are mazes)"}
     with kpax.KPAX(cfg, max_queries=1, node_capacity=1 << 20) as s:
         for e, o in enumerate(envs):
             s.set_environment(e, o)
@@ -134,7 +165,8 @@ def batched(cfg):
                                   "p50_time_to_solution_ms": statistics.median(r.plan_ms for r in seq),
                                   "success_rate": sum(r.status == kpax.OK and r.verified == 1 for r in seq) / len(seq)},
             "batched_over_single_plans_per_sec": (len(qs) / wall) / (len(qs) / wall_seq),
-            "results_identical_to_single_query_form": same}
+            "results_identical_to_single_query_form": same,
+            "with_learned_proposal": prop}
 
 
 def learned_proposal(cfg):
@@ -274,8 +306,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "12D quadcopter, 400-column forest, [0,10]^2x[0,5], RK4 dt=0.02, "
-                               "2^20-node tree cap, 16^3 regions x 4^3 sub-regions, 64^3 grid",
+        "config": {"workload": describe(args.workload, cfg, obs.shape[0]),
                    "plans_per_step": 1, "env_seed": args.env_seed, "max_tree": int(cfg.max_tree)},
         "p50_time_to_first_solution_ms": p50,
         "p99_time_to_first_solution_ms": p99,
@@ -288,10 +319,11 @@ def main():
         "phase_ms_per_plan": {k: v / max(1, st["plans"]) for k, v in st["phase_ms"].items()},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS,
-                     "traffic": per_launch_exp * MEASURED_EXPAND_TRAFFIC_PER_EXPANSION,
+                     "traffic": (per_launch_exp * MEASURED_EXPAND_TRAFFIC_PER_EXPANSION
+                                 if args.workload == 2 else None),
                      "traffic_basis": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE bytes per expansion x expansions "
-                                      "per launch",
-                     "kernel": "k_expand<quadcopter>", "bytes_per_expansion": bytes_per_expansion(S, C),
+                                      "per launch (measured on workload 2)",
+                     "kernel": f"k_expand<{MODEL_NAME[cfg.model][1]}>", "bytes_per_expansion": bytes_per_expansion(S, C),
                      "expansions_per_launch": per_launch_exp, "avg_launch_ms": per_launch_ms},
         "roofline_valu": {"bound": "valu_fp32", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS,
                           "unit": "TFLOP/s", "frac": achieved_tflops / FP32_PEAK_TFLOPS,

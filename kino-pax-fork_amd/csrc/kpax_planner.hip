@@ -261,6 +261,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
     constexpr uint32_t kNone = 0xffffffffu;
     const uint32_t total = MULTI ? A->bc->children : (A->s0.cur.status == kRunning ? A->s0.cur.M : 0u);
     if(total == 0) return;
+    // A wave whose first ticket would already lie past the end leaves without
+    // taking it: in a small iteration thousands of persistent waves would
+    // otherwise queue on the one ticket counter before the kernel could end.
+    // The waves that stay still take every chunk.
+    const uint32_t wave_id = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if((uint64_t)wave_id * A->chunk >= total) return;
     const int R1 = RP.r1_count;
     // slot-uniform in the single-slot form, per lane in the batched form
     uint32_t t = 0, fan = 1, k0 = 0, k1 = 0, qword = 0, slot = 0, base = 0;
@@ -588,6 +594,7 @@ __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, Bat
     __shared__ uint32_t s_tile, s_pre;
     HIP_DYNAMIC_SHARED(int32_t, s_hist)
     const uint32_t ntiles = bc->atiles, epoch = bc->epoch;
+    if(blockIdx.x >= ntiles) return;  // no ticket to take: leave the counter to the blocks with work
     const bool local      = R1 <= (uint32_t)kLdsRegions;
     for(;;)
     {
@@ -736,6 +743,7 @@ __global__ __launch_bounds__(NT) void k_frontier(Ctl* ctl, const QueryDev* qd, B
     __shared__ uint32_t lds[NT / 64];
     __shared__ uint32_t s_tile, s_pre;
     const uint32_t ntiles = bc->ftiles, epoch = bc->epoch;
+    if(blockIdx.x >= ntiles) return;  // no ticket to take: leave the counter to the blocks with work
     for(;;)
     {
         const uint32_t tile = next_tile(&bc->tile_frontier, &s_tile);
@@ -1001,12 +1009,6 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
 void capture_graphs(kpax_handle* h)
 {
     h->graph_tried = true;
-    const char* env = std::getenv("KPAX_GRAPH");
-    if(env && env[0] == '0')
-    {
-        h->use_graph = false;
-        return;
-    }
     for(int p = 0; p < 2; ++p)
     {
         hipGraph_t g = nullptr;
@@ -1504,10 +1506,6 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->ftile_off, Q + 1));
     TRY(dalloc(h, &h->order, T));
     TRY(dalloc(h, &h->order_hist, (size_t)kOrderBlocks * kOrderBuckets));
-    {
-        const char* env       = std::getenv("KPAX_ORDER");
-        h->order_children = !(env && env[0] == '0');
-    }
     TRY(dalloc(h, &h->slot_status, Q + 1));
     h->n_status = (uint32_t)(T / kTile + Q + 2);
     TRY(dalloc(h, &h->st_append, h->n_status));
