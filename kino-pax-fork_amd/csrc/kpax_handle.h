@@ -62,6 +62,7 @@ struct Env {
     std::vector<float> host;
 };
 
+constexpr int kTicketGroups = 64, kTicketStride = 32;  // expansion chunk counters, 128 B apart
 constexpr int kTile       = 1024;  // items per scan tile (256 threads x 4); the planner's smallest tile
 constexpr int kLdsRegions = 4096;  // R1 counts up to this size are summed in LDS
 constexpr int kMaxEnvs    = 64;    // environments of one handle (LDS table in batched expansion)
@@ -108,6 +109,7 @@ struct kpax_handle {
     uint32_t expand_blocks  = 1;
     uint32_t expand_chunk   = 64;
     bool order_children     = true;     // hand out children longest drawn edge first
+    uint32_t* tickets       = nullptr;  // kTicketGroups expansion chunk counters
     uint32_t refill_lanes   = 32;       // idle lanes that trigger a wave's service pass
     uint32_t scan_tile      = 4096;     // items per append/frontier tile: 1024 (256 threads) or 4096 (1024)
     uint32_t* order         = nullptr;  // T positions -> flattened child

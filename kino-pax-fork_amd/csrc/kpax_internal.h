@@ -63,7 +63,7 @@ struct BatchCtl {
     uint32_t children;        // children of all running slots
     uint32_t atiles, ftiles;  // append and frontier scan tiles (frontier: an upper bound)
     uint32_t running;         // slots running in this iteration
-    uint32_t tile_expand, tile_append, tile_frontier;  // work tickets
+    uint32_t tile_unused, tile_append, tile_frontier;  // work tickets of the scans
     uint32_t epoch;           // tag of this iteration's scan status words
     uint32_t pad[4];
 };

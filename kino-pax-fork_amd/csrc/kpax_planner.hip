@@ -143,6 +143,7 @@ struct ExpandArgs {
     uint32_t T, Q, R2;
     uint32_t chunk;  // children per work ticket (multiple of 64)
     const uint32_t* order;  // processing order of the children (null: index order)
+    uint32_t* tickets;      // kTicketGroups chunk counters, kTicketStride words apart
     uint32_t refill;        // idle lanes that trigger a wave's service pass (1..64)
     const float* prop;
     SlotUniform s0;
@@ -158,8 +159,9 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
                                                   uint32_t* atile_off, uint32_t* ftile_off, int32_t* slot_status,
                                                   uint32_t Q, uint64_t* st_append, uint64_t* st_frontier,
                                                   uint32_t n_status, PlanParams PP, const GridView* envs,
-                                                  SlotUniform* s0, uint32_t tile)
+                                                  SlotUniform* s0, uint32_t tile, uint32_t* tickets)
 {
+    if(threadIdx.x < kTicketGroups) tickets[threadIdx.x * kTicketStride] = 0;
     __shared__ uint32_t lds[16];
     __shared__ uint32_t s_epoch;
     const uint32_t q = threadIdx.x;
@@ -206,7 +208,6 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
         bc->atiles       = ta;
         bc->ftiles       = tf;
         bc->running      = tr;
-        bc->tile_expand  = 0;
         bc->tile_append  = 0;
         bc->tile_frontier = 0;
         uint32_t e       = bc->epoch + 1;
@@ -261,12 +262,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
     constexpr uint32_t kNone = 0xffffffffu;
     const uint32_t total = MULTI ? A->bc->children : (A->s0.cur.status == kRunning ? A->s0.cur.M : 0u);
     if(total == 0) return;
-    // A wave whose first ticket would already lie past the end leaves without
-    // taking it: in a small iteration thousands of persistent waves would
-    // otherwise queue on the one ticket counter before the kernel could end.
-    // The waves that stay still take every chunk.
-    const uint32_t wave_id = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if((uint64_t)wave_id * A->chunk >= total) return;
+    // Chunks of children are handed out by kTicketGroups counters, each on
+    // its own cache line (one device-scope counter for all waves serialised
+    // the launch): the blocks of group g (block index mod the group count, so
+    // one XCD of the round-robin dispatch feeds each counter) take chunks
+    // g, g + groups, g + 2 groups, ...  A wave whose first chunk would
+    // already lie past the end leaves without taking a ticket: in a small
+    // iteration thousands of persistent waves would otherwise queue on the
+    // counters before the kernel could end.  The waves that stay still take
+    // every chunk of their group, and every group has blocks.
+    const uint32_t groups  = gridDim.x < (uint32_t)kTicketGroups ? gridDim.x : (uint32_t)kTicketGroups;
+    const uint32_t group   = blockIdx.x % groups;
+    const uint32_t rank    = (blockIdx.x / groups) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if(((uint64_t)rank * groups + group) * A->chunk >= total) return;
+    uint32_t* const ticket = A->tickets + group * kTicketStride;
     const int R1 = RP.r1_count;
     // slot-uniform in the single-slot form, per lane in the batched form
     uint32_t t = 0, fan = 1, k0 = 0, k1 = 0, qword = 0, slot = 0, base = 0;
@@ -336,7 +345,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
                     {
                         const uint32_t chunk = A->chunk;
                         uint32_t b           = 0;
-                        if(lane == 0) b = atomicAdd(&A->bc->tile_expand, chunk);
+                        if(lane == 0)
+                        {
+                            const uint64_t c = ((uint64_t)atomicAdd(ticket, 1u) * groups + group) * chunk;
+                            b                = c < total ? (uint32_t)c : total;
+                        }
                         b = __shfl(b, 0, 64);
                         if(b >= total)
                         {
@@ -916,6 +929,7 @@ int upload_expand_args(kpax_handle* h)
     a.R2        = h->R2;
     a.chunk     = h->expand_chunk;
     a.order     = h->order_children ? h->order : nullptr;
+    a.tickets   = h->tickets;
     a.refill    = h->refill_lanes;
     a.prop      = h->prop;
     memset(&a.s0, 0, sizeof(a.s0));
@@ -970,7 +984,7 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
     const uint32_t Q = h->Qmax, R1 = (uint32_t)h->RP.r1_count;
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc, h->child_off, h->atile_off,
                        h->ftile_off, h->slot_status, Q, h->st_append, h->st_frontier, h->n_status, h->PP, h->envs_dev,
-                       &((ExpandArgs*)h->expand_args)->s0, h->scan_tile);
+                       &((ExpandArgs*)h->expand_args)->s0, h->scan_tile, h->tickets);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
     launch_expand(h);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][1], h->stream));
@@ -1501,6 +1515,8 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->ctl, Q));
     TRY(dalloc(h, &h->qd, Q));
     TRY(dalloc(h, &h->bc, 1));
+    TRY(dalloc(h, &h->tickets, (size_t)kTicketGroups * kTicketStride));
+    if(hipMemset(h->tickets, 0, (size_t)kTicketGroups * kTicketStride * 4) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
     TRY(dalloc(h, &h->child_off, Q + 1));
     TRY(dalloc(h, &h->atile_off, Q + 1));
     TRY(dalloc(h, &h->ftile_off, Q + 1));
