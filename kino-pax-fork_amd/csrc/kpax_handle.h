@@ -32,6 +32,7 @@ struct TreeView {
 // parent, R1 region, R2 cell}.  The append kernel reads every child's meta
 // word and the record only of the children it admits.
 constexpr int kChildWords = 16;
+constexpr uint32_t kPrepSlotShift = 17;  // prepared child record word 2: steps (< 2^17) | slot << 17
 constexpr uint32_t kMetaFlagShift = 24;  // integrated steps < 2^24 (an edge has at most 100000)
 struct ChildView {
     uint4* rec;   // kChildWords / 4 uint4 per child
@@ -110,9 +111,9 @@ struct kpax_handle {
     uint32_t expand_chunk   = 64;
     bool order_children     = true;     // hand out children longest drawn edge first
     uint32_t* tickets       = nullptr;  // kTicketGroups expansion chunk counters
-    uint32_t refill_lanes   = 32;       // idle lanes that trigger a wave's service pass
+    uint32_t refill_lanes   = 24;       // idle lanes that trigger a wave's service pass
     uint32_t scan_tile      = 4096;     // items per append/frontier tile: 1024 (256 threads) or 4096 (1024)
-    uint32_t* order         = nullptr;  // T positions -> flattened child
+    uint4* prep             = nullptr;  // per processing position: the child's prepared draws (2 x uint4)
     uint32_t* order_hist    = nullptr;  // buckets x order blocks, scanned in place
     uint32_t scan_blocks    = 1;
     uint32_t path_cap       = 0;        // rows per slot

@@ -142,7 +142,7 @@ struct ExpandArgs {
     GridParams GP;
     uint32_t T, Q, R2;
     uint32_t chunk;  // children per work ticket (multiple of 64)
-    const uint32_t* order;  // processing order of the children (null: index order)
+    uint4* prep;            // per processing position: {child, parent, steps | slot << 17, admit draw}, {controls}
     uint32_t* tickets;      // kTicketGroups chunk counters, kTicketStride words apart
     uint32_t refill;        // idle lanes that trigger a wave's service pass (1..64)
     const float* prop;
@@ -278,17 +278,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
     uint32_t* const ticket = A->tickets + group * kTicketStride;
     const int R1 = RP.r1_count;
     // slot-uniform in the single-slot form, per lane in the batched form
-    uint32_t t = 0, fan = 1, k0 = 0, k1 = 0, qword = 0, slot = 0, base = 0;
+    uint32_t t = 0, slot = 0, base = 0;
     GridView G;
     if constexpr(!MULTI)
     {
-        t     = A->s0.cur.t;
-        fan   = A->s0.cur.fan;
-        base  = A->s0.qd.base;
-        k0    = (uint32_t)A->s0.qd.seed;
-        k1    = (uint32_t)(A->s0.qd.seed >> 32);
-        qword = A->s0.qd.query;
-        G     = A->s0.G;
+        t    = A->s0.cur.t;
+        base = A->s0.qd.base;
+        G    = A->s0.G;
     }
     else
         G.gp = A->GP;  // grid parameters are shared; pointers are set per child
@@ -363,52 +359,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
                     const uint32_t rank  = (uint32_t)__popcll(need & below);
                     if(c == kNone && rank < avail)
                     {
-                        const uint32_t k = A->order ? A->order[q_next + rank] : q_next + rank;
-                        uint32_t ci      = k;
+                        // the child's draws and parent, prepared by k_prepare
+                        const uint4* pr = A->prep + 2 * (size_t)(q_next + rank);
+                        const uint4 r0 = pr[0], r1 = pr[1];
+                        uint32_t ci    = r0.x;
                         if constexpr(MULTI)
                         {
-                            const uint32_t s = segment_of(A->child_off, A->Q, k);
-                            ci               = k - A->child_off[s];
+                            const uint32_t s = r0.z >> kPrepSlotShift;
+                            ci               = r0.x - A->child_off[s];
                             t                = A->ctl[s].cur.t;
-                            fan              = A->ctl[s].cur.fan;
                             const QueryDev& d = A->qd[s];
                             base             = d.base;
-                            k0               = (uint32_t)d.seed;
-                            k1               = (uint32_t)(d.seed >> 32);
-                            qword            = d.query;
                             const GridView& g = A->envs[d.env];
                             G.start          = g.start;
                             G.box            = g.box;
                             slot             = s;
                         }
                         c                 = ci;
-                        parent            = R[base + c / fan];
+                        parent            = r0.y;
                         const uint32_t pn = base + parent;
 #pragma unroll
                         for(int d = 0; d < S; ++d) x[d] = tree.st[(size_t)d * T + pn];
-                        const uint4 dc        = philox(c, t, qword, PURPOSE_CONTROL, k0, k1);
-                        const uint4 ds        = philox(c, t, qword, PURPOSE_STEPS_ADMIT, k0, k1);
-                        const uint32_t dcw[4] = {dc.x, dc.y, dc.z, dc.w};
+                        const uint32_t uw[4] = {r1.x, r1.y, r1.z, r1.w};
 #pragma unroll
-                        for(int j = 0; j < C; ++j) u[j] = P.ctrl_min[j] + u01(dcw[j]) * P.ctrl_rng[j];
-                        // this query's weight of the learned proposal (0: uniform only)
-                        const float mixq = MULTI ? A->qd[slot].mix : A->s0.qd.mix;
-                        if(mixq > 0.0f && u01(ds.z) < mixq)
-                        {
-                            // the parent's Gaussian instead (learned proposal)
-                            const uint4 dp = philox(c, t, qword, PURPOSE_PROPOSAL, k0, k1);
-                            float mean[C], sg[C];
-#pragma unroll
-                            for(int j = 0; j < C; ++j)
-                            {
-                                mean[j] = A->prop[(size_t)j * T + pn];
-                                sg[j]   = A->prop[(size_t)(4 + j) * T + pn];
-                            }
-                            propose_control<C>(mean, sg, dp, P, u);
-                        }
-                        n          = draw_steps(P, ds.x);
+                        for(int j = 0; j < C; ++j) u[j] = __builtin_bit_cast(float, uw[j]);
+                        n          = (int)(r0.z & ((1u << kPrepSlotShift) - 1u));
                         i          = 0;
-                        admit_draw = ds.y;
+                        admit_draw = r0.w;
                         live       = true;
                     }
                     const uint32_t wanted = (uint32_t)__popcll(need);
@@ -440,7 +417,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
 // scatter, each of whose blocks sums the counts it needs.
 // ---------------------------------------------------------------------------
 constexpr int kOrderBlocks = 256, kOrderBuckets = 32;
-static_assert(256 % kOrderBuckets == 0, "k_order_scatter splits its 256 threads evenly over the buckets");
+static_assert(256 % kOrderBuckets == 0, "k_prepare splits its 256 threads evenly over the buckets");
 
 __device__ __forceinline__ uint32_t order_bucket(const ModelParams& P, int n)
 {
@@ -495,47 +472,113 @@ __global__ __launch_bounds__(256) void k_order_count(const ExpandArgs* __restric
     if(threadIdx.x < kOrderBuckets) hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-// Each block finds its own first position per bucket from the raw counts:
-// the children of all earlier buckets plus those of this bucket counted by
-// earlier blocks.  Thread (bucket b, part j) sums an eighth of b's block
-// counts; one thread per bucket then adds up the parts.
+// Preparation of the iteration's children, in a coalesced pass before the
+// expansion: each child's parent (repeat list), control draw (uniform, or the
+// parent's Gaussian with the query's proposal weight), step count and
+// admission draw, stored as a 32-byte record at its processing position, so
+// the expansion's refill is two vector loads and the parent state.  With
+// `hist` (the per-block bucket counts of k_order_count) the position follows
+// the longest-first order; without it, the child index.  Each ordering block
+// finds its own first position per bucket from the raw counts: the children
+// of all earlier buckets plus those of this bucket counted by earlier blocks.
+// Thread (bucket b, part j) sums an eighth of b's block counts; one thread
+// per bucket then adds up the parts.
 constexpr int kOrderParts = 256 / kOrderBuckets;
 
-template<bool MULTI>
-__global__ __launch_bounds__(256) void k_order_scatter(const ExpandArgs* __restrict__ A, const uint32_t* hist,
-                                                       uint32_t* order)
+template<int MODEL, bool MULTI>
+__global__ __launch_bounds__(256) void k_prepare(const ExpandArgs* __restrict__ A, const uint32_t* hist)
 {
+    constexpr int C = Dims<MODEL>::C;
     __shared__ uint32_t pos[kOrderBuckets];
     __shared__ uint32_t s_all[kOrderBuckets][kOrderParts], s_before[kOrderBuckets][kOrderParts];
+    if(hist)
     {
-        const uint32_t b = threadIdx.x / kOrderParts, part = threadIdx.x % kOrderParts;
-        const uint32_t per = (gridDim.x + kOrderParts - 1) / kOrderParts;
-        const uint32_t j0 = part * per, j1 = j0 + per < gridDim.x ? j0 + per : gridDim.x;
-        uint32_t all = 0, before = 0;
-        for(uint32_t j = j0; j < j1; ++j)
         {
-            const uint32_t v = hist[b * gridDim.x + j];
-            all += v;
-            before += j < blockIdx.x ? v : 0u;
+            const uint32_t b = threadIdx.x / kOrderParts, part = threadIdx.x % kOrderParts;
+            const uint32_t per = (gridDim.x + kOrderParts - 1) / kOrderParts;
+            const uint32_t j0 = part * per, j1 = j0 + per < gridDim.x ? j0 + per : gridDim.x;
+            uint32_t all = 0, before = 0;
+            for(uint32_t j = j0; j < j1; ++j)
+            {
+                const uint32_t v = hist[b * gridDim.x + j];
+                all += v;
+                before += j < blockIdx.x ? v : 0u;
+            }
+            s_all[b][part]    = all;
+            s_before[b][part] = before;
         }
-        s_all[b][part]    = all;
-        s_before[b][part] = before;
+        __syncthreads();
+        if(threadIdx.x < kOrderBuckets)
+        {
+            uint32_t p = 0;
+            for(uint32_t b = 0; b < threadIdx.x; ++b)
+                for(int j = 0; j < kOrderParts; ++j) p += s_all[b][j];
+            for(int j = 0; j < kOrderParts; ++j) p += s_before[threadIdx.x][j];
+            pos[threadIdx.x] = p;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if(threadIdx.x < kOrderBuckets)
-    {
-        uint32_t p = 0;
-        for(uint32_t b = 0; b < threadIdx.x; ++b)
-            for(int j = 0; j < kOrderParts; ++j) p += s_all[b][j];
-        for(int j = 0; j < kOrderParts; ++j) p += s_before[threadIdx.x][j];
-        pos[threadIdx.x] = p;
-    }
-    __syncthreads();
+    const ModelParams& P = A->P;
     const uint32_t total = order_total<MULTI>(A);
     const uint32_t per   = (total + gridDim.x - 1) / gridDim.x;
     const uint32_t b0 = blockIdx.x * per, b1 = b0 + per < total ? b0 + per : total;
     for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x)
-        order[atomicAdd(&pos[child_bucket<MULTI>(A, k)], 1u)] = k;
+    {
+        uint32_t c = k, t, fan, base, query, slot = 0;
+        uint64_t seed;
+        float mix;
+        if constexpr(MULTI)
+        {
+            slot              = segment_of(A->child_off, A->Q, k);
+            c                 = k - A->child_off[slot];
+            t                 = A->ctl[slot].cur.t;
+            fan               = A->ctl[slot].cur.fan;
+            const QueryDev& d = A->qd[slot];
+            base              = d.base;
+            seed              = d.seed;
+            query             = d.query;
+            mix               = d.mix;
+        }
+        else
+        {
+            t     = A->s0.cur.t;
+            fan   = A->s0.cur.fan;
+            base  = A->s0.qd.base;
+            seed  = A->s0.qd.seed;
+            query = A->s0.qd.query;
+            mix   = A->s0.qd.mix;
+        }
+        const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+        const uint32_t parent = A->R[base + c / fan];
+        const uint4 dc        = philox(c, t, query, PURPOSE_CONTROL, k0, k1);
+        const uint4 ds        = philox(c, t, query, PURPOSE_STEPS_ADMIT, k0, k1);
+        const uint32_t dcw[4] = {dc.x, dc.y, dc.z, dc.w};
+        float u[C];
+#pragma unroll
+        for(int j = 0; j < C; ++j) u[j] = P.ctrl_min[j] + u01(dcw[j]) * P.ctrl_rng[j];
+        if(mix > 0.0f && u01(ds.z) < mix)
+        {
+            // the parent's Gaussian instead (learned proposal)
+            const size_t pn = base + parent;
+            const uint4 dp  = philox(c, t, query, PURPOSE_PROPOSAL, k0, k1);
+            float mean[C], sg[C];
+#pragma unroll
+            for(int j = 0; j < C; ++j)
+            {
+                mean[j] = A->prop[(size_t)j * A->T + pn];
+                sg[j]   = A->prop[(size_t)(4 + j) * A->T + pn];
+            }
+            propose_control<C>(mean, sg, dp, P, u);
+        }
+        const int n      = draw_steps(P, ds.x);
+        const uint32_t p = hist ? atomicAdd(&pos[order_bucket(P, n)], 1u) : k;
+        uint32_t uw[4]   = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for(int j = 0; j < C; ++j) uw[j] = __builtin_bit_cast(uint32_t, u[j]);
+        uint4* pr = A->prep + 2 * (size_t)p;
+        pr[0]     = make_uint4(k, parent, (uint32_t)n | (slot << kPrepSlotShift), ds.y);
+        pr[1]     = make_uint4(uw[0], uw[1], uw[2], uw[3]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -928,7 +971,7 @@ int upload_expand_args(kpax_handle* h)
     a.Q         = h->Qmax;
     a.R2        = h->R2;
     a.chunk     = h->expand_chunk;
-    a.order     = h->order_children ? h->order : nullptr;
+    a.prep      = h->prep;
     a.tickets   = h->tickets;
     a.refill    = h->refill_lanes;
     a.prop      = h->prop;
@@ -948,31 +991,41 @@ void launch_expand_model(kpax_handle* h)
         hipLaunchKernelGGL((k_expand<MODEL, true>), dim3(h->expand_blocks), dim3(256), 0, h->stream, A);
 }
 
-void launch_order(kpax_handle* h)
+// The children's records for this iteration (k_prepare), in longest-first
+// order after the bucket counts of k_order_count, or in child order.
+template<int MODEL>
+void launch_prepare(kpax_handle* h)
 {
-    const ExpandArgs* A = (const ExpandArgs*)h->expand_args;
+    const ExpandArgs* A  = (const ExpandArgs*)h->expand_args;
+    const uint32_t* hist = h->order_children ? h->order_hist : nullptr;
     if(h->Qmax == 1)
     {
-        hipLaunchKernelGGL((k_order_count<false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
-        hipLaunchKernelGGL((k_order_scatter<false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist,
-                           h->order);
+        if(hist) hipLaunchKernelGGL((k_order_count<false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
+        hipLaunchKernelGGL((k_prepare<MODEL, false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, hist);
     }
     else
     {
-        hipLaunchKernelGGL((k_order_count<true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
-        hipLaunchKernelGGL((k_order_scatter<true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist,
-                           h->order);
+        if(hist) hipLaunchKernelGGL((k_order_count<true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
+        hipLaunchKernelGGL((k_prepare<MODEL, true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, hist);
     }
 }
 
 void launch_expand(kpax_handle* h)
 {
-    if(h->order_children) launch_order(h);
     switch(h->cfg.model)
     {
-        case KPAX_MODEL_DOUBLE_INTEGRATOR: launch_expand_model<KPAX_MODEL_DOUBLE_INTEGRATOR>(h); break;
-        case KPAX_MODEL_DUBINS_AIRPLANE: launch_expand_model<KPAX_MODEL_DUBINS_AIRPLANE>(h); break;
-        default: launch_expand_model<KPAX_MODEL_QUADCOPTER>(h); break;
+        case KPAX_MODEL_DOUBLE_INTEGRATOR:
+            launch_prepare<KPAX_MODEL_DOUBLE_INTEGRATOR>(h);
+            launch_expand_model<KPAX_MODEL_DOUBLE_INTEGRATOR>(h);
+            break;
+        case KPAX_MODEL_DUBINS_AIRPLANE:
+            launch_prepare<KPAX_MODEL_DUBINS_AIRPLANE>(h);
+            launch_expand_model<KPAX_MODEL_DUBINS_AIRPLANE>(h);
+            break;
+        default:
+            launch_prepare<KPAX_MODEL_QUADCOPTER>(h);
+            launch_expand_model<KPAX_MODEL_QUADCOPTER>(h);
+            break;
     }
 }
 
@@ -1520,7 +1573,7 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->child_off, Q + 1));
     TRY(dalloc(h, &h->atile_off, Q + 1));
     TRY(dalloc(h, &h->ftile_off, Q + 1));
-    TRY(dalloc(h, &h->order, T));
+    TRY(dalloc(h, &h->prep, 2 * T));
     TRY(dalloc(h, &h->order_hist, (size_t)kOrderBlocks * kOrderBuckets));
     TRY(dalloc(h, &h->slot_status, Q + 1));
     h->n_status = (uint32_t)(T / kTile + Q + 2);
