@@ -253,8 +253,8 @@ typedef struct kpax_launch {
     int32_t  order_children;   /* 1: hand out children longest drawn edge first                  */
     uint32_t refill_lanes;     /* idle lanes (1..64) before a wave writes out finished children
                                   and takes new ones; 1 services every lane as soon as it idles */
-    uint32_t scan_tile;        /* items per append/frontier scan tile: 1024 (256-thread blocks)
-                                  or 4096 (1024-thread blocks; fewer look-back steps)           */
+    uint32_t scan_tile;        /* items per append/frontier scan tile: 1024, 2048 or 4096
+                                  (blocks of 256, 512 or 1024 threads, 4 items each)            */
 } kpax_launch;
 
 /* Current launch shape of the handle. */

@@ -253,7 +253,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
 {
     const TreeView& tree   = A->tree;
     const ChildView& ch    = A->ch;
-    const uint32_t* R      = A->R;
     const RegionView& rv   = A->rv;
     const ModelParams& P   = A->P;
     const RegionParams& RP = A->RP;
@@ -416,8 +415,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
 // before the iteration).  Two launches: bucket counts per block, then the
 // scatter, each of whose blocks sums the counts it needs.
 // ---------------------------------------------------------------------------
-constexpr int kOrderBlocks = 256, kOrderBuckets = 32;
-static_assert(256 % kOrderBuckets == 0, "k_prepare splits its 256 threads evenly over the buckets");
+// 256 blocks of 1024 threads: four waves per SIMD hide the Philox chains and
+// the repeat-list loads (one wave per SIMD left both exposed), and the bucket
+// counts stay 256 per bucket.
+constexpr int kOrderBlocks = 256, kOrderThreads = 1024, kOrderBuckets = 32;
+static_assert(kOrderThreads % kOrderBuckets == 0, "k_prepare splits its threads evenly over the buckets");
 
 __device__ __forceinline__ uint32_t order_bucket(const ModelParams& P, int n)
 {
@@ -459,7 +461,7 @@ __device__ __forceinline__ uint32_t child_bucket(const ExpandArgs* A, uint32_t k
 }
 
 template<bool MULTI>
-__global__ __launch_bounds__(256) void k_order_count(const ExpandArgs* __restrict__ A, uint32_t* hist)
+__global__ __launch_bounds__(kOrderThreads) void k_order_count(const ExpandArgs* __restrict__ A, uint32_t* hist)
 {
     __shared__ uint32_t h[kOrderBuckets];
     if(threadIdx.x < kOrderBuckets) h[threadIdx.x] = 0;
@@ -481,15 +483,16 @@ __global__ __launch_bounds__(256) void k_order_count(const ExpandArgs* __restric
 // the longest-first order; without it, the child index.  Each ordering block
 // finds its own first position per bucket from the raw counts: the children
 // of all earlier buckets plus those of this bucket counted by earlier blocks.
-// Thread (bucket b, part j) sums an eighth of b's block counts; one thread
-// per bucket then adds up the parts.
-constexpr int kOrderParts = 256 / kOrderBuckets;
+// Thread (bucket b, part j) sums 1/kOrderParts of b's block counts; one
+// thread per bucket adds up its parts, then each bucket's position sums the
+// totals of the buckets before it.
+constexpr int kOrderParts = kOrderThreads / kOrderBuckets;
 
 template<int MODEL, bool MULTI>
-__global__ __launch_bounds__(256) void k_prepare(const ExpandArgs* __restrict__ A, const uint32_t* hist)
+__global__ __launch_bounds__(kOrderThreads) void k_prepare(const ExpandArgs* __restrict__ A, const uint32_t* hist)
 {
     constexpr int C = Dims<MODEL>::C;
-    __shared__ uint32_t pos[kOrderBuckets];
+    __shared__ uint32_t pos[kOrderBuckets], s_tot[kOrderBuckets];
     __shared__ uint32_t s_all[kOrderBuckets][kOrderParts], s_before[kOrderBuckets][kOrderParts];
     if(hist)
     {
@@ -510,10 +513,20 @@ __global__ __launch_bounds__(256) void k_prepare(const ExpandArgs* __restrict__ 
         __syncthreads();
         if(threadIdx.x < kOrderBuckets)
         {
-            uint32_t p = 0;
-            for(uint32_t b = 0; b < threadIdx.x; ++b)
-                for(int j = 0; j < kOrderParts; ++j) p += s_all[b][j];
-            for(int j = 0; j < kOrderParts; ++j) p += s_before[threadIdx.x][j];
+            uint32_t all = 0, before = 0;
+            for(int j = 0; j < kOrderParts; ++j)
+            {
+                all += s_all[threadIdx.x][j];
+                before += s_before[threadIdx.x][j];
+            }
+            s_tot[threadIdx.x] = all;
+            pos[threadIdx.x]   = before;
+        }
+        __syncthreads();
+        if(threadIdx.x < kOrderBuckets)
+        {
+            uint32_t p = pos[threadIdx.x];
+            for(uint32_t b = 0; b < threadIdx.x; ++b) p += s_tot[b];
             pos[threadIdx.x] = p;
         }
         __syncthreads();
@@ -1000,13 +1013,13 @@ void launch_prepare(kpax_handle* h)
     const uint32_t* hist = h->order_children ? h->order_hist : nullptr;
     if(h->Qmax == 1)
     {
-        if(hist) hipLaunchKernelGGL((k_order_count<false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
-        hipLaunchKernelGGL((k_prepare<MODEL, false>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, hist);
+        if(hist) hipLaunchKernelGGL((k_order_count<false>), dim3(kOrderBlocks), dim3(kOrderThreads), 0, h->stream, A, h->order_hist);
+        hipLaunchKernelGGL((k_prepare<MODEL, false>), dim3(kOrderBlocks), dim3(kOrderThreads), 0, h->stream, A, hist);
     }
     else
     {
-        if(hist) hipLaunchKernelGGL((k_order_count<true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, h->order_hist);
-        hipLaunchKernelGGL((k_prepare<MODEL, true>), dim3(kOrderBlocks), dim3(256), 0, h->stream, A, hist);
+        if(hist) hipLaunchKernelGGL((k_order_count<true>), dim3(kOrderBlocks), dim3(kOrderThreads), 0, h->stream, A, h->order_hist);
+        hipLaunchKernelGGL((k_prepare<MODEL, true>), dim3(kOrderBlocks), dim3(kOrderThreads), 0, h->stream, A, hist);
     }
 }
 
@@ -1029,26 +1042,41 @@ void launch_expand(kpax_handle* h)
     }
 }
 
+// The append and frontier scans in blocks of NT threads (tiles of 4 NT items).
+template<int NT>
+void launch_append(kpax_handle* h)
+{
+    hipLaunchKernelGGL(k_append<NT>, dim3(h->scan_blocks), dim3(NT), append_lds_bytes(h), h->stream, h->ctl, h->qd,
+                       h->bc, h->atile_off, h->Qmax, h->ch, h->tree, h->rv, h->st_append, h->PP, h->S, h->C, h->T,
+                       (uint32_t)h->RP.r1_count, h->R2);
+}
+
+template<int NT>
+void launch_frontier(kpax_handle* h)
+{
+    hipLaunchKernelGGL(k_frontier<NT>, dim3(h->scan_blocks), dim3(NT), 0, h->stream, h->ctl, h->qd, h->bc,
+                       h->ftile_off, h->slot_status, h->Qmax, h->tree, h->R, h->rv, h->st_frontier, h->PP,
+                       (uint32_t)h->RP.r1_count);
+}
+
 // The five launches of one iteration and the copy of the slot statuses into
 // the pinned buffer of parity p.  With `ev` set, phase boundaries are recorded
 // between the launches.
 int enqueue_iteration(kpax_handle* h, int p, bool ev)
 {
-    const uint32_t Q = h->Qmax, R1 = (uint32_t)h->RP.r1_count;
+    const uint32_t Q = h->Qmax;
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc, h->child_off, h->atile_off,
                        h->ftile_off, h->slot_status, Q, h->st_append, h->st_frontier, h->n_status, h->PP, h->envs_dev,
                        &((ExpandArgs*)h->expand_args)->s0, h->scan_tile, h->tickets);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
     launch_expand(h);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][1], h->stream));
-    if(h->scan_tile == 4 * 1024)
-        hipLaunchKernelGGL(k_append<1024>, dim3(h->scan_blocks), dim3(1024), append_lds_bytes(h), h->stream, h->ctl,
-                           h->qd, h->bc, h->atile_off, Q, h->ch, h->tree, h->rv, h->st_append, h->PP, h->S, h->C, h->T,
-                           R1, h->R2);
-    else
-        hipLaunchKernelGGL(k_append<256>, dim3(h->scan_blocks), dim3(256), append_lds_bytes(h), h->stream, h->ctl,
-                           h->qd, h->bc, h->atile_off, Q, h->ch, h->tree, h->rv, h->st_append, h->PP, h->S, h->C, h->T,
-                           R1, h->R2);
+    switch(h->scan_tile)
+    {
+        case 1024: launch_append<256>(h); break;
+        case 2048: launch_append<512>(h); break;
+        default: launch_append<1024>(h); break;
+    }
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][2], h->stream));
     if(h->mix > 0.0f)
     {
@@ -1058,12 +1086,12 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][3], h->stream));
     hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
-    if(h->scan_tile == 4 * 1024)
-        hipLaunchKernelGGL(k_frontier<1024>, dim3(h->scan_blocks), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc,
-                           h->ftile_off, h->slot_status, Q, h->tree, h->R, h->rv, h->st_frontier, h->PP, R1);
-    else
-        hipLaunchKernelGGL(k_frontier<256>, dim3(h->scan_blocks), dim3(256), 0, h->stream, h->ctl, h->qd, h->bc,
-                           h->ftile_off, h->slot_status, Q, h->tree, h->R, h->rv, h->st_frontier, h->PP, R1);
+    switch(h->scan_tile)
+    {
+        case 1024: launch_frontier<256>(h); break;
+        case 2048: launch_frontier<512>(h); break;
+        default: launch_frontier<1024>(h); break;
+    }
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][5], h->stream));
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(h->h_status + (size_t)p * (Q + 1), h->slot_status, (Q + 1) * sizeof(int32_t),
@@ -1456,7 +1484,7 @@ int apply_launch(kpax_handle* h, const kpax_launch& l)
 {
     if(l.expand_blocks < 1 || l.expand_blocks > (1u << 20) || l.expand_chunk < 64 || l.expand_chunk > 4096 ||
        l.expand_chunk % 64 != 0 || l.scan_blocks < 1 || l.scan_blocks > (1u << 20) || l.refill_lanes < 1 ||
-       l.refill_lanes > 64 || (l.scan_tile != 1024 && l.scan_tile != 4096))
+       l.refill_lanes > 64 || (l.scan_tile != 1024 && l.scan_tile != 2048 && l.scan_tile != 4096))
         return fail(h, KPAX_INVALID_ARG, "launch shape out of range");
     HIPC(hipStreamSynchronize(h->stream));
     h->expand_blocks = l.expand_blocks;
@@ -1970,7 +1998,7 @@ int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_
     std::vector<std::vector<kpax_launch>> knobs(7);
     for(uint32_t k = 1; k <= occ; ++k) knobs[0].push_back(kpax_launch{k * cu, 0, 0, 0, 0, 0, 0});
     for(uint32_t c : {64u, 128u, 256u, 512u}) knobs[1].push_back(kpax_launch{0, c, 0, 0, 0, 0, 0});
-    for(uint32_t t : {4096u, 1024u}) knobs[2].push_back(kpax_launch{0, 0, 0, 0, 0, 0, t});
+    for(uint32_t t : {4096u, 2048u, 1024u}) knobs[2].push_back(kpax_launch{0, 0, 0, 0, 0, 0, t});
     for(uint32_t m : {1u, 2u, 4u}) knobs[3].push_back(kpax_launch{0, 0, m * cu, 0, 0, 0, 0});
     for(int g : {1, 0}) knobs[4].push_back(kpax_launch{0, 0, 0, g, 0, 0, 0});
     for(int o : {1, 0}) knobs[5].push_back(kpax_launch{0, 0, 0, 0, o, 0, 0});

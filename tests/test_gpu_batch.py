@@ -162,7 +162,7 @@ def test_launch_knobs_leave_results_unchanged(kpax):
     on a single plan's tree and on a batch."""
     cfg, obs, start, goal = workload(2, seed=0, max_iter=25)
     qs = [kpax.Query(start, goal, seed=70 + i, budget=50000 + 9000 * i) for i in range(6)]
-    shapes = [(1, 1024), (24, 4096), (64, 1024), (16, 4096)]
+    shapes = [(1, 1024), (24, 4096), (64, 2048), (16, 4096)]
     out = {}
     for refill, tile in shapes:
         with kpax.KPAX(cfg, max_queries=3, node_capacity=300000) as p:
