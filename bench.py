@@ -57,11 +57,11 @@ def bytes_per_expansion(S, C):
 # HBM bytes per expansion that k_expand<quadcopter> moves on this workload,
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (average per
 # dispatch over the expansions per dispatch; FETCH scaled by 2, WRITE by 1,
-# the factors that make a 1 GiB device copy read and write 1 GiB): 188 B read
-# (parents are read in longest-edge-first order, a cache line per state word;
-# the grid's inline obstacle boxes) and 95 B written (the 80-byte child and
-# its share of partial lines).
-MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 188 + 95
+# the factors that make a 1 GiB device copy read and write 1 GiB): 237 B read
+# (the child's 32-byte prepared record; parents read in longest-edge-first
+# order, a cache line per state word; the grid's inline obstacle boxes) and
+# 95 B written (the 80-byte child and its share of partial lines).
+MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 237 + 95
 
 
 # FP32 operations per integration step as written in the kernel (add, sub,
