@@ -301,9 +301,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
         // Service pass, once `refill` lanes are idle (or all are): write out the
         // finished children, then hand the idle lanes new ones.  Deferring it
         // lets the wave run the write-out and the draws for several lanes at
-        // once instead of on nearly every trip.
+        // once instead of on nearly every trip.  Once the chunks are drained
+        // no lane can be refilled, so the finished children wait for one
+        // write-out when the last lane is done: the drain's trips then carry
+        // no service passes, which sit on the launch's critical path.
         const uint64_t idle = __ballot(!live);
-        if(idle == ~0ull || (uint32_t)__popcll(idle) >= refill)
+        if(idle == ~0ull || (!drained && (uint32_t)__popcll(idle) >= refill))
         {
             if(pend)
             {
