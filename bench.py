@@ -5,7 +5,7 @@ A step is one complete plan() of the 12D quadcopter forest workload (400
 columns in [0,10]^2 x [0,5], RK4 dt = 0.02, 2^20-node tree cap, 16^3 regions,
 64^3 collision grid), each step with its own seed.  Inputs (obstacles, grid)
 are resident in HBM before timing starts.  The timed plans run as they ship
-(each iteration one captured HIP graph, no profiling events); a second,
+(the default launch shape, no profiling events); a second,
 profiled pass over the same seeds gives the per-phase device times and the
 expansion kernel's roofline figures (the plans are deterministic, so both
 passes do identical work).

@@ -249,7 +249,8 @@ typedef struct kpax_launch {
     uint32_t expand_blocks;    /* work-groups of the expansion kernel (its waves are persistent) */
     uint32_t expand_chunk;     /* children a wave takes per work ticket (multiple of 64, <= 4096) */
     uint32_t scan_blocks;      /* work-groups of the append and frontier scans                   */
-    int32_t  use_graph;        /* 1: replay each iteration as one captured HIP graph             */
+    int32_t  use_graph;        /* 1: replay each iteration as one captured HIP graph (default 0:
+                                  direct launches, measured faster)                             */
     int32_t  order_children;   /* 1: hand out children longest drawn edge first                  */
     uint32_t refill_lanes;     /* idle lanes (1..64) before a wave writes out finished children
                                   and takes new ones; 1 services every lane as soon as it idles */

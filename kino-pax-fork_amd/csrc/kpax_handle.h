@@ -111,7 +111,7 @@ struct kpax_handle {
     uint32_t expand_chunk   = 64;
     bool order_children     = true;     // hand out children longest drawn edge first
     uint32_t* tickets       = nullptr;  // kTicketGroups expansion chunk counters
-    uint32_t refill_lanes   = 24;       // idle lanes that trigger a wave's service pass
+    uint32_t refill_lanes   = 16;       // idle lanes that trigger a wave's service pass
     uint32_t scan_tile      = 4096;     // items per append/frontier tile: 1024, 2048 or 4096 (4 per thread)
     uint4* prep             = nullptr;  // per processing position: the child's prepared draws (2 x uint4)
     uint32_t* order_hist    = nullptr;  // buckets x order blocks, scanned in place
@@ -138,7 +138,7 @@ struct kpax_handle {
     // one captured iteration per status-buffer parity, replayed while profiling is off
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     bool graph_tried        = false;
-    bool use_graph          = true;
+    bool use_graph          = false;
     // learned proposal (kpax_set_proposal)
     void* mlp               = nullptr;  // bf16 network layouts
     float* prop             = nullptr;  // per node: 4 means then 4 sigmas, column stride T

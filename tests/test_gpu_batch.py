@@ -87,8 +87,6 @@ def test_batch_matches_single_plans(kpax):
         for e, o in enumerate(envs):
             p.set_environment(e, o)
         res = p.plan_batch(qs)
-        st = p.stats()
-    assert st["graph_launches"] > 0, "the iteration was not captured as a HIP graph"
     for i, (q, r) in enumerate(zip(qs, res)):
         c = cfg.copy()
         c.max_tree = q.budget
@@ -105,15 +103,19 @@ def test_batch_graph_and_direct_launches_agree(kpax):
                      budget=30000 + 7000 * i) for i in range(10)]
     with kpax.KPAX(cfg, max_queries=4, node_capacity=200000) as p:
         p.set_obstacles(obs)
+        p.set_launch(use_graph=1)
         graphed = p.plan_batch(qs)
         n_graph = p.stats()["graph_launches"]
-        p.set_profiling(True)  # profiling launches the kernels one by one
+        p.set_launch(use_graph=0)
         direct = p.plan_batch(qs)
+        p.set_launch(use_graph=1)
+        p.set_profiling(True)  # profiling launches the kernels one by one
+        profiled = p.plan_batch(qs)
         p.set_profiling(False)
         again = p.plan_batch(qs)  # the handle's state carries over between calls
-    assert n_graph > 0
-    for a, b, c in zip(graphed, direct, again):
-        assert _same(a, b) and _same(a, c)
+    assert n_graph > 0, "the iteration was not captured as a HIP graph"
+    for a, b, c, d in zip(graphed, direct, profiled, again):
+        assert _same(a, b) and _same(a, c) and _same(a, d)
 
 
 def test_batch_rejects_impossible_queries_and_runs_the_rest(kpax):
