@@ -251,7 +251,8 @@ typedef struct kpax_launch {
     uint32_t scan_blocks;      /* work-groups of the append and frontier scans                   */
     int32_t  use_graph;        /* 1: replay each iteration as one captured HIP graph (default 0:
                                   direct launches, measured faster)                             */
-    int32_t  order_children;   /* 1: hand out children longest drawn edge first                  */
+    int32_t  order_children;   /* 1: hand out children longest drawn edge first (default 1 for a
+                                  single-query handle, 0 for a batch handle)                    */
     uint32_t refill_lanes;     /* idle lanes (1..64) before a wave writes out finished children
                                   and takes new ones; 1 services every lane as soon as it idles */
     uint32_t scan_tile;        /* items per append/frontier scan tile: 1024, 2048 or 4096

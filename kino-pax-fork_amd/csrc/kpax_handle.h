@@ -109,7 +109,7 @@ struct kpax_handle {
     void* expand_args       = nullptr;
     uint32_t expand_blocks  = 1;
     uint32_t expand_chunk   = 64;
-    bool order_children     = true;     // hand out children longest drawn edge first
+    bool order_children     = true;     // hand out children longest drawn edge first (default: single-query handles)
     uint32_t* tickets       = nullptr;  // kTicketGroups expansion chunk counters
     uint32_t refill_lanes   = 16;       // idle lanes that trigger a wave's service pass
     uint32_t scan_tile      = 4096;     // items per append/frontier tile: 1024, 2048 or 4096 (4 per thread)

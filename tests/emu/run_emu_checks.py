@@ -219,7 +219,7 @@ def batch_checks():
         p.set_environment(0, obs)
         p.set_environment(1, obs1)
         res = p.plan_batch(qs)
-        p.set_launch(expand_blocks=3, expand_chunk=192, scan_blocks=2, order_children=0, refill_lanes=1,
+        p.set_launch(expand_blocks=3, expand_chunk=192, scan_blocks=2, order_children=1, refill_lanes=1,
                      scan_tile=1024)
         res2 = p.plan_batch(qs)
         p.set_launch(refill_lanes=64)

@@ -158,23 +158,26 @@ def test_autotune_keeps_results_and_applies_the_shape(kpax):
 
 def test_launch_knobs_leave_results_unchanged(kpax):
     """A wave writes out finished children and takes new ones once
-    `refill_lanes` of its lanes are idle, and the append and frontier scans
-    run over tiles of `scan_tile` items; every child is a function of its
-    index and the scans are exact, so both knobs change only speed.  Checked
-    on a single plan's tree and on a batch."""
+    `refill_lanes` of its lanes are idle, the append and frontier scans run
+    over tiles of `scan_tile` items, and children are handed out in index
+    order or longest edge first (`order_children`, off by default for a
+    batch handle); every child is a function of its index and the scans are
+    exact, so the knobs change only speed.  Checked on a single plan's tree
+    and on a batch."""
     cfg, obs, start, goal = workload(2, seed=0, max_iter=25)
     qs = [kpax.Query(start, goal, seed=70 + i, budget=50000 + 9000 * i) for i in range(6)]
-    shapes = [(1, 1024), (24, 4096), (64, 2048), (16, 4096)]
+    shapes = [(1, 1024, 0), (24, 4096, 1), (64, 2048, 1), (16, 4096, 0)]
     out = {}
-    for refill, tile in shapes:
+    for refill, tile, order in shapes:
         with kpax.KPAX(cfg, max_queries=3, node_capacity=300000) as p:
             p.set_obstacles(obs)
-            p.set_launch(refill_lanes=refill, scan_tile=tile)
+            assert p.launch()["order_children"] == 0
+            p.set_launch(refill_lanes=refill, scan_tile=tile, order_children=order)
             assert p.launch()["refill_lanes"] == refill and p.launch()["scan_tile"] == tile
             res = p.plan_batch(qs)
             one = p.plan(start, goal, seed=5)
             tree = p.tree()
-        out[(refill, tile)] = (res, one, tree)
+        out[(refill, tile, order)] = (res, one, tree)
     ref_res, ref_one, ref_tree = out[shapes[0]]
     for shape in shapes[1:]:
         res, one, tree = out[shape]
