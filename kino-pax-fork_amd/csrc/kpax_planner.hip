@@ -11,9 +11,13 @@
 // KPAX.cu:303-543):
 //   k_advance   1 block   publish each slot's iteration record; scan the
 //                         slots' child / tile counts into offsets
-//   k_order_*   2 passes  children sorted longest drawn edge first (optional)
-//   k_expand    hot       one lane per child: Philox controls, RK4 with checks,
-//                         region counters, admission (propagateFrontier_kernel1/2)
+//   k_order_count         bucket counts for the longest-drawn-edge-first order
+//                         (optional: on by default for a single-query handle)
+//   k_prepare   coalesced each child's parent, Philox control and step draws,
+//                         written at its processing position
+//   k_expand    hot       one lane per child: RK4 with checks, region and
+//                         sub-region of the result, admission
+//                         (propagateFrontier_kernel1/2)
 //   k_append    scan      admitted children -> tree in slot order, goal test
 //                         (updateFrontier_kernel, first half)
 //   k_propose   (learned proposal only) the network on the appended nodes
@@ -22,8 +26,9 @@
 //                         (updateFrontier_kernel second half + findInd/repeatInd)
 // Work of all slots is flattened: children, append tiles and frontier tiles
 // are numbered across slots by the advance kernel's offsets, and each item
-// finds its slot by binary search.  With profiling off the launches and the
-// status copy are replayed as one captured HIP graph.
+// finds its slot by binary search.  The launches and the status copy go on
+// the stream directly, or (launch option) as one captured HIP graph per
+// status parity.
 
 #include <hip/hip_runtime.h>
 
