@@ -223,6 +223,13 @@ def test_plan_with_goal_progress_pruning_bit_exact(kpax, oracle, k, kw):
     assert np.array_equal(tree["states"].view(np.uint32), o["states"].view(np.uint32))
 
 
+def test_plan_full_size_tree_bit_exact_vs_oracle(kpax, oracle):
+    """The headline workload at its full size (12D quadcopter, 400-column
+    forest, 2^20-node cap, default iteration limit): tree, controls, step
+    counts, regions and scores equal the oracle's bit for bit."""
+    _compare_plans(kpax, oracle, 2, 5)
+
+
 def test_plan_quadcopter_full_size_finds_verified_path(kpax):
     cfg, obs, start, goal = workload(2, seed=0)
     with kpax.KPAX(cfg) as p:
