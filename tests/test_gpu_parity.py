@@ -199,6 +199,7 @@ def _compare_plans(kpax, oracle, k, seed, **kw):
 @pytest.mark.parametrize("k,seed,kw", [
     (1, 0, dict(max_tree=20000, max_iter=12)),
     (1, 1, dict(max_tree=65536, max_iter=8)),
+    (1, 2, dict()),  # the double-integrator configuration at its full size
     (2, 0, dict(max_tree=1 << 17)),
     (2, 3, dict(max_tree=1 << 16, max_iter=10)),
     (3, 0, dict(max_tree=1 << 17, max_iter=15)),
