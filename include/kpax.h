@@ -117,10 +117,12 @@ typedef struct kpax_result {
 } kpax_result;
 
 /* Per-phase device time, accumulated with HIP events on the handle's stream
- * while profiling is on (kpax_set_profiling).  Phases: 0 expand, 1 append,
- * 2 score, 3 frontier, 4 init, 5 propose (the learned proposal's network on
- * the appended nodes; zero while no proposal is set). */
-#define KPAX_N_PHASES 6
+ * while profiling is on (kpax_set_profiling).  Phases: 0 expand (the
+ * expansion kernel alone), 1 append, 2 score, 3 frontier, 4 init, 5 propose
+ * (the learned proposal's network on the appended nodes; zero while no
+ * proposal is set), 6 prepare (the children's ordering counts and prepared
+ * records, before the expansion). */
+#define KPAX_N_PHASES 7
 typedef struct kpax_stats {
     double   phase_ms[KPAX_N_PHASES];
     uint64_t phase_launches[KPAX_N_PHASES];

@@ -132,7 +132,7 @@ struct kpax_handle {
     int32_t* h_path_steps   = nullptr;
     int32_t* h_path_idx     = nullptr;
     hipEvent_t it_ev[2]     = {nullptr, nullptr};
-    hipEvent_t ph_ev[2][6]  = {};       // phase boundaries of iterations of each parity (profiling):
+    hipEvent_t ph_ev[2][7]  = {};       // phase boundaries of iterations of each parity (profiling):
                                         // after advance, expand, append, propose, score, frontier
     hipEvent_t init_ev[2]   = {nullptr, nullptr};
     // one captured iteration per status-buffer parity, replayed while profiling is off

@@ -1031,22 +1031,25 @@ void launch_prepare(kpax_handle* h)
     }
 }
 
-void launch_expand(kpax_handle* h)
+// The preparation passes, then the expansion; `between` (profiling) is
+// recorded after the former.
+template<int MODEL>
+int launch_prepare_and_expand(kpax_handle* h, hipEvent_t between)
+{
+    launch_prepare<MODEL>(h);
+    if(between) HIPC(hipEventRecord(between, h->stream));
+    launch_expand_model<MODEL>(h);
+    return KPAX_OK;
+}
+
+int launch_expand(kpax_handle* h, hipEvent_t between)
 {
     switch(h->cfg.model)
     {
         case KPAX_MODEL_DOUBLE_INTEGRATOR:
-            launch_prepare<KPAX_MODEL_DOUBLE_INTEGRATOR>(h);
-            launch_expand_model<KPAX_MODEL_DOUBLE_INTEGRATOR>(h);
-            break;
-        case KPAX_MODEL_DUBINS_AIRPLANE:
-            launch_prepare<KPAX_MODEL_DUBINS_AIRPLANE>(h);
-            launch_expand_model<KPAX_MODEL_DUBINS_AIRPLANE>(h);
-            break;
-        default:
-            launch_prepare<KPAX_MODEL_QUADCOPTER>(h);
-            launch_expand_model<KPAX_MODEL_QUADCOPTER>(h);
-            break;
+            return launch_prepare_and_expand<KPAX_MODEL_DOUBLE_INTEGRATOR>(h, between);
+        case KPAX_MODEL_DUBINS_AIRPLANE: return launch_prepare_and_expand<KPAX_MODEL_DUBINS_AIRPLANE>(h, between);
+        default: return launch_prepare_and_expand<KPAX_MODEL_QUADCOPTER>(h, between);
     }
 }
 
@@ -1077,30 +1080,27 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
                        h->ftile_off, h->slot_status, Q, h->st_append, h->st_frontier, h->n_status, h->PP, h->envs_dev,
                        &((ExpandArgs*)h->expand_args)->s0, h->scan_tile, h->tickets);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
-    launch_expand(h);
-    if(ev) HIPC(hipEventRecord(h->ph_ev[p][1], h->stream));
+    int rc = launch_expand(h, ev ? h->ph_ev[p][1] : nullptr);
+    if(rc != KPAX_OK) return rc;
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][2], h->stream));
     switch(h->scan_tile)
     {
         case 1024: launch_append<256>(h); break;
         case 2048: launch_append<512>(h); break;
         default: launch_append<1024>(h); break;
     }
-    if(ev) HIPC(hipEventRecord(h->ph_ev[p][2], h->stream));
-    if(h->mix > 0.0f)
-    {
-        const int rc = launch_propose(h, 0);
-        if(rc != KPAX_OK) return rc;
-    }
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][3], h->stream));
-    hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
+    if(h->mix > 0.0f && (rc = launch_propose(h, 0)) != KPAX_OK) return rc;
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
+    hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][5], h->stream));
     switch(h->scan_tile)
     {
         case 1024: launch_frontier<256>(h); break;
         case 2048: launch_frontier<512>(h); break;
         default: launch_frontier<1024>(h); break;
     }
-    if(ev) HIPC(hipEventRecord(h->ph_ev[p][5], h->stream));
+    if(ev) HIPC(hipEventRecord(h->ph_ev[p][6], h->stream));
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(h->h_status + (size_t)p * (Q + 1), h->slot_status, (Q + 1) * sizeof(int32_t),
                         hipMemcpyDeviceToHost, h->stream));
@@ -1185,10 +1185,10 @@ struct QueryIn {
 void account_iteration(kpax_handle* h, int p)
 {
     if(h->h_status[(size_t)p * (h->Qmax + 1) + h->Qmax] <= 0) return;  // no slot was running
-    // event pairs -> phases expand, append, propose, score, frontier
-    constexpr int kPhase[5] = {0, 1, 5, 2, 3};
+    // event pairs -> phases prepare, expand, append, propose, score, frontier
+    constexpr int kPhase[6] = {6, 0, 1, 5, 2, 3};
     float ms = 0.0f;
-    for(int k = 0; k < 5; ++k)
+    for(int k = 0; k < 6; ++k)
     {
         if(kPhase[k] == 5 && h->mix <= 0.0f) continue;
         if(hipEventElapsedTime(&ms, h->ph_ev[p][k], h->ph_ev[p][k + 1]) == hipSuccess)
@@ -1649,7 +1649,7 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     {
         if(hipEventCreateWithFlags(&h->it_ev[i], hipEventDisableTiming) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
         if(hipEventCreate(&h->init_ev[i]) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
-        for(int k = 0; k < 6; ++k)
+        for(int k = 0; k < 7; ++k)
             if(hipEventCreate(&h->ph_ev[i][k]) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
     }
     // every slot idle; scan state at epoch 0 (never current)

@@ -23,8 +23,8 @@ STATUS = {0: "OK", 1: "NOT_FOUND", 2: "BUDGET_EXHAUSTED", -1: "INVALID_ARG", -2:
           -3: "NO_MEMORY", -4: "NO_DEVICE", -5: "IO_ERROR"}
 OK, NOT_FOUND, BUDGET_EXHAUSTED, INVALID_ARG = 0, 1, 2, -1
 MODEL_DOUBLE_INTEGRATOR, MODEL_DUBINS_AIRPLANE, MODEL_QUADCOPTER = 1, 2, 3
-N_PHASES = 6
-PHASES = ("expand", "append", "score", "frontier", "init", "propose")
+N_PHASES = 7
+PHASES = ("expand", "append", "score", "frontier", "init", "propose", "prepare")
 
 
 class Config(ctypes.Structure):
