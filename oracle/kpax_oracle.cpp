@@ -542,12 +542,58 @@ inline float goal_distance(const float* p, const float* g)
     return std::sqrt(dx * dx + dy * dy + dz * dz);
 }
 
+// ---------------------------------------------------------------------------
+// The iteration's decisions, each as the reference states it.
+// ---------------------------------------------------------------------------
+
+// Children per repeat-list entry and children of the iteration
+// (propagateFrontier, KPAX.cu:323-333): the block size (fan-out 32) while
+// W * 32 fits the remaining capacity, else min(int(float(remaining) /
+// float(W)), 32), and 1 when that is 0.  M is capped at the remaining
+// capacity (the reference's kernel2 can overflow the tree there).
+inline void fan_out(const kpax_config& c, uint64_t W, uint32_t remaining, uint32_t* fan_out, uint64_t* M_out)
+{
+    uint32_t fan = c.fanout;
+    uint64_t M   = W * fan;
+    if(M > remaining)
+    {
+        fan = (uint32_t)(int)((float)remaining / (float)W);
+        if(fan > c.fanout) fan = c.fanout;
+        if(fan == 0) fan = 1;
+        M = std::min<uint64_t>(W * fan, remaining);
+    }
+    *fan_out = fan;
+    *M_out   = M;
+}
+
+// A valid child joins the next frontier on a score draw strictly below its
+// region's score of the previous iteration (KPAX.cu:393, 433:
+// curand_uniform(&randSeed) < vertexScores[x1Vertex]) ...
+inline bool admit_by_score(uint32_t word, float score) { return u01(word) < score; }
+
+// ... or when it is the first to reach its R2 cell (!activeSubVertices, KPAX.cu:393-395).
+
+// An old node re-enters the frontier on a draw at or below its region's
+// score plus fAccept (KPAX.cu:512: curand_uniform(&seed) <= vertexScores + fAccept).
+inline bool readmit(uint32_t word, float score, float fa) { return u01(word) <= score + fa; }
+
+// Repeat count of a new node: 15 while its region holds fewer than 10 valid
+// children, else 1 (KPAX.cu:476-479).
+inline uint32_t new_node_repeat(const kpax_config& c, int32_t region_valid)
+{
+    return region_valid < c.sparse_valid_below ? c.sparse_repeat : c.dense_repeat;
+}
+
 struct PlanOut {
     int32_t status = KPAX_NOT_FOUND;
     uint32_t iterations = 0, tree_size = 0;
     int32_t goal = -1;
     uint64_t expansions = 0;
 };
+
+// Per-iteration record of a plan (kTraceWords words per iteration).
+constexpr int kTraceWords = 8;
+enum : int { TR_W = 0, TR_FAN, TR_M, TR_ADMITTED, TR_TS, TR_FA_BITS, TR_GOAL, TR_NEXT_W };
 
 struct Planner {
     const kpax_config& c;
@@ -561,6 +607,7 @@ struct Planner {
     std::vector<int32_t> valid, total, covered;
     std::vector<int32_t> r2_first;
     std::vector<float> score;
+    std::vector<uint32_t> weight;  // repeat count of every node in the last repeat list built
     float score_total = 0.0f;
 
     Planner(const kpax_config& cfg, const float* obs, uint32_t n_obs)
@@ -577,9 +624,11 @@ struct Planner {
         covered.assign(R.r1_count, 0);
         r2_first.assign((size_t)R.r1_count * R.r2_per, std::numeric_limits<int32_t>::max());
         score.assign(R.r1_count, 0.0f);
+        weight.assign(N, 0u);
     }
 
-    PlanOut plan(const float* start, const float* goal, uint64_t seed, uint32_t query)
+    // trace: kTraceWords words per iteration for up to max_iter iterations (optional)
+    PlanOut plan(const float* start, const float* goal, uint64_t seed, uint32_t query, uint32_t* trace = nullptr)
     {
         PlanOut out;
         for(int d = 0; d < S; ++d) st[d] = start[d];
@@ -591,9 +640,11 @@ struct Planner {
         // frontier repeat list: the root, root_repeat times (KPAX.cu:75-77)
         std::vector<uint32_t> rep(c.root_repeat, 0u);
         uint64_t W = c.root_repeat;
+        weight[0]  = c.root_repeat;
         std::vector<float> cs((size_t)N * S), cc((size_t)N * C);
         std::vector<int32_t> csteps(N), cpar(N), creg(N), cr2(N);
-        std::vector<uint8_t> adm(N), cvalid(N), csadm(N);
+        std::vector<uint8_t> adm(N), cvalid(N);
+        std::vector<uint32_t> cword(N);
         for(uint32_t t = 1;; ++t)
         {
             if(t > c.max_iter)
@@ -601,7 +652,6 @@ struct Planner {
                 out.status = KPAX_NOT_FOUND;
                 break;
             }
-            // fan-out (KPAX.cu:323-350)
             const uint32_t remaining = N - ts;
             if(remaining == 0)
             {
@@ -609,15 +659,9 @@ struct Planner {
                 break;
             }
             out.iterations = t;
-            uint32_t fan = c.fanout;
-            uint64_t M   = W * fan;
-            if(M > remaining)
-            {
-                fan = (uint32_t)(int)((float)remaining / (float)W);
-                if(fan > c.fanout) fan = c.fanout;
-                if(fan == 0) fan = 1;
-                M = std::min<uint64_t>(W * fan, remaining);
-            }
+            uint32_t fan;
+            uint64_t M;
+            fan_out(c, W, remaining, &fan, &M);
             out.expansions += M;
             // expansion (propagateFrontier_kernel1/2, KPAX.cu:358-439)
 #pragma omp parallel for schedule(dynamic, 256)
@@ -645,11 +689,14 @@ struct Planner {
                 __atomic_fetch_add(&total[a1], 1, __ATOMIC_RELAXED);
                 if(fail == 0) __atomic_fetch_add(&valid[a1], 1, __ATOMIC_RELAXED);
                 cvalid[cix] = fail == 0 ? 1 : 0;
-                csadm[cix]  = fail == 0 && u01(ds.v[1]) < score[a1] ? 1 : 0;
+                cword[cix]  = ds.v[1];
             }
-            // Sub-region novelty as the reference's kernel decides it when its
-            // threads run one at a time in slot order (KPAX.cu:394-395): the
-            // first valid child to reach an uncovered R2 cell sees it empty.
+            // Admission.  Sub-region novelty is decided as the reference's
+            // kernel decides it when its threads run one at a time in slot
+            // order (KPAX.cu:394-395): the first valid child to reach an
+            // uncovered R2 cell sees it empty.  The score draw compares with
+            // the scores of the previous iteration (updateVertices runs after
+            // the expansion, KPAX.cu:96-97).
             for(uint32_t ci = 0; ci < M; ++ci)
             {
                 bool fresh = false;
@@ -659,7 +706,8 @@ struct Planner {
                     covered[creg[ci]]++;
                     fresh = true;
                 }
-                adm[ci] = (csadm[ci] || fresh) ? 1 : 0;
+                const bool by_score = cvalid[ci] && admit_by_score(cword[ci], score[creg[ci]]);
+                adm[ci]             = (by_score || fresh) ? 1 : 0;
                 if(adm[ci] && c.prune)
                 {
                     // goal-progress pruning (prune_updateFrontier_kernel, PruneKPAX.cu:472-489);
@@ -695,14 +743,23 @@ struct Planner {
                 reg[idx]    = creg[ci];
                 if(goal_idx < 0 && goal_distance(&cs[(size_t)ci * S], goal) < c.goal_radius) goal_idx = (int32_t)idx;
             }
+            // fAccept of this iteration, from the tree size after the append (KPAX.cu:527-528)
             const float fa = f_accept(c, t, ts);
-            if(std::getenv("KPAX_ORACLE_TRACE"))
+            uint32_t* tr   = trace ? trace + (size_t)(t - 1) * kTraceWords : nullptr;
+            if(tr)
             {
-                size_t nv = 0;
-                for(uint32_t ci = 0; ci < M; ++ci) nv += adm[ci];
-                std::fprintf(stderr, "t=%u W=%llu fan=%u M=%llu admitted=%zu ts=%u fa=%g\n", t, (unsigned long long)W,
-                             fan, (unsigned long long)M, nv, ts, fa);
+                tr[TR_W]        = (uint32_t)W;
+                tr[TR_FAN]      = fan;
+                tr[TR_M]        = (uint32_t)M;
+                tr[TR_ADMITTED] = ts - ts_old;
+                tr[TR_TS]       = ts;
+                std::memcpy(&tr[TR_FA_BITS], &fa, 4);
+                tr[TR_GOAL]   = (uint32_t)goal_idx;
+                tr[TR_NEXT_W] = 0;
             }
+            if(std::getenv("KPAX_ORACLE_TRACE"))
+                std::fprintf(stderr, "t=%u W=%llu fan=%u M=%llu admitted=%u ts=%u fa=%g\n", t, (unsigned long long)W,
+                             fan, (unsigned long long)M, ts - ts_old, ts, fa);
             if(goal_idx >= 0)
             {
                 out.status = KPAX_OK;
@@ -720,13 +777,15 @@ struct Planner {
                 if(j < ts_old)
                 {
                     const U4 dr = draw(seed, j, t, query, PURPOSE_READMIT);
-                    w           = u01(dr.v[0]) <= score[reg[j]] + fa ? 1u : 0u;
+                    w           = readmit(dr.v[0], score[reg[j]], fa) ? 1u : 0u;
                 }
                 else
-                    w = valid[reg[j]] < c.sparse_valid_below ? c.sparse_repeat : c.dense_repeat;
+                    w = new_node_repeat(c, valid[reg[j]]);
+                weight[j] = w;
                 for(uint32_t k = 0; k < w && rep.size() < N; ++k) rep.push_back(j);
                 W += w;
             }
+            if(tr) tr[TR_NEXT_W] = (uint32_t)W;
         }
         out.tree_size = ts;
         return out;
@@ -907,10 +966,15 @@ uint64_t oracle_repeat_expand(const uint32_t* weights, uint32_t n, uint32_t* out
 // Whole planner.  Outputs are optional (NULL to skip); SoA outputs use
 // max_tree as column stride.  Returns the status; *info receives
 // {iterations, tree_size, goal_index, expansions_lo, expansions_hi}.
-int oracle_plan(const kpax_config* cfg, const float* obs, uint32_t n_obs, const float* start, const float* goal,
-                uint64_t seed, int32_t threads, uint32_t* info, float* states_soa, float* controls_soa,
-                int32_t* steps_out, int32_t* parents_out, int32_t* regions_out, int32_t* valid_out,
-                int32_t* total_out, int32_t* covered_out, float* scores_out)
+// oracle_plan_ex adds: weights_out (max_tree entries: every node's repeat
+// count in the last repeat list built) and trace (8 words per iteration for up
+// to max_iter iterations: W, fan, M, admitted, tree size after the append,
+// fAccept's float bits, goal index or -1, next W).
+int oracle_plan_ex(const kpax_config* cfg, const float* obs, uint32_t n_obs, const float* start, const float* goal,
+                   uint64_t seed, int32_t threads, uint32_t* info, float* states_soa, float* controls_soa,
+                   int32_t* steps_out, int32_t* parents_out, int32_t* regions_out, int32_t* valid_out,
+                   int32_t* total_out, int32_t* covered_out, float* scores_out, uint32_t* weights_out,
+                   uint32_t* trace)
 {
 #ifdef _OPENMP
     if(threads > 0) omp_set_num_threads(threads);
@@ -918,7 +982,8 @@ int oracle_plan(const kpax_config* cfg, const float* obs, uint32_t n_obs, const 
     (void)threads;
 #endif
     Planner P(*cfg, obs, n_obs);
-    const PlanOut o = P.plan(start, goal, seed, 0);
+    const PlanOut o = P.plan(start, goal, seed, 0, trace);
+    if(weights_out) std::memcpy(weights_out, P.weight.data(), sizeof(uint32_t) * cfg->max_tree);
     const uint32_t N = cfg->max_tree;
     if(info)
     {
@@ -944,6 +1009,53 @@ int oracle_plan(const kpax_config* cfg, const float* obs, uint32_t n_obs, const 
     if(covered_out) std::memcpy(covered_out, P.covered.data(), sizeof(int32_t) * r1);
     if(scores_out) std::memcpy(scores_out, P.score.data(), sizeof(float) * r1);
     return o.status;
+}
+
+int oracle_plan(const kpax_config* cfg, const float* obs, uint32_t n_obs, const float* start, const float* goal,
+                uint64_t seed, int32_t threads, uint32_t* info, float* states_soa, float* controls_soa,
+                int32_t* steps_out, int32_t* parents_out, int32_t* regions_out, int32_t* valid_out,
+                int32_t* total_out, int32_t* covered_out, float* scores_out)
+{
+    return oracle_plan_ex(cfg, obs, n_obs, start, goal, seed, threads, info, states_soa, controls_soa, steps_out,
+                          parents_out, regions_out, valid_out, total_out, covered_out, scores_out, nullptr, nullptr);
+}
+
+// The quadcopter's derivative (ode, statePropagator.cu:325-370) at one state
+// and control, in the float form the planner integrates (use_float = 1, with
+// the project's sin/cos) or in double (0).
+void oracle_quad_ode(const kpax_config* cfg, const double* x, const double* u, double* out, int32_t use_float)
+{
+    if(use_float)
+    {
+        const QuadK<float> k = quad_constants<float>(*cfg);
+        float xf[12], uf[4], o[12], sn[3], cn[3];
+        const float zero[12] = {};
+        for(int j = 0; j < 12; ++j) xf[j] = (float)x[j];
+        for(int j = 0; j < 4; ++j) uf[j] = (float)u[j];
+        for(int j = 0; j < 3; ++j) sincos_f32(xf[3 + j], &sn[j], &cn[j]);
+        quad_ode<float>(o, xf, zero, 1.0f, uf, k, sn, cn, xf[4]);
+        for(int j = 0; j < 12; ++j) out[j] = o[j];
+    }
+    else
+    {
+        const QuadK<double> k = quad_constants<double>(*cfg);
+        double sn[3], cn[3];
+        const double zero[12] = {};
+        for(int j = 0; j < 3; ++j) Math<double>::sincos(x[3 + j], &sn[j], &cn[j]);
+        quad_ode<double>(out, x, zero, 1.0, u, k, sn, cn, x[4]);
+    }
+}
+
+// The iteration's decision predicates, for boundary tests.
+void oracle_fan_out(const kpax_config* cfg, uint64_t W, uint32_t remaining, uint32_t* fan, uint64_t* M)
+{
+    fan_out(*cfg, W, remaining, fan, M);
+}
+int32_t oracle_admit_by_score(uint32_t word, float score) { return admit_by_score(word, score) ? 1 : 0; }
+int32_t oracle_readmit(uint32_t word, float score, float fa) { return readmit(word, score, fa) ? 1 : 0; }
+uint32_t oracle_new_node_repeat(const kpax_config* cfg, int32_t region_valid)
+{
+    return new_node_repeat(*cfg, region_valid);
 }
 
 // ---------------------------------------------------------------------------

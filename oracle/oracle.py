@@ -14,32 +14,48 @@ LIB_PATH = os.path.join(HERE, "liboracle_kpax.so")
 _L = None
 
 
+def _sigs():
+    P, u32, i32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
+    f32 = ctypes.c_float
+    return {
+        "oracle_philox": (None, [P, P, P]),
+        "oracle_sincos_f32": (None, [P, u32, P, P]),
+        "oracle_sincos_shift": (None, [P, P, u32, P, P]),
+        "oracle_propagate": (i32, [P, P, u32, P, P, P, u32, i32, P, P, P]),
+        "oracle_collide": (None, [P, u32, P, P, u32, P]),
+        "oracle_grid_counts": (None, [P, P, u32, P]),
+        "oracle_regions": (None, [P, P, u32, P, P]),
+        "oracle_histogram": (None, [P, P, P, u32, i32, P, P, P, P, P]),
+        "oracle_scores": (None, [P, P, P, P, P, P]),
+        "oracle_f_accept": (f32, [P, u32, u32]),
+        "oracle_repeat_expand": (u64, [P, u32, P, u64]),
+        "oracle_plan": (i32, [P, P, u32, P, P, u64, i32, P, P, P, P, P, P, P, P, P, P]),
+        "oracle_plan_ex": (i32, [P, P, u32, P, P, u64, i32, P, P, P, P, P, P, P, P, P, P, P, P]),
+        "oracle_fan_out": (None, [P, u64, u32, P, P]),
+        "oracle_admit_by_score": (i32, [u32, f32]),
+        "oracle_readmit": (i32, [u32, f32, f32]),
+        "oracle_new_node_repeat": (u32, [P, i32]),
+        "oracle_quad_ode": (None, [P, P, P, P, i32]),
+    }
+
+
+def load(path):
+    """A CDLL of an oracle build at `path` with its signatures set (the
+    default build is lib(); tests load deliberately broken builds this way)."""
+    L = ctypes.CDLL(path)
+    for name, (res, args) in _sigs().items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
 def lib():
     global _L
     if _L is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with oracle/build.py")
-        L = ctypes.CDLL(LIB_PATH)
-        P, u32, i32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
-        sig = {
-            "oracle_philox": (None, [P, P, P]),
-            "oracle_sincos_f32": (None, [P, u32, P, P]),
-            "oracle_sincos_shift": (None, [P, P, u32, P, P]),
-            "oracle_propagate": (i32, [P, P, u32, P, P, P, u32, i32, P, P, P]),
-            "oracle_collide": (None, [P, u32, P, P, u32, P]),
-            "oracle_grid_counts": (None, [P, P, u32, P]),
-            "oracle_regions": (None, [P, P, u32, P, P]),
-            "oracle_histogram": (None, [P, P, P, u32, i32, P, P, P, P, P]),
-            "oracle_scores": (None, [P, P, P, P, P, P]),
-            "oracle_f_accept": (ctypes.c_float, [P, u32, u32]),
-            "oracle_repeat_expand": (u64, [P, u32, P, u64]),
-            "oracle_plan": (i32, [P, P, u32, P, P, u64, i32, P, P, P, P, P, P, P, P, P, P]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _L = L
+        _L = load(LIB_PATH)
     return _L
 
 
@@ -162,8 +178,13 @@ def repeat_expand(weights, cap=None):
     return out[: min(cap, total)], int(total)
 
 
-def plan(cfg, obstacles, start, goal, seed, threads=0, want_tree=True):
-    """Whole planner; returns a dict with status, iterations, tree arrays, counters."""
+def plan(cfg, obstacles, start, goal, seed, threads=0, want_tree=True, want_trace=False, L=None):
+    """Whole planner; returns a dict with status, iterations, tree arrays,
+    counters.  want_trace adds "weights" (every node's repeat count in the last
+    repeat list built) and "trace" (per iteration: W, fan, M, admitted, tree
+    size after the append, fAccept, goal index or -1, next W).  L: an oracle
+    build from load() instead of the default one."""
+    L = lib() if L is None else L
     o, no = _obs(obstacles)
     N = int(cfg.max_tree)
     S = 12 if cfg.model == 3 else 6
@@ -181,15 +202,55 @@ def plan(cfg, obstacles, start, goal, seed, threads=0, want_tree=True):
     reg = np.zeros(N, np.int32) if want_tree else None
     v, t, c = (np.zeros(R1, np.int32) for _ in range(3))
     sc = np.zeros(R1, np.float32)
-    status = lib().oracle_plan(ctypes.byref(cfg), _p(o), no, _p(x0), _p(g), int(seed) & (2**64 - 1), int(threads),
-                               _p(info), _p(st), _p(ct), _p(steps), _p(par), _p(reg), _p(v), _p(t), _p(c), _p(sc))
+    wts = np.zeros(N, np.uint32) if want_trace else None
+    tr = np.zeros((max(int(cfg.max_iter), 1), 8), np.uint32) if want_trace else None
+    status = L.oracle_plan_ex(ctypes.byref(cfg), _p(o), no, _p(x0), _p(g), int(seed) & (2**64 - 1), int(threads),
+                              _p(info), _p(st), _p(ct), _p(steps), _p(par), _p(reg), _p(v), _p(t), _p(c), _p(sc),
+                              _p(wts), _p(tr))
     ts = int(info[1])
     out = {"status": int(status), "iterations": int(info[0]), "tree_size": ts,
            "goal_index": int(np.int32(info[2])), "expansions": int(info[3]) | (int(info[4]) << 32),
            "valid": v, "total": t, "covered": c, "scores": sc}
     if want_tree:
         out.update(states=st[:, :ts], controls=ct[:, :ts], steps=steps[:ts], parents=par[:ts], regions=reg[:ts])
+    if want_trace:
+        n = out["iterations"]
+        rows = tr[:n]
+        out["weights"] = wts[:ts]
+        out["trace"] = [{"W": int(r[0]), "fan": int(r[1]), "M": int(r[2]), "admitted": int(r[3]), "ts": int(r[4]),
+                         "f_accept": float(np.array([r[5]], np.uint32).view(np.float32)[0]),
+                         "goal": int(np.int32(r[6])), "next_W": int(r[7])} for r in rows]
     return out
+
+
+def fan_out(cfg, W, remaining, L=None):
+    L = lib() if L is None else L
+    fan = ctypes.c_uint32()
+    M = ctypes.c_uint64()
+    L.oracle_fan_out(ctypes.byref(cfg), int(W), int(remaining), ctypes.byref(fan), ctypes.byref(M))
+    return fan.value, M.value
+
+
+def admit_by_score(word, score, L=None):
+    return bool((lib() if L is None else L).oracle_admit_by_score(int(word), float(score)))
+
+
+def readmit(word, score, fa, L=None):
+    return bool((lib() if L is None else L).oracle_readmit(int(word), float(score), float(fa)))
+
+
+def quad_ode(cfg, x, u, use_float=False, L=None):
+    """The quadcopter's state derivative at x (12) under control u (4): the
+    float form the planner integrates, or double."""
+    x = np.ascontiguousarray(x, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.zeros(12, np.float64)
+    (lib() if L is None else L).oracle_quad_ode(ctypes.byref(cfg), _p(x), _p(u), _p(out), int(use_float))
+    return out
+
+
+def new_node_repeat(cfg, region_valid, L=None):
+    return int((lib() if L is None else L).oracle_new_node_repeat(ctypes.byref(cfg), int(region_valid)))
 
 
 # ---- learned control proposal (the build's own definition; see mlp_oracle.py)
