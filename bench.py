@@ -103,21 +103,8 @@ def cpu_baseline(k, env_seed, plans, seeds):
                       f"(OpenMP over expansions); p50 time-to-solution {statistics.median(ms):.0f} ms"}
 
 
-def serving_workload(cfg, n_queries=256, n_env=32, seed=0):
-    rng = np.random.default_rng(seed)
-    envs = []
-    for e in range(n_env):
-        if e % 2 == 0:
-            envs.append(kpax.generate_environment(2, 500 + e, int(rng.integers(100, 601)), cfg.ws_min, cfg.ws_max))
-        else:
-            envs.append(kpax.generate_environment(3, 500 + e, 2000, cfg.ws_min, cfg.ws_max))
-    qs = proposal.serving_queries(cfg, envs, n_queries, seed=7, min_dist=1.0, budget_range=(16384, 1 << 20),
-                                  max_dist=proposal.budget_distance)
-    return envs, qs
-
-
 def batched(cfg):
-    envs, qs = serving_workload(cfg)
+    envs, qs = proposal.serving_workload(cfg)
     T = sum(q.budget for q in qs) + 1024
     with kpax.KPAX(cfg, max_queries=len(qs), node_capacity=T) as p:
         for e, o in enumerate(envs):

@@ -103,7 +103,11 @@ typedef struct kpax_result {
     uint32_t control_dim;
     float*   rows;             /* n_rows x (S + C + 1): [state, control, duration]; the
                                   root row has zero control and duration (reference row
-                                  format, KPAX.cu:485-500)                               */
+                                  format, KPAX.cu:485-500).  Controls in the reference's
+                                  storage order: the Dubins airplane's as (yawRate,
+                                  pitchRate, a) (statePropagatorSpatialHash.cu:223-225),
+                                  while kpax_config.ctrl_min/max and kpax_get_tree keep
+                                  the draw order (a, yawRate, pitchRate)                 */
     int32_t* steps;            /* n_rows: integer steps of the edge into each node       */
     int32_t* node_index;       /* n_rows: tree index of each node                        */
     int32_t  goal_index;       /* tree index of the goal node (h_pathToGoal_), -1 if none */

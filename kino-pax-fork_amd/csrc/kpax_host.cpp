@@ -273,7 +273,7 @@ bool verify_path_fp64(const kpax_config& c, const float* obs, uint32_t n_obs, co
     {
         double x[12], u[4];
         for(int d = 0; d < S; ++d) x[d] = rows[(size_t)(r - 1) * D + d];
-        for(int d = 0; d < C; ++d) u[d] = rows[(size_t)r * D + S + d];
+        for(int d = 0; d < C; ++d) u[row_control(c.model, d)] = rows[(size_t)r * D + S + d];
         if(steps[r] < 1 || !edge_fp64(c, obs, n_obs, x, u, steps[r])) ok = false;
         for(int d = 0; d < S; ++d)
         {

@@ -111,6 +111,14 @@ struct PlanParams {
 
 int state_dim(int model);
 int control_dim(int model);
+
+// Row format of trajectories and dumped trees: [state, control, duration]
+// with the controls in the reference's storage order.  The planner keeps a
+// Dubins airplane's controls in their draw order (a, yawRate, pitchRate,
+// statePropagatorSpatialHash.cu:141-143, the order of kpax_config.ctrl_min);
+// the reference stores (yawRate, pitchRate, a) (x1[6..8], :223-225).  Row
+// control column k holds the planner's control row_control(model, k).
+inline int row_control(int model, int k) { return model == KPAX_MODEL_DUBINS_AIRPLANE ? (k + 1) % 3 : k; }
 std::string validate_config(const kpax_config& c);
 ModelParams make_model_params(const kpax_config& c);
 RegionParams make_region_params(const kpax_config& c);

@@ -893,9 +893,11 @@ __global__ __launch_bounds__(NT) void k_frontier(Ctl* ctl, const QueryDev* qd, B
 // from every thread that reached the goal, KPAX.cu:482-501).  Lane 0 follows
 // the parent chain; then the block gathers the rows.  Rows go out in
 // goal -> root order; the host reverses them.
+// Row control column k holds control ctl_of_col.{x,y,z,w}[k] (the reference's
+// storage order, row_control).
 __global__ __launch_bounds__(64) void k_path(Ctl* ctl, const QueryDev* qd, const uint32_t* slots, TreeView tree, int S,
                                              int C, uint32_t T, float dt, float* rows, int32_t* steps, int32_t* idx,
-                                             uint32_t cap)
+                                             uint32_t cap, int4 ctl_of_col)
 {
     __shared__ uint32_t s_len;
     const uint32_t q   = slots[blockIdx.x];
@@ -926,7 +928,11 @@ __global__ __launch_bounds__(64) void k_path(Ctl* ctl, const QueryDev* qd, const
         if(col < (uint32_t)S)
             v = tree.st[(size_t)col * T + node];
         else if(col < (uint32_t)(S + C))
-            v = tree.ct[(size_t)(col - S) * T + node];
+        {
+            const uint32_t k = col - S;
+            const int j      = k == 0 ? ctl_of_col.x : k == 1 ? ctl_of_col.y : k == 2 ? ctl_of_col.z : ctl_of_col.w;
+            v                = tree.ct[(size_t)j * T + node];
+        }
         else
             v = dt * (float)tree.steps[node];
         my_rows[e] = v;
@@ -1201,7 +1207,7 @@ void account_iteration(kpax_handle* h, int p)
 
 // Runs queries through the handle's slots until every one has finished.
 // Results land in out[i] exactly as a plan of query i alone would return.
-int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
+int run_queries_impl(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
 {
     const auto t0   = std::chrono::steady_clock::now();
     const uint32_t Q = h->Qmax;
@@ -1228,7 +1234,7 @@ int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
             out[i].status = KPAX_INVALID_ARG;
     }
     HIPC(hipSetDevice(h->cfg.device));
-    if(!h->graph_tried) capture_graphs(h);
+    if(h->use_graph && !h->graph_tried) capture_graphs(h);
     const bool prof  = h->profiling;
     const bool graph = h->use_graph && !prof && !h->trace && !h->iter_hook;
     NodeRanges ranges(h->T);
@@ -1300,8 +1306,10 @@ int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
         for(uint32_t k = 0; k < nd; ++k) h->h_path_slots[k] = done[k];
         HIPC(hipMemcpyAsync(h->d_path_slots, h->h_path_slots, nd * sizeof(uint32_t), hipMemcpyHostToDevice,
                             h->stream));
+        const int m = h->cfg.model;
         hipLaunchKernelGGL(k_path, dim3(nd), dim3(64), 0, h->stream, h->ctl, h->qd, h->d_path_slots, h->tree, S, C,
-                           h->T, h->cfg.dt, h->path_rows, h->path_steps, h->path_idx, h->path_cap);
+                           h->T, h->cfg.dt, h->path_rows, h->path_steps, h->path_idx, h->path_cap,
+                           make_int4(row_control(m, 0), row_control(m, 1), row_control(m, 2), row_control(m, 3)));
         HIPC(hipGetLastError());
         HIPC(hipMemcpyAsync(h->h_ctl, h->ctl, Q * sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
         const size_t pc = h->path_cap;
@@ -1465,6 +1473,33 @@ int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
         }
     }
     return KPAX_OK;
+}
+
+// run_queries_impl, and on a failure: the stream drained, every slot idle
+// again on the device (a slot left running would be advanced by the next
+// call's iterations over node ranges given to new queries), and the rows of
+// results already built released.
+int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
+{
+    const int rc = run_queries_impl(h, qs, n, out);
+    if(rc == KPAX_OK) return rc;
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipGetLastError();
+    const uint32_t Q = h->Qmax;
+    std::vector<Ctl> c0(Q);
+    for(Ctl& c : c0)
+    {
+        memset(&c, 0, sizeof(c));
+        c.cur.status = kNotStarted;
+        c.nxt.status = kNotStarted;
+        c.goal       = INT_MAX;
+    }
+    std::vector<int32_t> s0(Q + 1, kNotStarted);
+    s0[Q] = 0;
+    (void)hipMemcpy(h->ctl, c0.data(), Q * sizeof(Ctl), hipMemcpyHostToDevice);
+    (void)hipMemcpy(h->slot_status, s0.data(), (Q + 1) * sizeof(int32_t), hipMemcpyHostToDevice);
+    for(uint32_t i = 0; i < n; ++i) kpax_result_free(&out[i]);
+    return rc;
 }
 
 void drop_graphs(kpax_handle* h)
@@ -1836,7 +1871,7 @@ int kpax_plan_dump(kpax_handle* h, const float* start, const float* goal, uint64
         for(uint32_t i = 0; i < ts; ++i)
         {
             for(int d = 0; d < S; ++d) row[d] = st[(size_t)d * ts + i];
-            for(int d = 0; d < C; ++d) row[S + d] = ct[(size_t)d * ts + i];
+            for(int d = 0; d < C; ++d) row[S + d] = ct[(size_t)row_control(h->cfg.model, d) * ts + i];
             row[S + C] = h->cfg.dt * (float)steps[i];
             rows[i]    = join(row.begin(), row.end(), 6);
         }

@@ -272,6 +272,14 @@ class PlanResult:
         return self.rows[::-1].copy()
 
 
+def row_control_order(model):
+    """Planner control index held by each control column of a result row.
+    Rows keep the reference's storage order, (yawRate, pitchRate, a) for the
+    Dubins airplane (statePropagatorSpatialHash.cu:223-225); configs and
+    kpax_get_tree keep its draw order (a, yawRate, pitchRate, :141-143)."""
+    return [1, 2, 0] if model == MODEL_DUBINS_AIRPLANE else list(range(model_dims(model)[1]))
+
+
 def _plan_result(r, S, C):
     D = S + C + 1
     n = r.n_rows
@@ -461,6 +469,9 @@ class KPAX:
         for i, q in enumerate(queries):
             x0 = _f32(np.asarray(q.start, np.float32).ravel()[: self.S])
             g = _f32(np.asarray(q.goal, np.float32).ravel()[:3])
+            if x0.shape[0] != self.S or g.shape[0] != 3:
+                raise ValueError(f"query {i}: start needs {self.S} values and goal 3, got {x0.shape[0]} and "
+                                 f"{g.shape[0]}")
             keep += [x0, g]
             arr[i] = _Query(x0.ctypes.data, g.ctypes.data, int(q.seed) & (2**64 - 1), int(q.budget), int(q.env))
         return arr, keep

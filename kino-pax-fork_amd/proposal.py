@@ -75,6 +75,24 @@ def serving_queries(cfg, envs, n, seed, budget=0, min_dist=4.0, budget_range=Non
     return out
 
 
+def serving_workload(cfg, n_queries=256, n_env=32, seed=0):
+    """The batched-serving workload: n_env environments over the configured
+    workspace, alternately a forest of 100-600 columns and a maze of 2,000
+    boxes, and n_queries quadcopter queries with budgets log-uniform in
+    [16k, 1M] nodes, start and goal 1 m apart at least and at most
+    budget_distance(budget).  Returns (environments, queries)."""
+    rng = np.random.default_rng(seed)
+    envs = []
+    for e in range(n_env):
+        if e % 2 == 0:
+            envs.append(kpax.generate_environment(2, 500 + e, int(rng.integers(100, 601)), cfg.ws_min, cfg.ws_max))
+        else:
+            envs.append(kpax.generate_environment(3, 500 + e, 2000, cfg.ws_min, cfg.ws_max))
+    qs = serving_queries(cfg, envs, n_queries, seed=7, min_dist=1.0, budget_range=(16384, 1 << 20),
+                         max_dist=budget_distance)
+    return envs, qs
+
+
 def normalise_controls(cfg, u):
     lo = np.array(cfg.ctrl_min[:4], np.float32)
     hi = np.array(cfg.ctrl_max[:4], np.float32)

@@ -51,6 +51,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 #include <cstdio>
 #include <cstdlib>
@@ -210,10 +211,90 @@ bool segment_free(const T bbmin[3], const T bbmax[3], const float* obs, uint32_t
     return true;
 }
 
+
+// Uniform grid over the workspace: the reference's spatial hash (build
+// spatialHash.cu:9-54, query :145-194) as CSR lists without its 16-per-cell
+// cap, with per-axis bounds.  The float planner's collision check walks the
+// cells of the segment's box and applies the broad-phase verdict to every
+// obstacle listed there.  The verdict equals the brute-force one: an obstacle
+// that is not separated from the box shares a point with it, and the cell map
+// (x - min) * cells-per-metre, truncated and clamped, is monotone, so both
+// cell ranges hold that point's cell.
+struct Grid {
+    int D[3];
+    float mn[3], inv[3];
+    std::vector<int32_t> start, items;
+    Grid(const kpax_config& c, const float* obs, uint32_t n)
+    {
+        for(int d = 0; d < 3; ++d)
+        {
+            D[d]   = c.grid_dims[d];
+            mn[d]  = c.ws_min[d];
+            inv[d] = (float)D[d] / (c.ws_max[d] - c.ws_min[d]);
+        }
+        const size_t G = (size_t)D[0] * D[1] * D[2];
+        start.assign(G + 1, 0);
+        for(int pass = 0; pass < 2; ++pass)
+        {
+            std::vector<int32_t> fill;
+            if(pass)
+            {
+                for(size_t g = 0; g < G; ++g) start[g + 1] += start[g];
+                items.resize(start[G]);
+                fill.assign(start.begin(), start.end() - 1);
+            }
+            for(uint32_t i = 0; i < n; ++i)
+            {
+                int a[3], b[3];
+                range(obs + 6 * (size_t)i, obs + 6 * (size_t)i + 3, a, b);
+                for(int z = a[2]; z <= b[2]; ++z)
+                    for(int y = a[1]; y <= b[1]; ++y)
+                        for(int x = a[0]; x <= b[0]; ++x)
+                        {
+                            const size_t g = x + (size_t)D[0] * (y + (size_t)D[1] * z);
+                            if(pass)
+                                items[fill[g]++] = (int32_t)i;
+                            else
+                                start[g + 1]++;
+                        }
+            }
+        }
+    }
+    void range(const float* lo, const float* hi, int a[3], int b[3]) const
+    {
+        for(int d = 0; d < 3; ++d)
+        {
+            a[d] = clamp_index((lo[d] - mn[d]) * inv[d], D[d]);
+            b[d] = clamp_index((hi[d] - mn[d]) * inv[d], D[d]);
+        }
+    }
+    bool free(const float bbmin[3], const float bbmax[3], const float* obs) const
+    {
+        int a[3], b[3];
+        range(bbmin, bbmax, a, b);
+        for(int z = a[2]; z <= b[2]; ++z)
+            for(int y = a[1]; y <= b[1]; ++y)
+                for(int x = a[0]; x <= b[0]; ++x)
+                {
+                    const size_t g = x + (size_t)D[0] * (y + (size_t)D[1] * z);
+                    for(int32_t k = start[g]; k < start[g + 1]; ++k)
+                    {
+                        const float* o = obs + 6 * (size_t)items[k];
+                        bool separated = false;
+                        for(int d = 0; d < 3 && !separated; ++d)
+                            separated = bbmax[d] <= o[d] || o[3 + d] <= bbmin[d];
+                        if(!separated) return false;
+                    }
+                }
+        return true;
+    }
+};
+
 struct Env {
     const kpax_config* cfg;
     const float* obs;
     uint32_t n_obs;
+    const Grid* grid = nullptr;  // float path only; nullptr = brute force over all obstacles
 };
 
 template<class T>
@@ -242,6 +323,8 @@ bool step_checks_ws_and_collision(const Env& e, const T p0[3], const T p1[3])
             bbmax[d] = p1[d];
         }
     }
+    if constexpr(std::is_same<T, float>::value)
+        if(e.grid) return e.grid->free(bbmin, bbmax, e.obs);
     return segment_free<T>(bbmin, bbmax, e.obs, e.n_obs);
 }
 
@@ -610,10 +693,13 @@ struct Planner {
     std::vector<uint32_t> weight;  // repeat count of every node in the last repeat list built
     float score_total = 0.0f;
 
+    Grid grid;
+
     Planner(const kpax_config& cfg, const float* obs, uint32_t n_obs)
         : c(cfg), env{&cfg, obs, n_obs}, R(cfg), S(state_dim(cfg.model)), C(control_dim(cfg.model)),
-          N(cfg.max_tree)
+          N(cfg.max_tree), grid(cfg, obs, n_obs)
     {
+        env.grid = &grid;
         st.assign((size_t)N * S, 0.0f);
         ct.assign((size_t)N * C, 0.0f);
         steps.assign(N, 0);
@@ -872,6 +958,24 @@ void oracle_collide(const float* obs, uint32_t n_obs, const float* bbmin_soa, co
             hi[d] = bbmax_soa[(size_t)d * n + i];
         }
         free_out[i] = segment_free<float>(lo, hi, obs, n_obs) ? 1 : 0;
+    }
+}
+
+// Verdicts of n segment boxes through the oracle's grid (the planner's
+// collision check), for comparison with the brute force above.
+void oracle_collide_grid(const kpax_config* cfg, const float* obs, uint32_t n_obs, const float* bbmin_soa,
+                         const float* bbmax_soa, uint32_t n, uint8_t* free_out)
+{
+    const Grid g(*cfg, obs, n_obs);
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        float lo[3], hi[3];
+        for(int d = 0; d < 3; ++d)
+        {
+            lo[d] = bbmin_soa[(size_t)d * n + i];
+            hi[d] = bbmax_soa[(size_t)d * n + i];
+        }
+        free_out[i] = g.free(lo, hi, obs) ? 1 : 0;
     }
 }
 

@@ -23,6 +23,7 @@ def _sigs():
         "oracle_sincos_shift": (None, [P, P, u32, P, P]),
         "oracle_propagate": (i32, [P, P, u32, P, P, P, u32, i32, P, P, P]),
         "oracle_collide": (None, [P, u32, P, P, u32, P]),
+        "oracle_collide_grid": (None, [P, P, u32, P, P, u32, P]),
         "oracle_grid_counts": (None, [P, P, u32, P]),
         "oracle_regions": (None, [P, P, u32, P, P]),
         "oracle_histogram": (None, [P, P, P, u32, i32, P, P, P, P, P]),
@@ -119,6 +120,17 @@ def collide(obstacles, bbmin, bbmax):
     o, no = _obs(obstacles)
     out = np.zeros(n, np.uint8)
     lib().oracle_collide(_p(o), no, _p(bbmin), _p(bbmax), n, _p(out))
+    return out.astype(bool)
+
+
+def collide_grid(cfg, obstacles, bbmin, bbmax):
+    """Verdicts through the oracle planner's uniform grid (cfg.grid_dims)."""
+    bbmin = np.ascontiguousarray(bbmin, np.float32)
+    n = bbmin.shape[1]
+    bbmax = np.ascontiguousarray(bbmax, np.float32).reshape(3, n)
+    o, no = _obs(obstacles)
+    out = np.zeros(n, np.uint8)
+    lib().oracle_collide_grid(ctypes.byref(cfg), _p(o), no, _p(bbmin), _p(bbmax), n, _p(out))
     return out.astype(bool)
 
 

@@ -45,7 +45,11 @@ struct uint4 {
 struct float4 {
     float x, y, z, w;
 };
+struct int4 {
+    int32_t x, y, z, w;
+};
 inline uint4 make_uint4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return {a, b, c, d}; }
+inline int4 make_int4(int32_t a, int32_t b, int32_t c, int32_t d) { return {a, b, c, d}; }
 inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
 
 inline thread_local emu_idx threadIdx{0, 0, 0};

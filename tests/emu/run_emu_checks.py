@@ -16,7 +16,7 @@ import numpy as np  # noqa: E402
 
 import kpax  # noqa: E402
 import oracle  # noqa: E402
-from workloads import random_states, workload  # noqa: E402
+from workloads import oracle_path_rows, random_states, workload  # noqa: E402
 
 
 T0 = time.perf_counter()
@@ -185,22 +185,6 @@ def mlp_checks():
         check(np.all(np.isfinite(losses)), f"fit runs: losses {losses}")
 
 
-def oracle_path(o, cfg):
-    """Root -> goal rows [state, control, dt * steps] of an oracle plan."""
-    j, nodes = o["goal_index"], []
-    while j != -1:
-        nodes.append(j)
-        j = int(o["parents"][j])
-    nodes.reverse()
-    S, C = cfg.S, cfg.C
-    rows = np.zeros((len(nodes), S + C + 1), np.float32)
-    for r, j in enumerate(nodes):
-        rows[r, :S] = o["states"][:, j]
-        rows[r, S:S + C] = o["controls"][:, j]
-        rows[r, S + C] = np.float32(cfg.dt) * np.float32(o["steps"][j])
-    return rows
-
-
 def batch_checks():
     """Uneven budgets over two environments on three slots: queries wait for
     node ranges, reuse the ranges and slots of finished ones, and each result
@@ -235,7 +219,7 @@ def batch_checks():
         same = (r.status == o["status"] and r.iterations == o["iterations"] and r.tree_size == o["tree_size"]
                 and r.goal_index == o["goal_index"] and r.expansions == o["expansions"])
         if same and r.status == kpax.OK:
-            same = np.array_equal(r.rows.view(np.uint32), oracle_path(o, c).view(np.uint32)) and r.verified == 1
+            same = np.array_equal(r.rows.view(np.uint32), oracle_path_rows(o, c).view(np.uint32)) and r.verified == 1
         check(same, f"batch query {i} (budget {q.budget}, env {q.env}): status {r.status} iterations "
                     f"{r.iterations} tree {r.tree_size} vs oracle {o['status']} {o['iterations']} {o['tree_size']}")
     check(any(r.status == kpax.OK for r in res) and any(r.status != kpax.OK for r in res),

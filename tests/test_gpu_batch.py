@@ -9,7 +9,7 @@ GPU plans for larger ones.
 import numpy as np
 import pytest
 
-from workloads import workload
+from workloads import oracle_path_rows, workload
 
 pytestmark = pytest.mark.gpu
 
@@ -24,21 +24,6 @@ def _same(r, o):
             and r.goal_index == o.goal_index and r.expansions == o.expansions
             and r.integrated_steps == o.integrated_steps and r.rows.shape == o.rows.shape
             and np.array_equal(r.rows.view(np.uint32), o.rows.view(np.uint32)) and np.array_equal(r.steps, o.steps))
-
-
-def _oracle_rows(o, cfg):
-    j, nodes = o["goal_index"], []
-    while j != -1:
-        nodes.append(j)
-        j = int(o["parents"][j])
-    nodes.reverse()
-    S, C = cfg.S, cfg.C
-    rows = np.zeros((len(nodes), S + C + 1), np.float32)
-    for r, j in enumerate(nodes):
-        rows[r, :S] = o["states"][:, j]
-        rows[r, S:S + C] = o["controls"][:, j]
-        rows[r, S + C] = np.float32(cfg.dt) * np.float32(o["steps"][j])
-    return rows
 
 
 def test_batch_matches_oracle_per_query(kpax, oracle):
@@ -67,7 +52,7 @@ def test_batch_matches_oracle_per_query(kpax, oracle):
             (o["status"], o["iterations"], o["tree_size"], o["expansions"])
         assert r.goal_index == o["goal_index"]
         if r.status == kpax.OK:
-            assert np.array_equal(r.rows.view(np.uint32), _oracle_rows(o, c).view(np.uint32))
+            assert np.array_equal(r.rows.view(np.uint32), oracle_path_rows(o, c).view(np.uint32))
             assert r.verified == 1
         statuses.add(r.status)
     assert kpax.OK in statuses and kpax.BUDGET_EXHAUSTED in statuses
@@ -186,3 +171,50 @@ def test_launch_knobs_leave_results_unchanged(kpax):
         assert _same(ref_one, one)
         for k in ref_tree:
             assert np.array_equal(ref_tree[k].view(np.uint8), tree[k].view(np.uint8)), (shape, k)
+
+
+def test_serving_batch_at_its_shape(kpax, oracle):
+    """Batched serving at its stated shape: 256 quadcopter queries over 32
+    environments (16 forests of 100-600 columns, 16 mazes of 2,000 boxes),
+    budgets log-uniform in [16k, 1M] nodes, in one kpax_plan_batch call.
+    Every query returns what it returns planned alone; eight of them (per
+    environment kind the two largest budgets and two budgets in [2^17, 2^19])
+    also equal the oracle's plan, path rows included."""
+    import proposal
+    cfg = workload(2)[0]
+    envs, qs = proposal.serving_workload(cfg)
+    assert len(qs) == 256 and len(envs) == 32
+    assert sum(o.shape[0] == 2000 for o in envs) == 16
+    assert max(q.budget for q in qs) > (1 << 19) and min(q.budget for q in qs) < (1 << 15)
+    T = sum(q.budget for q in qs) + 1024
+    print(f"serving batch: 256 queries, 32 environments, {T} nodes", flush=True)
+    with kpax.KPAX(cfg, max_queries=len(qs), node_capacity=T) as p:
+        for e, o in enumerate(envs):
+            p.set_environment(e, o)
+        res = p.plan_batch(qs)
+    print("single plans of the same queries", flush=True)
+    with kpax.KPAX(cfg, max_queries=1, node_capacity=1 << 20) as s:
+        for e, o in enumerate(envs):
+            s.set_environment(e, o)
+        seq = [s.plan_batch([q])[0] for q in qs]
+    diff = [i for i, (a, b) in enumerate(zip(res, seq)) if not _same(a, b)]
+    assert not diff, diff
+    found = [r.status == kpax.OK and r.verified == 1 for r in res]
+    assert sum(found) / len(found) >= 0.95, sum(found)
+    assert any(f for f, q in zip(found, qs) if q.env % 2 == 1)  # mazes solved too
+    order = sorted(range(len(qs)), key=lambda i: -qs[i].budget)
+    mid = [i for i in range(len(qs)) if (1 << 17) <= qs[i].budget <= (1 << 19)]
+    picks = []
+    for kind in (0, 1):
+        picks += [i for i in order if qs[i].env % 2 == kind][:2] + [i for i in mid if qs[i].env % 2 == kind][:2]
+    for i in picks:
+        q, r = qs[i], res[i]
+        c = cfg.copy()
+        c.max_tree = q.budget
+        o = oracle.plan(c, envs[q.env], q.start, q.goal, seed=q.seed, threads=16)
+        print(f"  query {i} (env {q.env}, budget {q.budget}): gpu {r.status} it {r.iterations} ts {r.tree_size}; "
+              f"oracle {o['status']} it {o['iterations']} ts {o['tree_size']}", flush=True)
+        assert (r.status, r.iterations, r.tree_size, r.expansions, r.goal_index) == \
+            (o["status"], o["iterations"], o["tree_size"], o["expansions"], o["goal_index"])
+        if r.status == kpax.OK:
+            assert np.array_equal(r.rows.view(np.uint32), oracle_path_rows(o, c).view(np.uint32))

@@ -8,7 +8,7 @@ oracle's FP64 path (|x32 - x64| <= 1e-5 * max(1, |x64|) per component).
 import numpy as np
 import pytest
 
-from workloads import random_states, workload
+from workloads import oracle_path_rows, random_states, workload
 
 pytestmark = pytest.mark.gpu
 
@@ -231,6 +231,49 @@ def test_plan_full_size_tree_bit_exact_vs_oracle(kpax, oracle):
     _compare_plans(kpax, oracle, 2, 5)
 
 
+def test_plan_dubins_full_size_tree_bit_exact_vs_oracle(kpax, oracle):
+    """The Dubins maze at its full size (2,000 boxes with a 0.1-wide passage,
+    128^3 grid, 2^21-node cap, default iteration limit): this plan passes the
+    passage and reaches the goal after ~140 iterations and 144 M expansions;
+    tree, regions and scores equal the oracle's bit for bit."""
+    cfg, obs, start, goal = workload(3, seed=0)
+    with kpax.KPAX(cfg) as p:
+        r = p.plan(start, goal, obs, seed=5)
+        tree = p.tree()
+        reg = p.regions()
+    print(f"full-size Dubins plan: gpu status {r.status} it {r.iterations} ts {r.tree_size}", flush=True)
+    o = oracle.plan(cfg, obs, start, goal, seed=5, threads=16)
+    print(f"  oracle status {o['status']} it {o['iterations']} ts {o['tree_size']}", flush=True)
+    assert (r.status, r.iterations, r.tree_size, r.expansions) == (o["status"], o["iterations"], o["tree_size"],
+                                                                   o["expansions"])
+    assert r.status == kpax.OK and r.goal_index == o["goal_index"] and r.verified == 1
+    assert np.array_equal(tree["parents"], o["parents"])
+    assert np.array_equal(tree["states"].view(np.uint32), o["states"].view(np.uint32))
+    assert np.array_equal(tree["controls"].view(np.uint32), o["controls"].view(np.uint32))
+    for key in ("valid", "total", "covered"):
+        assert np.array_equal(reg[key], o[key]), key
+    assert np.array_equal(reg["scores"].view(np.uint32), o["scores"].view(np.uint32))
+    assert np.array_equal(r.rows.view(np.uint32), oracle_path_rows(o, cfg).view(np.uint32))
+
+
+def test_dubins_rows_and_dump_in_reference_control_order(kpax, oracle, tmp_path):
+    """Result rows and planBench sample rows carry the Dubins controls as
+    (yawRate, pitchRate, a), x1[6..8] of statePropagatorSpatialHash.cu:223-225."""
+    cfg, obs, start, _ = workload(3, seed=0, max_tree=1 << 14, max_iter=30)
+    goal = np.array([0.22, 0.5, 0.5], np.float32)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        r = p.plan_dump(start, goal, tmp_path, seed=3, bench_itr=0)
+        tree = p.tree()
+    o = oracle.plan(cfg, obs, start, goal, seed=3)
+    assert r.status == kpax.OK and r.verified == 1
+    assert np.array_equal(r.rows.view(np.uint32), oracle_path_rows(o, cfg).view(np.uint32))
+    assert np.array_equal(r.rows[1:, 6:9], tree["controls"][[1, 2, 0]][:, r.node_index[1:]].T)
+    last = np.loadtxt(tmp_path / "Samples" / "Samples0" / f"samples{r.iterations}.csv", delimiter=",", ndmin=2)
+    assert last.shape == (r.tree_size, 10)
+    assert np.allclose(last[:, 6:9], tree["controls"][[1, 2, 0]].T, atol=1e-6)
+
+
 def test_plan_quadcopter_full_size_finds_verified_path(kpax):
     cfg, obs, start, goal = workload(2, seed=0)
     with kpax.KPAX(cfg) as p:
@@ -258,3 +301,21 @@ def test_plan_dump_matches_the_tree(kpax, tmp_path):
     assert last.shape == (r.tree_size, 17) and np.array_equal(par, tree["parents"])
     assert np.allclose(last[:, :12], tree["states"].T, atol=5e-6 * (1 + np.abs(tree["states"].T)))
     assert len(np.loadtxt(d / "TreeSize" / "TreeSize0" / "treeSize.csv", ndmin=1)) == r.iterations
+
+
+def test_failed_run_leaves_the_handle_usable(kpax, tmp_path):
+    """A plan stopped by an error mid-run (the planBench dump cannot write its
+    first samples file, forced from the host) leaves no slot running: the
+    handle's next plan equals the same plan on a fresh handle."""
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=1 << 16, max_iter=30)
+    (tmp_path / "Samples" / "Samples0" / "samples1.csv").mkdir(parents=True)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        with pytest.raises(kpax.KpaxError):
+            p.plan_dump(start, goal, tmp_path, seed=8, bench_itr=0)
+        r = p.plan(start, goal, seed=9)
+    with kpax.KPAX(cfg) as q:
+        fresh = q.plan(start, goal, obs, seed=9)
+    assert (r.status, r.iterations, r.tree_size, r.expansions) == (fresh.status, fresh.iterations,
+                                                                   fresh.tree_size, fresh.expansions)
+    assert np.array_equal(r.rows.view(np.uint32), fresh.rows.view(np.uint32))

@@ -191,3 +191,32 @@ def test_invalid_config_rejected(kpax):
     c.steps_min = 0
     with pytest.raises(kpax.KpaxError):
         kpax.KPAX(c)
+
+
+def test_dubins_rows_use_the_reference_control_order(kpax, oracle):
+    """Result rows hold the Dubins airplane's controls as (yawRate, pitchRate,
+    a), the reference's x1[6..8] (statePropagatorSpatialHash.cu:223-225).  An
+    oracle path written in that order passes the FP64 re-verifier; the same
+    path with the controls in draw order (a, yawRate, pitchRate) does not."""
+    from workloads import oracle_path_rows, workload
+    assert kpax.row_control_order(kpax.MODEL_DUBINS_AIRPLANE) == [1, 2, 0]
+    assert kpax.row_control_order(kpax.MODEL_QUADCOPTER) == [0, 1, 2, 3]
+    cfg, obs, start, _ = workload(3, seed=0, max_tree=1 << 14, max_iter=30)
+    goal = np.array([0.22, 0.5, 0.5], np.float32)
+    o = oracle.plan(cfg, obs, start, goal, seed=3)
+    assert o["status"] == kpax.OK
+    rows = oracle_path_rows(o, cfg)
+    j, nodes = o["goal_index"], []
+    while j != -1:
+        nodes.append(j)
+        j = int(o["parents"][j])
+    steps = o["steps"][nodes[::-1]]
+    assert rows.shape[0] >= 3
+    S = cfg.S
+    assert np.array_equal(rows[1:, S:S + 3], o["controls"][[1, 2, 0]][:, nodes[::-1][1:]].T)
+    ok, dev = kpax.verify_path(cfg, obs, rows, steps, goal)
+    assert ok and dev < 1e-4, dev
+    draw_order = rows.copy()
+    draw_order[:, S:S + 3] = rows[:, S:S + 3][:, [2, 0, 1]]
+    ok2, dev2 = kpax.verify_path(cfg, obs, draw_order, steps, goal)
+    assert not ok2 and dev2 > 1e-3, dev2

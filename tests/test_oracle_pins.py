@@ -271,6 +271,29 @@ def test_collide_vectorised_brute_force(oracle, rng):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("k", [2, 3])
+def test_grid_verdicts_equal_brute_force(kpax, oracle, rng, k):
+    """The oracle planner's grid (the reference's spatial hash as CSR) gives
+    the brute-force verdict for random segments, segments touching obstacle
+    faces (free), zero-length segments and segments beyond the grid."""
+    cfg, obs, _, _ = workload(k, seed=1)
+    lo_ws = np.array(cfg.ws_min, np.float32)[:, None]
+    ext = (np.array(cfg.ws_max, np.float32) - np.array(cfg.ws_min, np.float32))[:, None]
+    n = 40000
+    a = (lo_ws + ext * rng.uniform(-0.1, 1.1, (3, n))).astype(np.float32)
+    b = (a + ext * rng.uniform(-0.02, 0.02, (3, n))).astype(np.float32)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    # boxes ending exactly on an obstacle's face, and points
+    pick = rng.integers(0, obs.shape[0], 2000)
+    hi[0, :2000] = obs[pick, 0]
+    lo[0, :2000] = np.minimum(lo[0, :2000], hi[0, :2000])
+    lo[:, 2000:4000] = hi[:, 2000:4000]
+    got = oracle.collide_grid(cfg, obs, lo, hi)
+    want = oracle.collide(obs, lo, hi)
+    assert np.array_equal(got, want)
+    assert 0.05 < want.mean() < 0.999  # both verdicts occur
+
+
 def test_grid_counts_cover_every_obstacle_point(kpax, oracle, rng):
     cfg, obs, _, _ = workload(2, seed=1)
     counts = oracle.grid_counts(cfg, obs)

@@ -55,3 +55,21 @@ def random_states(cfg, n, rng, margin=0.05):
     u = (cmin + (cmax - cmin) * rng.uniform(0, 1, (C, n))).astype(np.float32)
     steps = rng.integers(cfg.steps_min, cfg.steps_max + 1, n).astype(np.int32)
     return x0.astype(np.float32), u, steps
+
+
+def oracle_path_rows(o, cfg):
+    """Root -> goal rows [state, control, dt * steps] of an oracle plan, in the
+    result's row format (controls in the reference's storage order)."""
+    j, nodes = o["goal_index"], []
+    while j != -1:
+        nodes.append(j)
+        j = int(o["parents"][j])
+    nodes.reverse()
+    S, C = cfg.S, cfg.C
+    order = kpax.row_control_order(cfg.model)
+    rows = np.zeros((len(nodes), S + C + 1), np.float32)
+    for r, j in enumerate(nodes):
+        rows[r, :S] = o["states"][:, j]
+        rows[r, S:S + C] = o["controls"][order, j]
+        rows[r, S + C] = np.float32(cfg.dt) * np.float32(o["steps"][j])
+    return rows
