@@ -23,7 +23,8 @@ Also measured, each on the same GPU and stated beside the headline:
            with and without the budget policy (uniform sampling below 2^17)
 Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-proposal] [--batched-only] [--proposal-only]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-proposal] [--no-others]
+                    [--cpu-seconds S] [--batched-only] [--proposal-only]
 """
 import argparse
 import json
@@ -64,9 +65,20 @@ def bytes_per_expansion(S, C):
 MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 237 + 95
 
 
-# FP32 operations per integration step as written in the kernel (add, sub,
-# mul, div, sqrt, sin, cos = 1 each; compares not counted).
-FLOP_PER_STEP = {kpax.MODEL_DOUBLE_INTEGRATOR: 60, kpax.MODEL_DUBINS_AIRPLANE: 67, kpax.MODEL_QUADCOPTER: 624}
+# Algorithmic FP32 operations per integration step: the reference's step
+# formula counted as written (add, sub, mul, div and negation 1, a fused
+# multiply-add 2, each sin/cos/tan/sqrt call 1; compile-time constants such
+# as STEP_SIZE / 6.0f and (IY - IZ) / IX folded; compares not counted):
+#   double integrator  statePropagatorSpatialHash.cu:92-97: 3 x (12 + 8) = 60
+#   Dubins airplane    :160-178: x 32 + y 32 + z 22 arithmetic, yaw, pitch 2 + 2,
+#                      v 6, plus 20 sin/cos calls = 116
+#   quadcopter         ode (statePropagator.cu:341-369) 109 arithmetic + 18 for
+#                      the stage inputs + 44 calls = 171 per evaluation, 4 per
+#                      step, plus the RK4 combine (statePropagatorSpatialHash.cu:
+#                      262-265) 7 x 12 = 768
+# The kernel itself does less arithmetic per step than the formula (shared
+# sin/cos, angle-shifted stage sines, fused multiply-adds).
+FLOP_PER_STEP = {kpax.MODEL_DOUBLE_INTEGRATOR: 60, kpax.MODEL_DUBINS_AIRPLANE: 116, kpax.MODEL_QUADCOPTER: 768}
 
 
 MODEL_NAME = {kpax.MODEL_DOUBLE_INTEGRATOR: ("6D double integrator", "double_integrator"),
@@ -84,23 +96,77 @@ def describe(k, cfg, n_obs):
             f"2^{int(cfg.max_tree).bit_length() - 1}-node tree cap, {l1}^3 regions x {l2}^3 sub-regions, {g}^3 grid")
 
 
-def cpu_baseline(k, env_seed, plans, seeds):
+def host_cpu():
+    """CPU model, online CPUs and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(k, env_seed, seeds, budget_s=20.0, min_plans=10, max_plans=50):
+    """The CPU restatement of the same planner (OpenMP over expansions) on the
+    host's cores: seeded full plans of the bench workload until `budget_s`
+    seconds have passed and at least `min_plans` ran (at most `max_plans`).
+    The box's CPU share is 16 cores per GPU, so at most 16 threads run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # checker only: times the CPU restatement of the same planner
-    threads = min(os.cpu_count() or 1, 16)
+    model, nproc, allowed = host_cpu()
+    threads = max(1, min(allowed, 16))
     cfg, obs, start, goal = workload(k, seed=env_seed)
-    exp = 0
-    ms = []
+    exp, ms, found = 0, [], 0
     t0 = time.perf_counter()
-    for s in seeds[:plans]:
+    for s in seeds:
         t = time.perf_counter()
         r = oracle.plan(cfg, obs, start, goal, seed=s, threads=threads, want_tree=False)
         ms.append((time.perf_counter() - t) * 1e3)
         exp += r["expansions"]
+        found += r["status"] == kpax.OK
+        el = time.perf_counter() - t0
+        if len(ms) >= max_plans or (len(ms) >= min_plans and el >= budget_s):
+            break
     dt = time.perf_counter() - t0
     return {"value": exp / dt, "unit": "node-expansions/s", "cores": threads, "kind": "port",
-            "sample": f"{plans} full plan(s) of the same workload (seeds {seeds[:plans]}) on {threads} host threads "
-                      f"(OpenMP over expansions); p50 time-to-solution {statistics.median(ms):.0f} ms"}
+            "cpu_model": model, "nproc": nproc, "cpus_allowed": allowed,
+            "plans": len(ms), "success_rate": found / len(ms),
+            "p50_time_to_solution_ms": statistics.median(ms), "p99_time_to_solution_ms": float(np.percentile(ms, 99)),
+            "plans_per_sec": len(ms) / dt,
+            "sample": f"{len(ms)} full seeded plans of the same workload (seeds {seeds[0]}..{seeds[len(ms) - 1]}), "
+                      f"{dt:.1f} s, {threads} OpenMP threads over expansions on {model} "
+                      f"(nproc {nproc}, {allowed} CPUs allowed); same iteration semantics and draws as the GPU "
+                      f"planner, collision through the same kind of uniform grid"}
+
+
+def other_workload(k, plans=20, env_seed=0, launch=None, max_iter=None):
+    """Success, time-to-solution and expansion rate of workload k (single-query form)."""
+    cfg, obs, start, goal = workload(k, seed=env_seed, max_iter=max_iter)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        if launch:
+            p.set_launch(**launch)
+        for s in range(3):
+            p.plan(start, goal, seed=5000 + s)
+        res = []
+        t0 = time.perf_counter()
+        for s in range(plans):
+            res.append(p.plan(start, goal, seed=6000 + s))
+        wall = time.perf_counter() - t0
+    ok = [r for r in res if r.status == kpax.OK and r.verified == 1]
+    status = {name: sum(r.status == code for r in res) for name, code in
+              (("found", kpax.OK), ("iteration_limit", kpax.NOT_FOUND), ("tree_cap", kpax.BUDGET_EXHAUSTED))}
+    return {"workload": describe(k, cfg, obs.shape[0]), "plans": plans, "success_rate": len(ok) / plans,
+            "outcomes": status,
+            "p50_time_to_solution_ms": statistics.median(r.plan_ms for r in ok) if ok else None,
+            "p50_plan_ms_all": statistics.median(r.plan_ms for r in res),
+            "plans_per_sec": plans / wall, "node_expansions_per_sec": sum(r.expansions for r in res) / wall,
+            "mean_iterations": statistics.mean(r.iterations for r in res),
+            "mean_tree_size": statistics.mean(r.tree_size for r in res)}
 
 
 def batched(cfg):
@@ -209,8 +275,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", type=int, default=2)
     ap.add_argument("--env-seed", type=int, default=0)
-    ap.add_argument("--cpu-plans", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="CPU baseline: seeded plans until this many seconds (at least 10 plans)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-others", action="store_true", help="skip the double-integrator and Dubins sections")
+    ap.add_argument("--launch", default="", metavar="K=V,...",
+                    help="launch-shape overrides for the single-query planners (kpax_launch fields)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-proposal", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
@@ -244,6 +314,9 @@ def main():
         return
     planner = kpax.KPAX(cfg)
     planner.set_obstacles(obs)
+    overrides = {k: int(v) for k, v in (kv.split("=") for kv in args.launch.split(",") if kv)}
+    if overrides:
+        planner.set_launch(**overrides)
     seeds = [1000 + i for i in range(args.warmup + args.steps)]
     for s in seeds[: args.warmup]:
         planner.plan(start, goal, seed=s)
@@ -262,6 +335,15 @@ def main():
     for s in seeds[args.warmup:]:
         planner.plan(start, goal, seed=s)
     st = planner.stats()
+    # time-to-solution with the environment's grid built inside the timed call
+    # (the reference rebuilds its grid inside plan(), KPAX.cu:306)
+    planner.set_profiling(False)
+    with_grid = []
+    for s in seeds[args.warmup:args.warmup + 10]:
+        t = time.perf_counter()
+        planner.set_obstacles(obs)
+        r = planner.plan(start, goal, seed=s)
+        with_grid.append((time.perf_counter() - t) * 1e3)
     planner.close()
 
     ok = [r for r in results if r.status == kpax.OK and r.verified == 1]
@@ -279,6 +361,25 @@ def main():
     achieved_gbs = per_launch_exp * bytes_per_expansion(S, C) / (per_launch_ms * 1e-3) / 1e9
     flops = st["integrated_steps"] * FLOP_PER_STEP[cfg.model]
     achieved_tflops = flops / (exp_ms * 1e-3) / 1e12
+    kernel = f"k_expand<{MODEL_NAME[cfg.model][1]}>"
+    compute = {"bound": "valu", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "frac": achieved_tflops / FP32_PEAK_TFLOPS,
+               "traffic": (per_launch_exp * MEASURED_EXPAND_TRAFFIC_PER_EXPANSION if args.workload == 2 else None),
+               "kernel": kernel, "ceiling": "FP32 vector (VALU) issue: MI355X's published fp32 157.3 TFLOP/s",
+               "flop_per_step": FLOP_PER_STEP[cfg.model],
+               "flop_convention": "reference step formula as written: add/sub/mul/div/neg 1, fma 2, "
+                                  "sin/cos/tan/sqrt 1",
+               "flop_per_launch": flops / launches, "avg_launch_ms": per_launch_ms}
+    memory = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+              "frac": achieved_gbs / HBM_PEAK_GBS,
+              "traffic": (per_launch_exp * MEASURED_EXPAND_TRAFFIC_PER_EXPANSION if args.workload == 2 else None),
+              "traffic_basis": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE bytes per expansion x expansions "
+                               "per launch (measured on workload 2)",
+              "kernel": kernel, "bytes_per_expansion": bytes_per_expansion(S, C),
+              "expansions_per_launch": per_launch_exp, "avg_launch_ms": per_launch_ms}
+    # the quadcopter forest's binding ceiling is FP32 issue (about 80 FLOP per
+    # algorithmic byte); the double integrator's is bytes
+    valu_bound = cfg.model != kpax.MODEL_DOUBLE_INTEGRATOR
 
     line = {
         "metric": METRIC,
@@ -294,9 +395,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": describe(args.workload, cfg, obs.shape[0]),
-                   "plans_per_step": 1, "env_seed": args.env_seed, "max_tree": int(cfg.max_tree)},
+                   "plans_per_step": 1, "env_seed": args.env_seed, "max_tree": int(cfg.max_tree),
+                   "max_iter": int(cfg.max_iter), "start": [float(x) for x in start], "goal": [float(x) for x in goal],
+                   "goal_radius": float(cfg.goal_radius), "steps_per_edge": [int(cfg.steps_min), int(cfg.steps_max)],
+                   "velocity_box": [float(cfg.vel_min), float(cfg.vel_max)],
+                   "controls": [[float(a), float(b)] for a, b in zip(cfg.ctrl_min[:C], cfg.ctrl_max[:C])]},
         "p50_time_to_first_solution_ms": p50,
         "p99_time_to_first_solution_ms": p99,
+        "p50_time_to_first_solution_ms_incl_grid_build": statistics.median(with_grid),
         "plans_per_sec": len(results) / wall,
         "success_rate": success,
         "mean_iterations": statistics.mean(r.iterations for r in results),
@@ -304,24 +410,19 @@ def main():
         "integrated_steps": steps_int,
         "graph_launches": graph_launches,
         "phase_ms_per_plan": {k: v / max(1, st["plans"]) for k, v in st["phase_ms"].items()},
-        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS,
-                     "traffic": (per_launch_exp * MEASURED_EXPAND_TRAFFIC_PER_EXPANSION
-                                 if args.workload == 2 else None),
-                     "traffic_basis": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE bytes per expansion x expansions "
-                                      "per launch (measured on workload 2)",
-                     "kernel": f"k_expand<{MODEL_NAME[cfg.model][1]}>", "bytes_per_expansion": bytes_per_expansion(S, C),
-                     "expansions_per_launch": per_launch_exp, "avg_launch_ms": per_launch_ms},
-        "roofline_valu": {"bound": "valu_fp32", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS,
-                          "unit": "TFLOP/s", "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-                          "flop_per_step": FLOP_PER_STEP[cfg.model]},
+        "roofline": compute if valu_bound else memory,
+        "roofline_hbm" if valu_bound else "roofline_valu": memory if valu_bound else compute,
     }
+    if not args.no_others:
+        line["other_workloads"] = {"double_integrator": other_workload(1, launch=overrides),
+                                   "dubins_maze": other_workload(3, launch=overrides)}
     if not args.no_batch:
         line["batched"] = batched(cfg)
     if not args.no_proposal:
         line["learned_proposal"] = learned_proposal(cfg)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.workload, args.env_seed, args.cpu_plans, seeds[args.warmup:])
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.env_seed, [2000 + i for i in range(50)],
+                                            budget_s=args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

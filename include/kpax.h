@@ -263,6 +263,11 @@ typedef struct kpax_launch {
                                   and takes new ones; 1 services every lane as soon as it idles */
     uint32_t scan_tile;        /* items per append/frontier scan tile: 1024, 2048 or 4096
                                   (blocks of 256, 512 or 1024 threads, 4 items each)            */
+    int32_t  lds_grid;         /* 1 (default): single-query expansions and the collision / propagation
+                                  operators stage the environment's collision grid in LDS when its image
+                                  fits (kpax_get_lds_grid); 0: they query the grid in global memory    */
+    uint32_t expand_threads;   /* work-group size of the single-query expansion: a multiple of 64, at most
+                                  640 (the batched expansion always runs 256)                          */
 } kpax_launch;
 
 /* Current launch shape of the handle. */
@@ -403,6 +408,15 @@ int kpax_scan_compact(kpax_handle* h, const uint8_t* flags, uint32_t n, uint32_t
  * obstacles past the 16th of a cell (spatialHash.cu:41-48). */
 int kpax_get_grid(kpax_handle* h, int32_t* cell_start, int32_t* items, uint32_t items_cap,
                   uint32_t* n_cells, uint32_t* n_items);
+
+/* LDS form of environment `env`'s collision grid: a coarse uniform grid of
+ * the workspace whose cells list 16-bit obstacle ids, with the obstacles'
+ * boxes, chosen to minimise the mean number of obstacles per cell within the
+ * LDS budget (spatialHash.cu:9-54 builds the reference's grid, which its
+ * query reads from global memory, :145-194).  dims = {0, 0, 0} and bytes = 0
+ * when the environment's image does not fit: its queries then read the CSR
+ * grid in global memory. */
+int kpax_get_lds_grid(kpax_handle* h, uint32_t env, int32_t* dims, uint32_t* bytes);
 
 /* ---- environments (input formats either side of the path) -------------- */
 

@@ -87,7 +87,7 @@ __device__ __forceinline__ void sincos_f32(float x, float* s, float* c)
 
 // Truncation toward zero clamped to [0, len-1], NaN -> 0; identical to the
 // reference's (int) cast + clamp for finite inputs (Graph.cu:107-109).
-__device__ __forceinline__ int clamp_index(float f, int len)
+__host__ __device__ __forceinline__ int clamp_index(float f, int len)
 {
     if(!(f >= 0.0f)) return 0;
     if(f >= (float)len) return len - 1;
@@ -103,18 +103,35 @@ __device__ __forceinline__ bool inb(float v, float lo, float hi) { return v >= l
 // The CSR items hold the obstacles' boxes themselves (two float4 per item,
 // copied from the obstacle list after the build), so testing an item is one
 // pair of independent loads instead of an index load and then the box.
+// An environment's collision grid in a form a work-group stages into LDS:
+// a coarse uniform grid of the workspace (dimensions D, the same kind of
+// monotone cell map as the CSR grid) whose cells list 16-bit obstacle ids,
+// and every obstacle's box.  Image layout in bytes: box i at 24 i as
+// (min x, min y), (min z, max x), (max y, max z); cell starts (uint16, one per
+// cell plus the end) at off_start; ids (uint16) at off_items; `bytes` a
+// multiple of 16.  bytes = 0: the environment has no image (too large for
+// the LDS budget) and queries use the CSR grid in global memory.
+struct LdsGrid {
+    int32_t D[3];
+    float mn[3], inv[3];
+    uint32_t off_start, off_items, bytes;
+    const uint4* img;  // the image in global memory
+};
+
 struct GridView {
     GridParams gp;
     const int32_t* start;
     const float4* box;
     const uint32_t* occ;  // occupancy bitmap (learned proposal's features)
+    LdsGrid lg;
+    const uint8_t* lds;  // the staged image (set by stage_lds_grid inside a kernel), else nullptr
 };
 
 // Cell of coordinate x: a multiply by cells per metre instead of a division
 // by the cell size.  The grid build and every query use this same monotone
 // map, so a box and every obstacle it overlaps share a cell whatever the
 // rounding.
-__device__ __forceinline__ int cell_of(float x, float mn, float inv, int D)
+__host__ __device__ __forceinline__ int cell_of(float x, float mn, float inv, int D)
 {
     return clamp_index((x - mn) * inv, D);
 }
@@ -132,8 +149,58 @@ __device__ __forceinline__ bool box_hits(const GridView& g, int k, const float b
     return !sep;
 }
 
+// The same query through the staged LDS image: every obstacle listed in the
+// coarse cells the box overlaps is tested (isMotionValidSpatialHash,
+// spatialHash.cu:145-194; broad phase collisionCheck.cu:3-10).  An obstacle
+// that overlaps the box shares a coarse cell with it, so the verdict is the
+// brute-force one, as with the CSR grid.
+__device__ __forceinline__ bool seg_free_lds(const GridView& g, const float bmin[3], const float bmax[3])
+{
+    const LdsGrid& L = g.lg;
+    int lo[3], hi[3];
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        lo[d] = cell_of(bmin[d], L.mn[d], L.inv[d], L.D[d]);
+        hi[d] = cell_of(bmax[d], L.mn[d], L.inv[d], L.D[d]);
+    }
+    const uint16_t* st = (const uint16_t*)(g.lds + L.off_start);
+    const uint16_t* it = (const uint16_t*)(g.lds + L.off_items);
+    const float2* bx   = (const float2*)g.lds;
+    for(int z = lo[2]; z <= hi[2]; ++z)
+        for(int y = lo[1]; y <= hi[1]; ++y)
+        {
+            // the cells of one x-row are contiguous
+            const int c0 = lo[0] + L.D[0] * (y + L.D[1] * z);
+            const int e  = st[c0 + 1 + hi[0] - lo[0]];
+            for(int k = st[c0]; k < e; ++k)
+            {
+                const int o    = it[k];
+                const float2 a = bx[3 * o], b = bx[3 * o + 1], c = bx[3 * o + 2];
+                const bool sep = bmax[0] <= a.x || b.y <= bmin[0] || bmax[1] <= a.y || c.x <= bmin[1] ||
+                                 bmax[2] <= b.x || c.y <= bmin[2];
+                if(!sep) return false;
+            }
+        }
+    return true;
+}
+
+// Copies g's LDS image into `lds` with every thread of the work-group and
+// points g at it (no-op without an image).  A work-group barrier: call it
+// with all threads, before any of them returns.
+__device__ __forceinline__ void stage_lds_grid(GridView& g, uint4* lds)
+{
+    g.lds = nullptr;
+    const uint32_t n = g.lg.bytes / 16;
+    if(n == 0) return;
+    for(uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g.lg.img[i];
+    __syncthreads();
+    g.lds = (const uint8_t*)lds;
+}
+
 __device__ __forceinline__ bool seg_free(const GridView& g, const float bmin[3], const float bmax[3])
 {
+    if(g.lds) return seg_free_lds(g, bmin, bmax);
     int lo[3], hi[3];
 #pragma unroll
     for(int d = 0; d < 3; ++d)

@@ -58,10 +58,19 @@ struct Env {
     float4* hi      = nullptr;
     float4* box     = nullptr;  // per CSR item its obstacle's box: box[2k] min corner, box[2k + 1] max corner
     uint32_t* occ   = nullptr;  // occupancy bitmap of the learned proposal's features
+    uint4* lds_img  = nullptr;  // the grid as a work-group stages it into LDS (dev::LdsGrid)
+    dev::LdsGrid lg{};
     uint32_t n_obs  = 0;
     uint32_t k      = 0;
     std::vector<float> host;
 };
+
+// LDS image of an environment (built on the host from its boxes): the
+// coarse grid dimensions that minimise the mean obstacle count per cell
+// within `budget` bytes, or none (lg.bytes = 0) when no image fits or the
+// best one's mean exceeds max_mean.
+std::vector<uint4> build_lds_image(const kpax_config& cfg, const float* aabb6, uint32_t n, uint32_t budget,
+                                   double max_mean, dev::LdsGrid* lg);
 
 constexpr int kTicketGroups = 64, kTicketStride = 32;  // expansion chunk counters, 128 B apart
 constexpr int kTile       = 1024;  // items per scan tile (256 threads x 4); the planner's smallest tile
@@ -108,7 +117,12 @@ struct kpax_handle {
     kpax::dev::GridView* envs_dev = nullptr;  // kMaxEnvs grids
     void* expand_args       = nullptr;
     uint32_t expand_blocks  = 1;
+    uint32_t expand_threads = 256;      // work-group size of the expansion (640 with an LDS-staged grid)
     uint32_t expand_chunk   = 64;
+    bool use_lds            = true;     // stage the collision grid in LDS where the environment's image fits
+    uint32_t lds_budget     = 0;        // largest LDS image of an environment, bytes
+    uint32_t lds_bytes      = 0;        // dynamic LDS of the single-query expansion: the largest image in use
+    bool launch_set         = false;    // the launch shape came from kpax_set_launch (kept across environments)
     bool order_children     = true;     // hand out children longest drawn edge first (default: single-query handles)
     uint32_t* tickets       = nullptr;  // kTicketGroups expansion chunk counters
     uint32_t refill_lanes   = 16;       // idle lanes that trigger a wave's service pass
@@ -192,6 +206,8 @@ inline dev::GridView grid_view(const kpax_handle* h, uint32_t env)
     g.start = h->envs[env].start;
     g.box   = h->envs[env].box;
     g.occ   = h->envs[env].occ;
+    g.lg    = h->envs[env].lg;
+    g.lds   = nullptr;
     return g;
 }
 
@@ -222,5 +238,9 @@ int launch_propose(kpax_handle* h, uint32_t n_init);
 // After a change of what the iteration launches: drops the captured graphs
 // and rewrites the expansion's argument block (kpax_planner.hip).
 int refresh_launch_state(kpax_handle* h);
+// After a change of the environments' LDS images: the single-query
+// expansion's dynamic LDS, work-group size and (unless kpax_set_launch fixed
+// them) work-group count (kpax_planner.hip).
+int refresh_lds_launch(kpax_handle* h);
 
 }  // namespace kpax

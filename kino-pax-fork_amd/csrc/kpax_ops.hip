@@ -19,12 +19,16 @@ using namespace kpax::dev;
 
 namespace {
 
+// The operators stage the grid's LDS image like the expansion does when the
+// host passes its bytes as dynamic LDS (G.lg.bytes = 0: the CSR grid).
 template<int MODEL>
 __global__ __launch_bounds__(256) void k_propagate(ModelParams P, GridView G, const float* x0s, const float* us,
                                                    const int32_t* steps, uint32_t n, float* x1s, uint8_t* valid,
                                                    int32_t* fail)
 {
+    HIP_DYNAMIC_SHARED(uint4, s_grid)
     constexpr int S = Dims<MODEL>::S, C = Dims<MODEL>::C;
+    stage_lds_grid(G, s_grid);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n) return;
     float x0[S], u[C], x1[S];
@@ -42,6 +46,8 @@ __global__ __launch_bounds__(256) void k_propagate(ModelParams P, GridView G, co
 __global__ __launch_bounds__(256) void k_collide(GridView G, const float* bmin, const float* bmax, uint32_t n,
                                                  uint8_t* out)
 {
+    HIP_DYNAMIC_SHARED(uint4, s_grid)
+    stage_lds_grid(G, s_grid);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n) return;
     const float lo[3] = {bmin[i], bmin[(size_t)n + i], bmin[2 * (size_t)n + i]};
@@ -218,9 +224,123 @@ __global__ void k_grid_boxes(const int32_t* items, const float4* lo, const float
     box[2 * (size_t)k + 1] = hi[o];
 }
 
+// Environment 0's grid for the operators: with its LDS image unless the
+// launch option is off or it has none.
+GridView operator_grid(const kpax_handle* h)
+{
+    GridView G = grid_view(h, 0);
+    if(!h->use_lds || !h->envs[0].lds_img) G.lg.bytes = 0;
+    return G;
+}
+
 }  // namespace
 
 namespace kpax {
+
+std::vector<uint4> build_lds_image(const kpax_config& cfg, const float* aabb6, uint32_t n, uint32_t budget,
+                                   double max_mean, dev::LdsGrid* lg)
+{
+    *lg = dev::LdsGrid{};
+    if(n > 65535u) return {};
+    // candidate dimensions per axis: powers of two up to the CSR grid's
+    std::vector<int> cand[3];
+    for(int d = 0; d < 3; ++d)
+    {
+        for(int v = 1; v < cfg.grid_dims[d]; v *= 2) cand[d].push_back(v);
+        cand[d].push_back(cfg.grid_dims[d]);
+    }
+    auto align16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+    float mn[3], ext[3];
+    for(int d = 0; d < 3; ++d)
+    {
+        mn[d]  = cfg.ws_min[d];
+        ext[d] = cfg.ws_max[d] - cfg.ws_min[d];
+    }
+    const size_t box_bytes = align16(24 * (size_t)n);
+    int best[3]            = {0, 0, 0};
+    double best_cost       = 0.0;
+    size_t best_bytes      = 0;
+    for(int a : cand[0])
+        for(int b : cand[1])
+            for(int c : cand[2])
+            {
+                const int D[3]  = {a, b, c};
+                const size_t G  = (size_t)a * b * c;
+                const size_t lo = box_bytes + align16(2 * (G + 1));
+                if(lo > budget) continue;
+                float inv[3];
+                for(int d = 0; d < 3; ++d) inv[d] = (float)D[d] / ext[d];
+                size_t items = 0;
+                for(uint32_t i = 0; i < n; ++i)
+                {
+                    size_t span = 1;
+                    for(int d = 0; d < 3; ++d)
+                        span *= (size_t)(dev::cell_of(aabb6[6 * (size_t)i + 3 + d], mn[d], inv[d], D[d]) -
+                                         dev::cell_of(aabb6[6 * (size_t)i + d], mn[d], inv[d], D[d]) + 1);
+                    items += span;
+                }
+                const size_t bytes = lo + align16(2 * items);
+                if(items > 65535u || bytes > budget) continue;
+                // mean obstacles per cell (tests per query), then the smaller image
+                const double cost = (double)items / (double)G;
+                if(!best[0] || cost < best_cost || (cost == best_cost && bytes < best_bytes))
+                {
+                    best[0] = a, best[1] = b, best[2] = c;
+                    best_cost  = cost;
+                    best_bytes = bytes;
+                }
+            }
+    if(!best[0] || best_cost > max_mean) return {};
+    dev::LdsGrid L{};
+    for(int d = 0; d < 3; ++d)
+    {
+        L.D[d]   = best[d];
+        L.mn[d]  = mn[d];
+        L.inv[d] = (float)best[d] / ext[d];
+    }
+    const size_t G = (size_t)best[0] * best[1] * best[2];
+    std::vector<uint32_t> count(G + 1, 0);
+    auto cells = [&](uint32_t i, int lo[3], int hi[3]) {
+        for(int d = 0; d < 3; ++d)
+        {
+            lo[d] = dev::cell_of(aabb6[6 * (size_t)i + d], L.mn[d], L.inv[d], L.D[d]);
+            hi[d] = dev::cell_of(aabb6[6 * (size_t)i + 3 + d], L.mn[d], L.inv[d], L.D[d]);
+        }
+    };
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        int lo[3], hi[3];
+        cells(i, lo, hi);
+        for(int z = lo[2]; z <= hi[2]; ++z)
+            for(int y = lo[1]; y <= hi[1]; ++y)
+                for(int x = lo[0]; x <= hi[0]; ++x) count[x + (size_t)L.D[0] * (y + (size_t)L.D[1] * z) + 1]++;
+    }
+    for(size_t g = 0; g < G; ++g) count[g + 1] += count[g];
+    const size_t items = count[G];
+    L.off_start = (uint32_t)box_bytes;
+    L.off_items = (uint32_t)(box_bytes + align16(2 * (G + 1)));
+    L.bytes     = (uint32_t)(L.off_items + align16(2 * items));
+    std::vector<uint4> img(L.bytes / 16);
+    uint8_t* base = (uint8_t*)img.data();
+    std::memset(base, 0, L.bytes);
+    float* bx = (float*)base;
+    for(uint32_t i = 0; i < n; ++i)
+        for(int j = 0; j < 6; ++j) bx[6 * (size_t)i + j] = aabb6[6 * (size_t)i + j];
+    uint16_t* st = (uint16_t*)(base + L.off_start);
+    uint16_t* it = (uint16_t*)(base + L.off_items);
+    for(size_t g = 0; g <= G; ++g) st[g] = (uint16_t)count[g];
+    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
+    for(uint32_t i = 0; i < n; ++i)  // ascending ids per cell
+    {
+        int lo[3], hi[3];
+        cells(i, lo, hi);
+        for(int z = lo[2]; z <= hi[2]; ++z)
+            for(int y = lo[1]; y <= hi[1]; ++y)
+                for(int x = lo[0]; x <= hi[0]; ++x) it[fill[x + (size_t)L.D[0] * (y + (size_t)L.D[1] * z)]++] = (uint16_t)i;
+    }
+    *lg = L;
+    return img;
+}
 
 void free_env(Env& e)
 {
@@ -230,6 +350,7 @@ void free_env(Env& e)
     if(e.lo) (void)hipFree(e.lo);
     if(e.hi) (void)hipFree(e.hi);
     if(e.occ) (void)hipFree(e.occ);
+    if(e.lds_img) (void)hipFree(e.lds_img);
     e = Env{};
 }
 
@@ -290,6 +411,20 @@ int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n)
         const int rc = build_occupancy(h, env);
         if(rc != KPAX_OK) return rc;
     }
+    {
+        // an image only where its cells list about as few obstacles as the
+        // CSR grid's: a coarser grid tests more boxes per query
+        const double fine_mean       = (double)K / (double)std::max<uint32_t>(G, 1);
+        const std::vector<uint4> img = build_lds_image(h->cfg, aabb6, n, h->lds_budget, 1.5 * fine_mean + 0.05, &e.lg);
+        if(!img.empty())
+        {
+            HIPC(hipMalloc(&e.lds_img, img.size() * sizeof(uint4)));
+            HIPC(hipMemcpy(e.lds_img, img.data(), img.size() * sizeof(uint4), hipMemcpyHostToDevice));
+            e.lg.img = e.lds_img;
+        }
+        const int rc = refresh_lds_launch(h);
+        if(rc != KPAX_OK) return rc;
+    }
     const GridView g = grid_view(h, env);
     HIPC(hipMemcpyAsync(h->envs_dev + env, &g, sizeof(g), hipMemcpyHostToDevice, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
@@ -317,6 +452,16 @@ int kpax_get_grid(kpax_handle* h, int32_t* cell_start, int32_t* items, uint32_t 
     return KPAX_OK;
 }
 
+int kpax_get_lds_grid(kpax_handle* h, uint32_t env, int32_t* dims, uint32_t* bytes)
+{
+    if(!h) return KPAX_INVALID_ARG;
+    if(env >= (uint32_t)kMaxEnvs) return fail(h, KPAX_INVALID_ARG, "environment index out of range");
+    const Env& e = h->envs[env];
+    for(int d = 0; dims && d < 3; ++d) dims[d] = e.lds_img ? e.lg.D[d] : 0;
+    if(bytes) *bytes = e.lds_img ? e.lg.bytes : 0;
+    return KPAX_OK;
+}
+
 int kpax_propagate_batch(kpax_handle* h, const float* x0_soa, const float* u_soa, const int32_t* steps, uint32_t n,
                          float* x1_soa, uint8_t* valid, int32_t* fail_step)
 {
@@ -338,21 +483,22 @@ int kpax_propagate_batch(kpax_handle* h, const float* x0_soa, const float* u_soa
     HIPC(hipMemcpy(dx0, x0_soa, (size_t)S * n * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(du, u_soa, (size_t)C * n * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(dst, steps, (size_t)n * 4, hipMemcpyHostToDevice));
-    const GridView G = grid_view(h, 0);
-    const uint32_t b = blocks_for(n, 256);
+    const GridView G  = operator_grid(h);
+    const uint32_t b  = blocks_for(n, 256);
+    const size_t lds  = G.lg.bytes;
     switch(h->cfg.model)
     {
         case KPAX_MODEL_DOUBLE_INTEGRATOR:
-            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_DOUBLE_INTEGRATOR>, dim3(b), dim3(256), 0, h->stream, h->P, G,
+            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_DOUBLE_INTEGRATOR>, dim3(b), dim3(256), lds, h->stream, h->P, G,
                                dx0, du, dst, n, dx1, dv, df);
             break;
         case KPAX_MODEL_DUBINS_AIRPLANE:
-            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_DUBINS_AIRPLANE>, dim3(b), dim3(256), 0, h->stream, h->P, G, dx0,
-                               du, dst, n, dx1, dv, df);
+            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_DUBINS_AIRPLANE>, dim3(b), dim3(256), lds, h->stream, h->P, G,
+                               dx0, du, dst, n, dx1, dv, df);
             break;
         default:
-            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_QUADCOPTER>, dim3(b), dim3(256), 0, h->stream, h->P, G, dx0, du,
-                               dst, n, dx1, dv, df);
+            hipLaunchKernelGGL(k_propagate<KPAX_MODEL_QUADCOPTER>, dim3(b), dim3(256), lds, h->stream, h->P, G, dx0,
+                               du, dst, n, dx1, dv, df);
             break;
     }
     HIPC(hipGetLastError());
@@ -375,7 +521,9 @@ int kpax_collide_batch(kpax_handle* h, const float* bbmin_soa, const float* bbma
     if(!lo || !hi || !o) return fail(h, KPAX_NO_MEMORY, "scratch allocation failed");
     HIPC(hipMemcpy(lo, bbmin_soa, 12 * (size_t)n, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(hi, bbmax_soa, 12 * (size_t)n, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_collide, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, grid_view(h, 0), lo, hi, n, o);
+    const GridView G = operator_grid(h);
+    hipLaunchKernelGGL(k_collide, dim3(blocks_for(n, 256)), dim3(256), (size_t)G.lg.bytes, h->stream, G, lo, hi, n,
+                       o);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     HIPC(hipMemcpy(free_out, o, n, hipMemcpyDeviceToHost));

@@ -150,6 +150,7 @@ struct ExpandArgs {
     uint4* prep;            // per processing position: {child, parent, steps | slot << 17, admit draw}, {controls}
     uint32_t* tickets;      // kTicketGroups chunk counters, kTicketStride words apart
     uint32_t refill;        // idle lanes that trigger a wave's service pass (1..64)
+    uint32_t use_lds;       // single-slot form: stage the environment's grid image in LDS
     const float* prop;
     SlotUniform s0;
 };
@@ -251,11 +252,15 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
 // ---------------------------------------------------------------------------
 // Five resident waves per SIMD (96 VGPRs for the quadcopter) hide more of
 // the grid lookups' latency than four; the batched form carries a slot's
-// state per lane and keeps four.
+// state per lane and keeps four.  The single-slot form stages its
+// environment's grid image in LDS (work-groups of 640 threads: two per CU
+// hold the five waves per SIMD and two images); the batched form's lanes
+// query the grids of several environments in global memory (256 threads).
 template<int MODEL, bool MULTI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 : 5))) void k_expand(
+__global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 : 5))) void k_expand(
   const ExpandArgs* __restrict__ A)
 {
+    HIP_DYNAMIC_SHARED(uint4, s_grid)
     const TreeView& tree   = A->tree;
     const ChildView& ch    = A->ch;
     const RegionView& rv   = A->rv;
@@ -277,9 +282,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
     // every chunk of their group, and every group has blocks.
     const uint32_t groups  = gridDim.x < (uint32_t)kTicketGroups ? gridDim.x : (uint32_t)kTicketGroups;
     const uint32_t group   = blockIdx.x % groups;
-    const uint32_t rank    = (blockIdx.x / groups) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if(((uint64_t)rank * groups + group) * A->chunk >= total) return;
-    uint32_t* const ticket = A->tickets + group * kTicketStride;
+    const uint32_t rank0   = (blockIdx.x / groups) * (blockDim.x >> 6);  // the work-group's first wave
+    const uint32_t rank    = rank0 + (threadIdx.x >> 6);
+    // a work-group none of whose waves has a first chunk leaves at once
+    // (before the LDS staging, whose barrier needs every thread)
+    if(((uint64_t)rank0 * groups + group) * A->chunk >= total) return;
     const int R1 = RP.r1_count;
     // slot-uniform in the single-slot form, per lane in the batched form
     uint32_t t = 0, slot = 0, base = 0;
@@ -289,9 +296,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 :
         t    = A->s0.cur.t;
         base = A->s0.qd.base;
         G    = A->s0.G;
+        if(A->use_lds)
+            stage_lds_grid(G, s_grid);
+        else
+            G.lds = nullptr;
     }
     else
-        G.gp = A->GP;  // grid parameters are shared; pointers are set per child
+    {
+        G.gp  = A->GP;  // grid parameters are shared; pointers are set per child
+        G.lds = nullptr;
+    }
+    if(((uint64_t)rank * groups + group) * A->chunk >= total) return;
+    uint32_t* const ticket = A->tickets + group * kTicketStride;
     const int lane       = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     uint32_t c = kNone, parent = 0, admit_draw = 0;
@@ -955,8 +971,8 @@ template<int MODEL, bool MULTI>
 uint32_t expand_blocks_for(const kpax_handle* h)
 {
     int per_cu = 0;
-    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand<MODEL, MULTI>, 256, 0) !=
-         hipSuccess ||
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand<MODEL, MULTI>, (int)h->expand_threads,
+                                                    MULTI ? 0 : h->lds_bytes) != hipSuccess ||
        per_cu < 1)
         per_cu = 1;
     return (uint32_t)per_cu * (uint32_t)h->cu_count;
@@ -1001,6 +1017,7 @@ int upload_expand_args(kpax_handle* h)
     a.prep      = h->prep;
     a.tickets   = h->tickets;
     a.refill    = h->refill_lanes;
+    a.use_lds   = h->Qmax == 1 && h->use_lds && h->lds_bytes > 0 ? 1u : 0u;
     a.prop      = h->prop;
     memset(&a.s0, 0, sizeof(a.s0));
     a.s0.cur.status = kNotStarted;
@@ -1013,9 +1030,10 @@ void launch_expand_model(kpax_handle* h)
 {
     const ExpandArgs* A = (const ExpandArgs*)h->expand_args;
     if(h->Qmax == 1)
-        hipLaunchKernelGGL((k_expand<MODEL, false>), dim3(h->expand_blocks), dim3(256), 0, h->stream, A);
+        hipLaunchKernelGGL((k_expand<MODEL, false>), dim3(h->expand_blocks), dim3(h->expand_threads),
+                           h->use_lds ? h->lds_bytes : 0u, h->stream, A);
     else
-        hipLaunchKernelGGL((k_expand<MODEL, true>), dim3(h->expand_blocks), dim3(256), 0, h->stream, A);
+        hipLaunchKernelGGL((k_expand<MODEL, true>), dim3(h->expand_blocks), dim3(h->expand_threads), 0, h->stream, A);
 }
 
 // The children's records for this iteration (k_prepare), in longest-first
@@ -1521,15 +1539,57 @@ int kpax::refresh_launch_state(kpax_handle* h)
     return upload_expand_args(h);
 }
 
+// The single-query expansion's work-group size: the one of 256, 320, 512 and
+// 640 threads with the most resident waves per CU under its dynamic LDS (the
+// smaller on a tie).  Measured on the quadcopter forest (22 KB image): 256
+// and 512 threads ran the expansion in 2.79 ms per plan, 320 in 2.88 and 640
+// (one resident work-group per CU) in 3.10.
+static uint32_t pick_expand_threads(kpax_handle* h)
+{
+    if(h->Qmax != 1) return 256u;
+    uint32_t best = 256u, best_waves = 0;
+    for(uint32_t t : {256u, 320u, 512u, 640u})
+    {
+        h->expand_threads = t;
+        const uint32_t waves = expand_blocks(h) / (uint32_t)h->cu_count * (t / 64);
+        if(waves > best_waves)
+        {
+            best       = t;
+            best_waves = waves;
+        }
+    }
+    return best;
+}
+
+int kpax::refresh_lds_launch(kpax_handle* h)
+{
+    uint32_t bytes = 0;
+    if(h->Qmax == 1)
+        for(const Env& e : h->envs)
+            if(e.lds_img) bytes = std::max(bytes, e.lg.bytes);
+    h->lds_bytes = bytes;
+    if(!h->launch_set)
+    {
+        h->expand_threads = pick_expand_threads(h);
+        h->expand_blocks  = expand_blocks(h);
+    }
+    return refresh_launch_state(h);
+}
+
 namespace {
 
 int apply_launch(kpax_handle* h, const kpax_launch& l)
 {
     if(l.expand_blocks < 1 || l.expand_blocks > (1u << 20) || l.expand_chunk < 64 || l.expand_chunk > 4096 ||
        l.expand_chunk % 64 != 0 || l.scan_blocks < 1 || l.scan_blocks > (1u << 20) || l.refill_lanes < 1 ||
-       l.refill_lanes > 64 || (l.scan_tile != 1024 && l.scan_tile != 2048 && l.scan_tile != 4096))
+       l.refill_lanes > 64 || (l.scan_tile != 1024 && l.scan_tile != 2048 && l.scan_tile != 4096) ||
+       (l.lds_grid != 0 && l.lds_grid != 1) || l.expand_threads < 64 || l.expand_threads > 640 ||
+       l.expand_threads % 64 != 0)
         return fail(h, KPAX_INVALID_ARG, "launch shape out of range");
     HIPC(hipStreamSynchronize(h->stream));
+    h->launch_set    = true;
+    h->use_lds       = l.lds_grid != 0;
+    h->expand_threads = h->Qmax == 1 ? l.expand_threads : 256u;
     h->expand_blocks = l.expand_blocks;
     h->expand_chunk  = l.expand_chunk;
     h->scan_blocks   = l.scan_blocks;
@@ -1553,6 +1613,8 @@ kpax_launch current_launch(const kpax_handle* h)
     l.order_children = h->order_children ? 1 : 0;
     l.refill_lanes   = h->refill_lanes;
     l.scan_tile      = h->scan_tile;
+    l.lds_grid       = h->use_lds ? 1 : 0;
+    l.expand_threads = h->expand_threads;
     return l;
 }
 
@@ -1622,7 +1684,17 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     if(hipGetDeviceProperties(&prop, cfg->device) == hipSuccess && prop.multiProcessorCount > 0)
         h->cu_count = prop.multiProcessorCount;
     if(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
-    h->expand_blocks = expand_blocks(h);
+    {
+        // an environment's LDS image: at most 72 KiB, so that two 640-thread
+        // expansion work-groups (five waves per SIMD) fit in one CU's LDS
+        size_t lds = 72u << 10;
+        if(prop.sharedMemPerBlock > 0) lds = std::min<size_t>(lds, prop.sharedMemPerBlock);
+        if(prop.maxSharedMemoryPerMultiProcessor > 0)
+            lds = std::min<size_t>(lds, prop.maxSharedMemoryPerMultiProcessor / 2);
+        h->lds_budget = (uint32_t)(lds & ~(size_t)15);
+    }
+    h->expand_threads = 256u;
+    h->expand_blocks  = expand_blocks(h);
     h->expand_chunk  = 64;
     const uint32_t Q = h->Qmax;
     const size_t T   = h->T;
@@ -2044,7 +2116,7 @@ int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_
     cur.expand_blocks = occ * cu;
     double cur_ms     = 0.0;
     int rc            = run(cur, &cur_ms, ref);
-    std::vector<std::vector<kpax_launch>> knobs(7);
+    std::vector<std::vector<kpax_launch>> knobs(8);
     for(uint32_t k = 1; k <= occ; ++k) knobs[0].push_back(kpax_launch{k * cu, 0, 0, 0, 0, 0, 0});
     for(uint32_t c : {64u, 128u, 256u, 512u}) knobs[1].push_back(kpax_launch{0, c, 0, 0, 0, 0, 0});
     for(uint32_t t : {4096u, 2048u, 1024u}) knobs[2].push_back(kpax_launch{0, 0, 0, 0, 0, 0, t});
@@ -2052,7 +2124,8 @@ int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_
     for(int g : {1, 0}) knobs[4].push_back(kpax_launch{0, 0, 0, g, 0, 0, 0});
     for(int o : {1, 0}) knobs[5].push_back(kpax_launch{0, 0, 0, 0, o, 0, 0});
     for(uint32_t r : {1u, 8u, 16u, 24u, 32u}) knobs[6].push_back(kpax_launch{0, 0, 0, 0, 0, r, 0});
-    for(int k = 0; rc == KPAX_OK && k < 7; ++k)
+    for(int g : {1, 0}) knobs[7].push_back(kpax_launch{0, 0, 0, 0, 0, 0, 0, g});
+    for(int k = 0; rc == KPAX_OK && k < 8; ++k)
         for(const kpax_launch& v : knobs[k])
         {
             kpax_launch cand = cur;
@@ -2063,6 +2136,7 @@ int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_
             if(k == 4) cand.use_graph = v.use_graph;
             if(k == 5) cand.order_children = v.order_children;
             if(k == 6) cand.refill_lanes = v.refill_lanes;
+            if(k == 7) cand.lds_grid = v.lds_grid;
             if(memcmp(&cand, &cur, sizeof(cand)) == 0) continue;
             double ms = 0.0;
             if((rc = run(cand, &ms, got)) != KPAX_OK) break;

@@ -90,7 +90,8 @@ class _Stats(ctypes.Structure):
 class _Launch(ctypes.Structure):
     _fields_ = [("expand_blocks", ctypes.c_uint32), ("expand_chunk", ctypes.c_uint32),
                 ("scan_blocks", ctypes.c_uint32), ("use_graph", ctypes.c_int32), ("order_children", ctypes.c_int32),
-                ("refill_lanes", ctypes.c_uint32), ("scan_tile", ctypes.c_uint32)]
+                ("refill_lanes", ctypes.c_uint32), ("scan_tile", ctypes.c_uint32), ("lds_grid", ctypes.c_int32),
+                ("expand_threads", ctypes.c_uint32)]
 
 
 class _Query(ctypes.Structure):
@@ -146,6 +147,7 @@ def lib():
         "kpax_get_tree": (i32, [P, u32, P, P, P, P, P, P]),
         "kpax_get_regions": (i32, [P, P, P, P, P]),
         "kpax_get_grid": (i32, [P, P, P, u32, P, P]),
+        "kpax_get_lds_grid": (i32, [P, u32, P, P]),
         "kpax_set_profiling": (i32, [P, i32]),
         "kpax_get_stats": (i32, [P, P]),
         "kpax_reset_stats": (i32, [P]),
@@ -521,7 +523,8 @@ class KPAX:
         return out
 
     def launch(self):
-        """Launch shape: expand_blocks, expand_chunk, scan_blocks, use_graph, order_children, refill_lanes, scan_tile."""
+        """Launch shape: expand_blocks, expand_chunk, scan_blocks, use_graph, order_children, refill_lanes, scan_tile,
+        lds_grid, expand_threads."""
         l = _Launch()
         self._check(lib().kpax_get_launch(self._h, ctypes.byref(l)), "get_launch")
         return {k: getattr(l, k) for k, _ in _Launch._fields_}
@@ -559,6 +562,13 @@ class KPAX:
         s = np.zeros(n, np.float32)
         self._check(lib().kpax_get_regions(self._h, _p(v), _p(t), _p(c), _p(s)), "get_regions")
         return {"valid": v, "total": t, "covered": c, "scores": s}
+
+    def lds_grid(self, env=0):
+        """(dims, bytes) of environment env's LDS-staged grid; ((0, 0, 0), 0) when it has none."""
+        dims = np.zeros(3, np.int32)
+        b = ctypes.c_uint32()
+        self._check(lib().kpax_get_lds_grid(self._h, int(env), _p(dims), ctypes.byref(b)), "get_lds_grid")
+        return tuple(int(x) for x in dims), int(b.value)
 
     def grid(self):
         nc, nk = ctypes.c_uint32(), ctypes.c_uint32()

@@ -45,6 +45,9 @@ struct uint4 {
 struct float4 {
     float x, y, z, w;
 };
+struct float2 {
+    float x, y;
+};
 struct int4 {
     int32_t x, y, z, w;
 };
@@ -203,6 +206,8 @@ typedef struct emu_stream_t* hipStream_t;
 typedef struct emu_event_t* hipEvent_t;
 struct hipDeviceProp_t {
     int multiProcessorCount;
+    size_t sharedMemPerBlock;
+    size_t maxSharedMemoryPerMultiProcessor;
 };
 #define hipStreamNonBlocking 1
 #define hipEventDisableTiming 2
@@ -216,7 +221,9 @@ inline hipError_t hipGetDeviceCount(int* n)
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int)
 {
-    p->multiProcessorCount = 1;
+    p->multiProcessorCount              = 1;
+    p->sharedMemPerBlock                = 160 << 10;
+    p->maxSharedMemoryPerMultiProcessor = 160 << 10;
     return hipSuccess;
 }
 inline hipError_t hipMalloc(void** p, size_t n)

@@ -90,6 +90,56 @@ def test_grid_csr_matches_oracle_counts(kpax, oracle, k):
         assert np.all(np.diff(seg) > 0)
 
 
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_lds_grid_verdicts_equal_brute_force(kpax, oracle, rng, k):
+    """The collision grid staged in LDS (a coarse grid of 16-bit obstacle
+    lists) and the CSR grid in global memory give the brute-force verdicts for
+    random segments, segments ending on an obstacle face (free) and
+    zero-length segments, and propagated edges are bit-identical both ways.
+    The 2,000-box maze gets no image: the coarse grids that fit the LDS
+    budget list ~17x more obstacles per cell than its 128^3 CSR grid."""
+    cfg, obs, _, _ = workload(k, seed=1)
+    lo_ws = np.array(cfg.ws_min, np.float32)[:, None]
+    ext = (np.array(cfg.ws_max, np.float32) - np.array(cfg.ws_min, np.float32))[:, None]
+    n = 200000
+    a = (lo_ws + ext * rng.uniform(-0.05, 1.05, (3, n))).astype(np.float32)
+    b = (a + ext * rng.uniform(-0.03, 0.03, (3, n))).astype(np.float32)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    pick = rng.integers(0, obs.shape[0], 2000)
+    hi[0, :2000] = obs[pick, 0]
+    lo[0, :2000] = np.minimum(lo[0, :2000], hi[0, :2000])
+    lo[:, 2000:4000] = hi[:, 2000:4000]
+    x0, u, steps = random_states(cfg, 20000, rng)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        dims, nbytes = p.lds_grid()
+        print(f"workload {k}: LDS grid {dims}, {nbytes} bytes", flush=True)
+        assert p.launch()["lds_grid"] == 1
+        assert (nbytes == 0) if k == 3 else (nbytes > 0)
+        staged = p.collide_batch(lo, hi)
+        e1 = p.propagate_batch(x0, u, steps)
+        p.set_launch(lds_grid=0)
+        csr = p.collide_batch(lo, hi)
+        e0 = p.propagate_batch(x0, u, steps)
+    want = oracle.collide(obs, lo, hi)
+    assert np.array_equal(staged, want) and np.array_equal(csr, want)
+    assert 0.01 < want.mean() < 0.999
+    assert np.array_equal(e1[0].view(np.uint32), e0[0].view(np.uint32)) and np.array_equal(e1[2], e0[2])
+
+
+def test_plan_identical_with_and_without_lds_grid(kpax):
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=1 << 18)
+    with kpax.KPAX(cfg) as p:
+        r1 = p.plan(start, goal, obs, seed=4)
+        t1 = p.tree()
+        p.set_launch(lds_grid=0)
+        r0 = p.plan(start, goal, obs, seed=4)
+        t0 = p.tree()
+    assert (r1.status, r1.tree_size, r1.iterations, r1.expansions) == (r0.status, r0.tree_size, r0.iterations,
+                                                                       r0.expansions)
+    assert np.array_equal(t1["states"].view(np.uint32), t0["states"].view(np.uint32))
+
+
 @pytest.mark.parametrize("k", MODELS)
 def test_regions_match_oracle(kpax, oracle, rng, k):
     p, cfg, _, _, _ = _planner(kpax, k, max_tree=4096)
