@@ -391,6 +391,20 @@ int kpax_scores(kpax_handle* h, const int32_t* valid_count, const int32_t* total
 int kpax_repeat_expand(kpax_handle* h, const uint32_t* weights, uint32_t n, uint32_t* out,
                        uint32_t out_cap, uint32_t* total);
 
+/* kpax_repeat_expand without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Inputs and outputs are DEVICE buffers: d_weights (n), d_out (out_cap; at
+ * most out_cap entries are written), *d_total (the sum of the weights, which
+ * must be below 2^32).  The work is enqueued on the handle's stream and not
+ * waited for.  The look-back status words live in one library-owned device
+ * array per GPU, sized for n <= KPAX_NOALLOC_MAX_ITEMS (larger n is rejected
+ * with KPAX_INVALID_ARG): calls that overlap on one GPU, from any handle,
+ * race on it. */
+#define KPAX_NOALLOC_MAX_ITEMS (1u << 24)
+int kpax_repeat_expand_noalloc(kpax_handle* h, const uint32_t* d_weights, uint32_t n, uint32_t* d_out,
+                               uint32_t out_cap, uint32_t* d_total);
+
 /* Stream compaction of a flag array: out lists every index i with
  * flags[i] != 0, in increasing i; *count = their number (out must hold
  * out_cap entries; at most out_cap are written).  Replaces the frontier mask

@@ -123,6 +123,7 @@ struct kpax_handle {
     uint32_t lds_budget     = 0;        // largest LDS image of an environment, bytes
     uint32_t lds_bytes      = 0;        // dynamic LDS of the single-query expansion: the largest image in use
     bool launch_set         = false;    // the launch shape came from kpax_set_launch (kept across environments)
+    uint64_t* noalloc       = nullptr;  // device address of the allocation-free operators' scratch array
     bool order_children     = true;     // hand out children longest drawn edge first (default: single-query handles)
     uint32_t* tickets       = nullptr;  // kTicketGroups expansion chunk counters
     uint32_t refill_lanes   = 16;       // idle lanes that trigger a wave's service pass
@@ -238,6 +239,9 @@ int launch_propose(kpax_handle* h, uint32_t n_init);
 // After a change of what the iteration launches: drops the captured graphs
 // and rewrites the expansion's argument block (kpax_planner.hip).
 int refresh_launch_state(kpax_handle* h);
+// Device address of the allocation-free operators' scratch array on the
+// current device (kpax_ops.hip), looked up once per device.
+int noalloc_scratch(kpax_handle* h, uint64_t** out);
 // After a change of the environments' LDS images: the single-query
 // expansion's dynamic LDS, work-group size and (unless kpax_set_launch fixed
 // them) work-group count (kpax_planner.hip).

@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -224,6 +226,21 @@ __global__ void k_grid_boxes(const int32_t* items, const float4* lo, const float
     box[2 * (size_t)k + 1] = hi[o];
 }
 
+// ---- allocation-free repeat-expansion (kpax_repeat_expand_noalloc) --------
+// The look-back status words and the tile ticket of the expansion, for up to
+// KPAX_NOALLOC_MAX_ITEMS weights: one array per GPU, held until the process
+// ends and shared by every call on that GPU.  In host code `scratch` only
+// names the array; its device address comes from hipGetSymbolAddress.
+constexpr uint32_t kNoAllocWords = KPAX_NOALLOC_MAX_ITEMS / kTile + 4;
+static __device__ __align__(256) uint64_t scratch[kNoAllocWords];
+
+// Zeroes the status words and the ticket a call uses, and its total.
+__global__ __launch_bounds__(256) void k_noalloc_clear(uint64_t* words, uint32_t n, uint32_t* total)
+{
+    for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) words[i] = 0;
+    if(blockIdx.x == 0 && threadIdx.x == 0) *total = 0;
+}
+
 // Environment 0's grid for the operators: with its LDS image unless the
 // launch option is off or it has none.
 GridView operator_grid(const kpax_handle* h)
@@ -236,6 +253,28 @@ GridView operator_grid(const kpax_handle* h)
 }  // namespace
 
 namespace kpax {
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the device must be current).
+int noalloc_scratch(kpax_handle* h, uint64_t** out)
+{
+    static std::mutex mu;
+    static std::map<int, uint64_t*> addr;
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if(it == addr.end())
+    {
+        void* p            = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if(e != hipSuccess || !p)
+            return fail(h, KPAX_DEVICE_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        it = addr.emplace(dev, (uint64_t*)p).first;
+    }
+    *out = it->second;
+    return KPAX_OK;
+}
 
 std::vector<uint4> build_lds_image(const kpax_config& cfg, const float* aabb6, uint32_t n, uint32_t budget,
                                    double max_mean, dev::LdsGrid* lg)
@@ -617,6 +656,32 @@ int kpax_repeat_expand(kpax_handle* h, const uint32_t* weights, uint32_t n, uint
     HIPC(hipMemcpy(total, ticket + 1, 4, hipMemcpyDeviceToHost));
     const uint32_t k = std::min<uint32_t>(*total, out_cap);
     if(k) HIPC(hipMemcpy(out, dout, (size_t)k * 4, hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
+int kpax_repeat_expand_noalloc(kpax_handle* h, const uint32_t* d_weights, uint32_t n, uint32_t* d_out,
+                               uint32_t out_cap, uint32_t* d_total)
+{
+    if(!h || !d_total || (n && !d_weights) || (out_cap && !d_out)) return fail(h, KPAX_INVALID_ARG, "null argument");
+    if(n > KPAX_NOALLOC_MAX_ITEMS)
+        return fail(h, KPAX_INVALID_ARG, "n exceeds KPAX_NOALLOC_MAX_ITEMS (" +
+                                             std::to_string(KPAX_NOALLOC_MAX_ITEMS) + ") weights");
+    HIPC(hipSetDevice(h->cfg.device));
+    uint64_t* ptr = h->noalloc;
+    if(!ptr)
+    {
+        const int rc = noalloc_scratch(h, &ptr);
+        if(rc != KPAX_OK) return rc;
+        h->noalloc = ptr;
+    }
+    const uint32_t ntiles = (n + kTile - 1) / kTile;
+    const uint32_t words  = ntiles + 2;  // status words [0, ntiles], the ticket in word ntiles + 1
+    hipLaunchKernelGGL(k_noalloc_clear, dim3(std::min<uint32_t>(blocks_for(words, 256), 64)), dim3(256), 0,
+                       h->stream, ptr, words, d_total);
+    if(n)
+        hipLaunchKernelGGL(k_repeat_expand, dim3(std::min<uint32_t>(ntiles, 4 * h->cu_count)), dim3(256), 0,
+                           h->stream, d_weights, n, d_out, out_cap, ptr, (uint32_t*)(ptr + ntiles + 1), d_total);
+    HIPC(hipGetLastError());
     return KPAX_OK;
 }
 

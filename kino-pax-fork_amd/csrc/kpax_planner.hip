@@ -1695,6 +1695,7 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     }
     h->expand_threads = 256u;
     h->expand_blocks  = expand_blocks(h);
+    TRY(noalloc_scratch(h, &h->noalloc));  // the allocation-free operators' array on this device
     h->expand_chunk  = 64;
     const uint32_t Q = h->Qmax;
     const size_t T   = h->T;

@@ -148,6 +148,7 @@ def lib():
         "kpax_get_regions": (i32, [P, P, P, P, P]),
         "kpax_get_grid": (i32, [P, P, P, u32, P, P]),
         "kpax_get_lds_grid": (i32, [P, u32, P, P]),
+        "kpax_repeat_expand_noalloc": (i32, [P, P, u32, P, u32, P]),
         "kpax_set_profiling": (i32, [P, i32]),
         "kpax_get_stats": (i32, [P, P]),
         "kpax_reset_stats": (i32, [P]),
@@ -562,6 +563,12 @@ class KPAX:
         s = np.zeros(n, np.float32)
         self._check(lib().kpax_get_regions(self._h, _p(v), _p(t), _p(c), _p(s)), "get_regions")
         return {"valid": v, "total": t, "covered": c, "scores": s}
+
+    def repeat_expand_noalloc(self, d_weights, n, d_out, out_cap, d_total):
+        """kpax_repeat_expand_noalloc on device buffers (integer addresses);
+        enqueued on the handle's stream, not waited for."""
+        self._check(lib().kpax_repeat_expand_noalloc(self._h, ctypes.c_void_p(d_weights), int(n), ctypes.c_void_p(d_out),
+                                                     int(out_cap), ctypes.c_void_p(d_total)), "repeat_expand_noalloc")
 
     def lds_grid(self, env=0):
         """(dims, bytes) of environment env's LDS-staged grid; ((0, 0, 0), 0) when it has none."""

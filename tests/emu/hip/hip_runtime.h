@@ -29,6 +29,8 @@
 #define __launch_bounds__(...)
 #define amdgpu_waves_per_eu(...) unused
 #define __shared__ static
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) ((const void*)&(x))
 inline char* emu_dyn_shared = nullptr;  // dynamic LDS of the running work-group
 #define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(emu_dyn_shared);
 
@@ -219,6 +221,16 @@ inline hipError_t hipGetDeviceCount(int* n)
     return hipSuccess;
 }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDevice(int* d)
+{
+    *d = 0;
+    return hipSuccess;
+}
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol)
+{
+    *p = const_cast<void*>(symbol);
+    return hipSuccess;
+}
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int)
 {
     p->multiProcessorCount              = 1;
