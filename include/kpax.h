@@ -307,6 +307,12 @@ int kpax_mlp_init_params(uint64_t seed, float* params);
  * whose results are then unchanged bit for bit. */
 int kpax_set_proposal(kpax_handle* h, const float* params, float mix);
 
+/* The proposal's Gaussian of every node of the last kpax_plan's tree: 4
+ * normalised means then 4 sigmas, SoA with column stride cap (>= tree size),
+ * as the expansion samples them (the network's outputs after its mapping).
+ * For replaying a proposal plan elsewhere, e.g. on a CPU restatement. */
+int kpax_get_proposal(kpax_handle* h, uint32_t cap, float* mean_sigma, uint32_t* tree_size);
+
 /* Budget policy of the proposal: queries whose tree budget (kpax_query.budget,
  * or max_tree for kpax_plan) is below min_budget sample uniformly, exactly as
  * the plain planner; the others use the mixture.  0 (the default) applies it

@@ -2032,6 +2032,21 @@ int kpax_get_tree(kpax_handle* h, uint32_t cap, float* states, float* controls, 
     return KPAX_OK;
 }
 
+int kpax_get_proposal(kpax_handle* h, uint32_t cap, float* mean_sigma, uint32_t* tree_size)
+{
+    if(!h || !mean_sigma) return fail(h, KPAX_INVALID_ARG, "null argument");
+    if(!h->planned) return fail(h, KPAX_INVALID_ARG, "no single plan has run on this handle");
+    if(!h->prop || h->mix <= 0.0f) return fail(h, KPAX_INVALID_ARG, "no learned proposal is set");
+    const uint32_t ts = h->last_tree;
+    if(tree_size) *tree_size = ts;
+    if(cap < ts) return fail(h, KPAX_INVALID_ARG, "cap is smaller than the tree");
+    HIPC(hipSetDevice(h->cfg.device));
+    for(int j = 0; j < 8; ++j)
+        HIPC(hipMemcpy(mean_sigma + (size_t)j * cap, h->prop + (size_t)j * h->T, (size_t)ts * 4,
+                       hipMemcpyDeviceToHost));
+    return KPAX_OK;
+}
+
 int kpax_get_regions(kpax_handle* h, int32_t* valid, int32_t* total, int32_t* covered, float* scores)
 {
     if(!h) return KPAX_INVALID_ARG;

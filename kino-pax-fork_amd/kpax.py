@@ -148,6 +148,7 @@ def lib():
         "kpax_get_regions": (i32, [P, P, P, P, P]),
         "kpax_get_grid": (i32, [P, P, P, u32, P, P]),
         "kpax_get_lds_grid": (i32, [P, u32, P, P]),
+        "kpax_get_proposal": (i32, [P, u32, P, P]),
         "kpax_repeat_expand_noalloc": (i32, [P, P, u32, P, u32, P]),
         "kpax_set_profiling": (i32, [P, i32]),
         "kpax_get_stats": (i32, [P, P]),
@@ -569,6 +570,15 @@ class KPAX:
         enqueued on the handle's stream, not waited for."""
         self._check(lib().kpax_repeat_expand_noalloc(self._h, ctypes.c_void_p(d_weights), int(n), ctypes.c_void_p(d_out),
                                                      int(out_cap), ctypes.c_void_p(d_total)), "repeat_expand_noalloc")
+
+    def proposal_table(self):
+        """Per-node Gaussians (8, n) of the last plan's tree under the learned proposal."""
+        ts = ctypes.c_uint32()
+        tree = self.tree()
+        n = tree["parents"].shape[0]
+        out = np.zeros((8, max(n, 1)), np.float32)
+        self._check(lib().kpax_get_proposal(self._h, max(n, 1), _p(out), ctypes.byref(ts)), "get_proposal")
+        return out[:, :ts.value]
 
     def lds_grid(self, env=0):
         """(dims, bytes) of environment env's LDS-staged grid; ((0, 0, 0), 0) when it has none."""

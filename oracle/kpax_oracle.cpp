@@ -674,6 +674,145 @@ struct PlanOut {
     uint64_t expansions = 0;
 };
 
+// ---------------------------------------------------------------------------
+// Learned control proposal: no counterpart in the reference (its controls are
+// uniform draws, statePropagatorSpatialHash.cu:236-241).  Restated from the
+// build's own definition: features of a node, the occupancy lattice they read
+// and the Gaussian sampler of a child's control.  Plain float operations in
+// source order (fma where the definition says fma), so the results are
+// bit-identical to the GPU's.
+// ---------------------------------------------------------------------------
+namespace prop {
+
+constexpr int kOcc = 16;
+
+struct Feat {
+    float mid[3], half[3], mn[3], cell[3], inv_pi, inv_vel, rate;
+    int S;
+};
+
+Feat feat_params(const kpax_config& c)
+{
+    Feat F;
+    for(int d = 0; d < 3; ++d)
+    {
+        F.mid[d]  = 0.5f * (c.ws_min[d] + c.ws_max[d]);
+        F.half[d] = 0.5f * (c.ws_max[d] - c.ws_min[d]);
+        F.mn[d]   = c.ws_min[d];
+        F.cell[d] = (c.ws_max[d] - c.ws_min[d]) / (float)kOcc;
+    }
+    F.inv_pi      = 0.31830988618379067f;
+    const float v = std::max(std::fabs(c.vel_min), std::fabs(c.vel_max));
+    F.inv_vel     = v > 0.0f ? 1.0f / v : 1.0f;
+    F.rate        = 0.25f;
+    F.S           = c.model == KPAX_MODEL_QUADCOPTER ? 12 : 6;
+    return F;
+}
+
+// cell (a, b, c) set when an obstacle overlaps its open box
+void occupancy(const Feat& F, const float* obs, uint32_t n, uint32_t* bits)
+{
+    for(int i = 0; i < kOcc * kOcc * kOcc / 32; ++i) bits[i] = 0;
+    for(int a = 0; a < kOcc; ++a)
+        for(int b = 0; b < kOcc; ++b)
+            for(int c = 0; c < kOcc; ++c)
+            {
+                const float lo[3] = {F.mn[0] + (float)a * F.cell[0], F.mn[1] + (float)b * F.cell[1],
+                                     F.mn[2] + (float)c * F.cell[2]};
+                const float hi[3] = {F.mn[0] + (float)(a + 1) * F.cell[0], F.mn[1] + (float)(b + 1) * F.cell[1],
+                                     F.mn[2] + (float)(c + 1) * F.cell[2]};
+                bool hit = false;
+                for(uint32_t o = 0; o < n && !hit; ++o)
+                {
+                    const float* q = obs + 6 * (size_t)o;
+                    hit = q[0] < hi[0] && q[3] > lo[0] && q[1] < hi[1] && q[4] > lo[1] && q[2] < hi[2] && q[5] > lo[2];
+                }
+                const int i = (a * kOcc + b) * kOcc + c;
+                if(hit) bits[i >> 5] |= 1u << (i & 31);
+            }
+}
+
+int cell_index(float x, float mn, float cell)
+{
+    float v = (x - mn) / cell + 0.5f;
+    if(!(v >= -8.0f)) v = -8.0f;
+    if(v > 24.0f) v = 24.0f;
+    return (int)std::floor(v) - 2;
+}
+
+float bit(const uint32_t* occ, int a, int b, int c)
+{
+    if(a < 0 || b < 0 || c < 0 || a >= kOcc || b >= kOcc || c >= kOcc) return 1.0f;
+    const int i = (a * kOcc + b) * kOcc + c;
+    return (occ[i >> 5] >> (i & 31)) & 1u ? 1.0f : 0.0f;
+}
+
+void features(const Feat& F, const uint32_t* occ, const float* x, const float* g, float* f)
+{
+    for(int d = 0; d < 3; ++d) f[d] = (x[d] - F.mid[d]) / F.half[d];
+    for(int d = 0; d < 3; ++d) f[3 + d] = x[3 + d] * F.inv_pi;
+    for(int d = 0; d < 3; ++d) f[6 + d] = x[6 + d] * F.inv_vel;
+    for(int d = 0; d < 3; ++d) f[9 + d] = x[9 + d] * F.rate;
+    for(int d = 0; d < 3; ++d) f[12 + d] = (g[d] - x[d]) / F.half[d];
+    const int a0 = cell_index(x[0], F.mn[0], F.cell[0]), b0 = cell_index(x[1], F.mn[1], F.cell[1]),
+              c0 = cell_index(x[2], F.mn[2], F.cell[2]);
+    for(int i = 0; i < 4; ++i)
+        for(int j = 0; j < 4; ++j)
+            for(int k = 0; k < 4; ++k) f[15 + 16 * i + 4 * j + k] = bit(occ, a0 + i, b0 + j, c0 + k);
+    f[79] = 0.0f;
+}
+
+// ln x for x in (0, 1]: exponent * ln 2 + 2 atanh((m-1)/(m+1)), odd series to s^11
+float ln_f32(float x)
+{
+    uint32_t bits;
+    std::memcpy(&bits, &x, 4);
+    int e               = (int)(bits >> 23) - 127;
+    const uint32_t mb   = (bits & 0x007fffffu) | 0x3f800000u;
+    float m;
+    std::memcpy(&m, &mb, 4);
+    if(m > 1.41421356f)
+    {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    const float s  = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p        = std::fma(s2, 0.09090909f, 0.11111111f);
+    p              = std::fma(s2, p, 0.14285714f);
+    p              = std::fma(s2, p, 0.2f);
+    p              = std::fma(s2, p, 0.33333333f);
+    const float lm = 2.0f * std::fma(s * s2, p, s);
+    return std::fma((float)e, 0.69314718f, lm);
+}
+
+void box_muller(uint32_t w0, uint32_t w1, float* z0, float* z1)
+{
+    const float u1 = 1.0f - u01(w0);
+    const float u2 = u01(w1);
+    const float r  = std::sqrt(-2.0f * ln_f32(u1));
+    float sn, cs;
+    sincos_f32(6.28318531f * u2, &sn, &cs);
+    *z0 = r * cs;
+    *z1 = r * sn;
+}
+
+void propose(const float* mean, const float* sigma, const uint32_t* w, const kpax_config& c, float* u)
+{
+    float z[4];
+    box_muller(w[0], w[1], &z[0], &z[1]);
+    box_muller(w[2], w[3], &z[2], &z[3]);
+    for(int j = 0; j < 4; ++j)
+    {
+        float un = mean[j] + sigma[j] * z[j];
+        un       = un < -1.0f ? -1.0f : (un > 1.0f ? 1.0f : un);
+        const float rng = c.ctrl_max[j] - c.ctrl_min[j];
+        u[j]     = c.ctrl_min[j] + (un + 1.0f) * 0.5f * rng;
+    }
+}
+
+}  // namespace prop
+
 // Per-iteration record of a plan (kTraceWords words per iteration).
 constexpr int kTraceWords = 8;
 enum : int { TR_W = 0, TR_FAN, TR_M, TR_ADMITTED, TR_TS, TR_FA_BITS, TR_GOAL, TR_NEXT_W };
@@ -712,6 +851,13 @@ struct Planner {
         score.assign(R.r1_count, 0.0f);
         weight.assign(N, 0u);
     }
+
+    // Learned control proposal (the build's own addition): with probability
+    // `mix` a child draws its control from its parent's Gaussian, given here
+    // per tree node as 8 x max_tree floats (4 means, 4 sigmas; the GPU's
+    // network outputs, injected), instead of uniformly.
+    const float* prop_table = nullptr;
+    float mix               = 0.0f;
 
     // trace: kTraceWords words per iteration for up to max_iter iterations (optional)
     PlanOut plan(const float* start, const float* goal, uint64_t seed, uint32_t query, uint32_t* trace = nullptr)
@@ -759,6 +905,17 @@ struct Planner {
                 const U4 ds        = draw(seed, cix, t, query, PURPOSE_STEPS_ADMIT);
                 float u[4];
                 for(int i = 0; i < C; ++i) u[i] = c.ctrl_min[i] + u01(dc.v[i]) * (c.ctrl_max[i] - c.ctrl_min[i]);
+                if(prop_table && mix > 0.0f && u01(ds.v[2]) < mix)
+                {
+                    const U4 dp = draw(seed, cix, t, query, 3u);  // the proposal's purpose word
+                    float mean[4], sg[4];
+                    for(int j = 0; j < 4; ++j)
+                    {
+                        mean[j] = prop_table[(size_t)j * N + p];
+                        sg[j]   = prop_table[(size_t)(4 + j) * N + p];
+                    }
+                    prop::propose(mean, sg, dp.v, c, u);
+                }
                 const int span = c.steps_max - c.steps_min + 1;
                 int k          = (int)(u01(ds.v[0]) * (float)span);
                 if(k > span - 1) k = span - 1;
@@ -1074,11 +1231,13 @@ uint64_t oracle_repeat_expand(const uint32_t* weights, uint32_t n, uint32_t* out
 // count in the last repeat list built) and trace (8 words per iteration for up
 // to max_iter iterations: W, fan, M, admitted, tree size after the append,
 // fAccept's float bits, goal index or -1, next W).
+// prop_table (8 x max_tree floats, optional) with mix > 0: the learned
+// proposal's per-node Gaussians, see Planner::prop_table.
 int oracle_plan_ex(const kpax_config* cfg, const float* obs, uint32_t n_obs, const float* start, const float* goal,
                    uint64_t seed, int32_t threads, uint32_t* info, float* states_soa, float* controls_soa,
                    int32_t* steps_out, int32_t* parents_out, int32_t* regions_out, int32_t* valid_out,
                    int32_t* total_out, int32_t* covered_out, float* scores_out, uint32_t* weights_out,
-                   uint32_t* trace)
+                   uint32_t* trace, const float* prop_table, float mix)
 {
 #ifdef _OPENMP
     if(threads > 0) omp_set_num_threads(threads);
@@ -1086,6 +1245,8 @@ int oracle_plan_ex(const kpax_config* cfg, const float* obs, uint32_t n_obs, con
     (void)threads;
 #endif
     Planner P(*cfg, obs, n_obs);
+    P.prop_table = prop_table;
+    P.mix        = mix;
     const PlanOut o = P.plan(start, goal, seed, 0, trace);
     if(weights_out) std::memcpy(weights_out, P.weight.data(), sizeof(uint32_t) * cfg->max_tree);
     const uint32_t N = cfg->max_tree;
@@ -1121,7 +1282,8 @@ int oracle_plan(const kpax_config* cfg, const float* obs, uint32_t n_obs, const 
                 int32_t* total_out, int32_t* covered_out, float* scores_out)
 {
     return oracle_plan_ex(cfg, obs, n_obs, start, goal, seed, threads, info, states_soa, controls_soa, steps_out,
-                          parents_out, regions_out, valid_out, total_out, covered_out, scores_out, nullptr, nullptr);
+                          parents_out, regions_out, valid_out, total_out, covered_out, scores_out, nullptr, nullptr,
+                          nullptr, 0.0f);
 }
 
 // The quadcopter's derivative (ode, statePropagator.cu:325-370) at one state
@@ -1162,144 +1324,6 @@ uint32_t oracle_new_node_repeat(const kpax_config* cfg, int32_t region_valid)
     return new_node_repeat(*cfg, region_valid);
 }
 
-// ---------------------------------------------------------------------------
-// Learned control proposal: no counterpart in the reference (its controls are
-// uniform draws, statePropagatorSpatialHash.cu:236-241).  Restated from the
-// build's own definition: features of a node, the occupancy lattice they read
-// and the Gaussian sampler of a child's control.  Plain float operations in
-// source order (fma where the definition says fma), so the results are
-// bit-identical to the GPU's.
-// ---------------------------------------------------------------------------
-namespace prop {
-
-constexpr int kOcc = 16;
-
-struct Feat {
-    float mid[3], half[3], mn[3], cell[3], inv_pi, inv_vel, rate;
-    int S;
-};
-
-Feat feat_params(const kpax_config& c)
-{
-    Feat F;
-    for(int d = 0; d < 3; ++d)
-    {
-        F.mid[d]  = 0.5f * (c.ws_min[d] + c.ws_max[d]);
-        F.half[d] = 0.5f * (c.ws_max[d] - c.ws_min[d]);
-        F.mn[d]   = c.ws_min[d];
-        F.cell[d] = (c.ws_max[d] - c.ws_min[d]) / (float)kOcc;
-    }
-    F.inv_pi      = 0.31830988618379067f;
-    const float v = std::max(std::fabs(c.vel_min), std::fabs(c.vel_max));
-    F.inv_vel     = v > 0.0f ? 1.0f / v : 1.0f;
-    F.rate        = 0.25f;
-    F.S           = c.model == KPAX_MODEL_QUADCOPTER ? 12 : 6;
-    return F;
-}
-
-// cell (a, b, c) set when an obstacle overlaps its open box
-void occupancy(const Feat& F, const float* obs, uint32_t n, uint32_t* bits)
-{
-    for(int i = 0; i < kOcc * kOcc * kOcc / 32; ++i) bits[i] = 0;
-    for(int a = 0; a < kOcc; ++a)
-        for(int b = 0; b < kOcc; ++b)
-            for(int c = 0; c < kOcc; ++c)
-            {
-                const float lo[3] = {F.mn[0] + (float)a * F.cell[0], F.mn[1] + (float)b * F.cell[1],
-                                     F.mn[2] + (float)c * F.cell[2]};
-                const float hi[3] = {F.mn[0] + (float)(a + 1) * F.cell[0], F.mn[1] + (float)(b + 1) * F.cell[1],
-                                     F.mn[2] + (float)(c + 1) * F.cell[2]};
-                bool hit = false;
-                for(uint32_t o = 0; o < n && !hit; ++o)
-                {
-                    const float* q = obs + 6 * (size_t)o;
-                    hit = q[0] < hi[0] && q[3] > lo[0] && q[1] < hi[1] && q[4] > lo[1] && q[2] < hi[2] && q[5] > lo[2];
-                }
-                const int i = (a * kOcc + b) * kOcc + c;
-                if(hit) bits[i >> 5] |= 1u << (i & 31);
-            }
-}
-
-int cell_index(float x, float mn, float cell)
-{
-    float v = (x - mn) / cell + 0.5f;
-    if(!(v >= -8.0f)) v = -8.0f;
-    if(v > 24.0f) v = 24.0f;
-    return (int)std::floor(v) - 2;
-}
-
-float bit(const uint32_t* occ, int a, int b, int c)
-{
-    if(a < 0 || b < 0 || c < 0 || a >= kOcc || b >= kOcc || c >= kOcc) return 1.0f;
-    const int i = (a * kOcc + b) * kOcc + c;
-    return (occ[i >> 5] >> (i & 31)) & 1u ? 1.0f : 0.0f;
-}
-
-void features(const Feat& F, const uint32_t* occ, const float* x, const float* g, float* f)
-{
-    for(int d = 0; d < 3; ++d) f[d] = (x[d] - F.mid[d]) / F.half[d];
-    for(int d = 0; d < 3; ++d) f[3 + d] = x[3 + d] * F.inv_pi;
-    for(int d = 0; d < 3; ++d) f[6 + d] = x[6 + d] * F.inv_vel;
-    for(int d = 0; d < 3; ++d) f[9 + d] = x[9 + d] * F.rate;
-    for(int d = 0; d < 3; ++d) f[12 + d] = (g[d] - x[d]) / F.half[d];
-    const int a0 = cell_index(x[0], F.mn[0], F.cell[0]), b0 = cell_index(x[1], F.mn[1], F.cell[1]),
-              c0 = cell_index(x[2], F.mn[2], F.cell[2]);
-    for(int i = 0; i < 4; ++i)
-        for(int j = 0; j < 4; ++j)
-            for(int k = 0; k < 4; ++k) f[15 + 16 * i + 4 * j + k] = bit(occ, a0 + i, b0 + j, c0 + k);
-    f[79] = 0.0f;
-}
-
-// ln x for x in (0, 1]: exponent * ln 2 + 2 atanh((m-1)/(m+1)), odd series to s^11
-float ln_f32(float x)
-{
-    uint32_t bits;
-    std::memcpy(&bits, &x, 4);
-    int e               = (int)(bits >> 23) - 127;
-    const uint32_t mb   = (bits & 0x007fffffu) | 0x3f800000u;
-    float m;
-    std::memcpy(&m, &mb, 4);
-    if(m > 1.41421356f)
-    {
-        m = m * 0.5f;
-        e = e + 1;
-    }
-    const float s  = (m - 1.0f) / (m + 1.0f);
-    const float s2 = s * s;
-    float p        = std::fma(s2, 0.09090909f, 0.11111111f);
-    p              = std::fma(s2, p, 0.14285714f);
-    p              = std::fma(s2, p, 0.2f);
-    p              = std::fma(s2, p, 0.33333333f);
-    const float lm = 2.0f * std::fma(s * s2, p, s);
-    return std::fma((float)e, 0.69314718f, lm);
-}
-
-void box_muller(uint32_t w0, uint32_t w1, float* z0, float* z1)
-{
-    const float u1 = 1.0f - u01(w0);
-    const float u2 = u01(w1);
-    const float r  = std::sqrt(-2.0f * ln_f32(u1));
-    float sn, cs;
-    sincos_f32(6.28318531f * u2, &sn, &cs);
-    *z0 = r * cs;
-    *z1 = r * sn;
-}
-
-void propose(const float* mean, const float* sigma, const uint32_t* w, const kpax_config& c, float* u)
-{
-    float z[4];
-    box_muller(w[0], w[1], &z[0], &z[1]);
-    box_muller(w[2], w[3], &z[2], &z[3]);
-    for(int j = 0; j < 4; ++j)
-    {
-        float un = mean[j] + sigma[j] * z[j];
-        un       = un < -1.0f ? -1.0f : (un > 1.0f ? 1.0f : un);
-        const float rng = c.ctrl_max[j] - c.ctrl_min[j];
-        u[j]     = c.ctrl_min[j] + (un + 1.0f) * 0.5f * rng;
-    }
-}
-
-}  // namespace prop
 
 void oracle_occupancy(const kpax_config* cfg, const float* obs, uint32_t n_obs, uint32_t* bits)
 {

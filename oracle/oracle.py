@@ -31,7 +31,7 @@ def _sigs():
         "oracle_f_accept": (f32, [P, u32, u32]),
         "oracle_repeat_expand": (u64, [P, u32, P, u64]),
         "oracle_plan": (i32, [P, P, u32, P, P, u64, i32, P, P, P, P, P, P, P, P, P, P]),
-        "oracle_plan_ex": (i32, [P, P, u32, P, P, u64, i32, P, P, P, P, P, P, P, P, P, P, P, P]),
+        "oracle_plan_ex": (i32, [P, P, u32, P, P, u64, i32, P, P, P, P, P, P, P, P, P, P, P, P, P, f32]),
         "oracle_fan_out": (None, [P, u64, u32, P, P]),
         "oracle_admit_by_score": (i32, [u32, f32]),
         "oracle_readmit": (i32, [u32, f32, f32]),
@@ -190,12 +190,15 @@ def repeat_expand(weights, cap=None):
     return out[: min(cap, total)], int(total)
 
 
-def plan(cfg, obstacles, start, goal, seed, threads=0, want_tree=True, want_trace=False, L=None):
+def plan(cfg, obstacles, start, goal, seed, threads=0, want_tree=True, want_trace=False, L=None, proposal=None,
+         mix=0.0):
     """Whole planner; returns a dict with status, iterations, tree arrays,
     counters.  want_trace adds "weights" (every node's repeat count in the last
     repeat list built) and "trace" (per iteration: W, fan, M, admitted, tree
     size after the append, fAccept, goal index or -1, next W).  L: an oracle
-    build from load() instead of the default one."""
+    build from load() instead of the default one.  proposal (8 x n floats:
+    per tree node 4 means then 4 sigmas, n <= max_tree) with mix > 0: children
+    draw their control from the parent's Gaussian with probability mix."""
     L = lib() if L is None else L
     o, no = _obs(obstacles)
     N = int(cfg.max_tree)
@@ -216,9 +219,14 @@ def plan(cfg, obstacles, start, goal, seed, threads=0, want_tree=True, want_trac
     sc = np.zeros(R1, np.float32)
     wts = np.zeros(N, np.uint32) if want_trace else None
     tr = np.zeros((max(int(cfg.max_iter), 1), 8), np.uint32) if want_trace else None
+    table = None
+    if proposal is not None:
+        proposal = np.asarray(proposal, np.float32)
+        table = np.zeros((8, N), np.float32)
+        table[:, :proposal.shape[1]] = proposal
     status = L.oracle_plan_ex(ctypes.byref(cfg), _p(o), no, _p(x0), _p(g), int(seed) & (2**64 - 1), int(threads),
                               _p(info), _p(st), _p(ct), _p(steps), _p(par), _p(reg), _p(v), _p(t), _p(c), _p(sc),
-                              _p(wts), _p(tr))
+                              _p(wts), _p(tr), _p(table), float(mix if table is not None else 0.0))
     ts = int(info[1])
     out = {"status": int(status), "iterations": int(info[0]), "tree_size": ts,
            "goal_index": int(np.int32(info[2])), "expansions": int(info[3]) | (int(info[4]) << 32),

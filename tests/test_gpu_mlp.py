@@ -7,10 +7,13 @@ checked against the numpy oracle, which rounds to bf16 at the same points, to
 a relative error of 1e-2 (the accumulation order and the last bit of tanh
 differ; a wrong lane layout or operand gives errors of order 1).
 """
+import os
+
 import numpy as np
 import pytest
 
 import mlp_oracle
+from conftest import ROOT
 from workloads import random_states, workload
 
 pytestmark = pytest.mark.gpu
@@ -172,3 +175,66 @@ def test_proposal_budget_policy(kpax):
         assert (r.status, r.tree_size, r.expansions, r.iterations) == (o.status, o.tree_size, o.expansions,
                                                                        o.iterations), q.budget
         assert np.array_equal(r.rows.view(np.uint32), o.rows.view(np.uint32))
+
+
+PARAMS = os.path.join(ROOT, "kino-pax-fork_amd", "data", "proposal_forest.npy")
+
+
+@pytest.mark.parametrize("budget,seed", [(1 << 17, 21), (1 << 18, 22)])
+def test_proposal_plan_matches_oracle_with_injected_gaussians(kpax, oracle, budget, seed):
+    """A plan with the shipped network mixed in (50/50) equals the oracle's plan
+    tree for tree when the oracle is given each node's Gaussian as the GPU's
+    network produced it (the network's own bf16 arithmetic is pinned against
+    its float64 oracle in the tests above): controls drawn from the proposal,
+    the mixture draws and everything downstream agree bit for bit."""
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=budget)
+    params = np.load(PARAMS)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        p.set_proposal(params, 0.5)
+        r = p.plan(start, goal, seed=seed)
+        tree = p.tree()
+        table = p.proposal_table()
+    o = oracle.plan(cfg, obs, start, goal, seed=seed, threads=16, proposal=table, mix=0.5)
+    print(f"proposal plan {budget}: gpu {r.status} it {r.iterations} ts {r.tree_size}; oracle {o['status']} "
+          f"it {o['iterations']} ts {o['tree_size']}", flush=True)
+    assert (r.status, r.iterations, r.tree_size, r.expansions) == (o["status"], o["iterations"], o["tree_size"],
+                                                                   o["expansions"])
+    assert np.array_equal(tree["parents"], o["parents"])
+    assert np.array_equal(tree["states"].view(np.uint32), o["states"].view(np.uint32))
+    assert np.array_equal(tree["controls"].view(np.uint32), o["controls"].view(np.uint32))
+    # the plan really used the proposal: the uniform sampler never hits the thrust bound exactly
+    assert np.mean(tree["controls"][0, 1:] == np.float32(cfg.ctrl_max[0])) > 0.005
+    # and the oracle with no proposal differs
+    plain = oracle.plan(cfg, obs, start, goal, seed=seed, threads=16, want_tree=False)
+    assert (plain["tree_size"], plain["expansions"]) != (o["tree_size"], o["expansions"])
+
+
+def test_trained_proposal_gains_over_uniform_sampling(kpax):
+    """The shipped network (trained on plain-planner paths in other forests),
+    mixed 50/50 with uniform sampling, on 96 held-out queries in 8 held-out
+    forests at a 2^18-node budget: it solves at least as many queries as the
+    uniform sampler and, on the queries both solve, needs fewer iterations and
+    fewer expansions on average."""
+    import proposal
+    cfg = workload(2)[0]
+    cfg.max_tree = 1 << 18
+    cfg.max_iter = 100
+    envs = proposal.forest_environments(cfg, 8, seed=2)
+    qs = proposal.serving_queries(cfg, envs, 96, seed=4)
+    params = np.load(PARAMS)
+    with kpax.KPAX(cfg) as p:
+        for e, o in enumerate(envs):
+            p.set_environment(e, o)
+        plain = proposal.evaluate(p, qs, None, phases=False)
+        learned = proposal.evaluate(p, qs, params, 0.5, phases=False)
+    both = [(a, b) for a, b in zip(plain["results"], learned["results"]) if a.status == 0 and b.status == 0]
+    it_p = np.mean([a.iterations for a, _ in both])
+    it_l = np.mean([b.iterations for _, b in both])
+    ex_p = np.mean([a.expansions for a, _ in both])
+    ex_l = np.mean([b.expansions for _, b in both])
+    print(f"success plain {plain['success']:.3f} learned {learned['success']:.3f}; on {len(both)} both solve: "
+          f"iterations {it_p:.2f} -> {it_l:.2f}, expansions {ex_p:.0f} -> {ex_l:.0f}", flush=True)
+    assert learned["success"] >= plain["success"] and learned["success"] >= 0.95
+    assert len(both) >= 80
+    assert it_l < 0.95 * it_p and ex_l < ex_p
