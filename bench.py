@@ -58,11 +58,11 @@ def bytes_per_expansion(S, C):
 # HBM bytes per expansion that k_expand<quadcopter> moves on this workload,
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (average per
 # dispatch over the expansions per dispatch; FETCH scaled by 2, WRITE by 1,
-# the factors that make a 1 GiB device copy read and write 1 GiB): 237 B read
+# the factors that make a 1 GiB device copy read and write 1 GiB): 163 B read
 # (the child's 32-byte prepared record; parents read in longest-edge-first
-# order, a cache line per state word; the grid's inline obstacle boxes) and
-# 95 B written (the 80-byte child and its share of partial lines).
-MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 237 + 95
+# order, a cache line per state word; each work-group's 22 KB grid image) and
+# 97 B written (the 80-byte child and its share of partial lines).
+MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 163 + 97
 
 
 # Algorithmic FP32 operations per integration step: the reference's step
@@ -94,6 +94,26 @@ def describe(k, cfg, n_obs):
     l1, l2, g = cfg.r1_len[0], cfg.r2_len[0], cfg.grid_dims[0]
     return (f"{MODEL_NAME[cfg.model][0]}, {n_obs}-obstacle {ENV_NAME[k]}, {ws}, RK4 dt={cfg.dt:g}, "
             f"2^{int(cfg.max_tree).bit_length() - 1}-node tree cap, {l1}^3 regions x {l2}^3 sub-regions, {g}^3 grid")
+
+
+def dubins_environments(launch, plans=10):
+    """Outcomes of the Dubins maze in four seeded mazes, at the reference's
+    300-iteration limit and with the 2^21-node tree cap as the only budget."""
+    out = {}
+    for env in range(4):
+        row = {}
+        for max_iter in (300, 3000):
+            cfg, obs, start, goal = workload(3, seed=env, max_iter=max_iter)
+            with kpax.KPAX(cfg) as p:
+                p.set_obstacles(obs)
+                if launch:
+                    p.set_launch(**launch)
+                res = [p.plan(start, goal, seed=7000 + s) for s in range(plans)]
+            row[f"max_iter_{max_iter}"] = {name: sum(r.status == code for r in res) for name, code in
+                                           (("found", kpax.OK), ("iteration_limit", kpax.NOT_FOUND),
+                                            ("tree_cap", kpax.BUDGET_EXHAUSTED))}
+        out[f"env_seed_{env}"] = row
+    return out
 
 
 def host_cpu():
@@ -415,7 +435,8 @@ def main():
     }
     if not args.no_others:
         line["other_workloads"] = {"double_integrator": other_workload(1, launch=overrides),
-                                   "dubins_maze": other_workload(3, launch=overrides)}
+                                   "dubins_maze": other_workload(3, launch=overrides),
+                                   "dubins_maze_by_environment": dubins_environments(overrides)}
     if not args.no_batch:
         line["batched"] = batched(cfg)
     if not args.no_proposal:

@@ -161,12 +161,36 @@ struct ExpandArgs {
 // the tree grows by at most M) into offsets, resets the work tickets and
 // moves the scan epoch on.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, BatchCtl* bc, uint32_t* child_off,
-                                                  uint32_t* atile_off, uint32_t* ftile_off, int32_t* slot_status,
-                                                  uint32_t Q, uint64_t* st_append, uint64_t* st_frontier,
-                                                  uint32_t n_status, PlanParams PP, const GridView* envs,
-                                                  SlotUniform* s0, uint32_t tile, uint32_t* tickets)
+struct AdvanceArgs {
+    Ctl* ctl;
+    const QueryDev* qd;
+    BatchCtl* bc;
+    uint32_t *child_off, *atile_off, *ftile_off;
+    int32_t* slot_status;
+    uint32_t Q;
+    uint64_t *st_append, *st_frontier;
+    uint32_t n_status;
+    PlanParams PP;
+    const GridView* envs;
+    SlotUniform* s0;
+    uint32_t tile;
+    uint32_t* tickets;
+};
+
+// The advance, by one block of 1024 threads (see k_advance).
+__device__ __forceinline__ void advance_block(const AdvanceArgs& V)
 {
+    Ctl* ctl = V.ctl;
+    const QueryDev* qd = V.qd;
+    BatchCtl* bc = V.bc;
+    uint32_t *child_off = V.child_off, *atile_off = V.atile_off, *ftile_off = V.ftile_off;
+    int32_t* slot_status = V.slot_status;
+    const uint32_t Q = V.Q, n_status = V.n_status, tile = V.tile;
+    uint64_t *st_append = V.st_append, *st_frontier = V.st_frontier;
+    const PlanParams& PP = V.PP;
+    const GridView* envs = V.envs;
+    SlotUniform* s0 = V.s0;
+    uint32_t* tickets = V.tickets;
     if(threadIdx.x < kTicketGroups) tickets[threadIdx.x * kTicketStride] = 0;
     __shared__ uint32_t lds[16];
     __shared__ uint32_t s_epoch;
@@ -232,6 +256,8 @@ __global__ __launch_bounds__(1024) void k_advance(Ctl* ctl, const QueryDev* qd, 
             st_frontier[i] = 0;
         }
 }
+
+__global__ __launch_bounds__(1024) void k_advance(AdvanceArgs V) { advance_block(V); }
 
 // ---------------------------------------------------------------------------
 // Expansion: propagateFrontier_kernel1/2 (KPAX.cu:358-439) with counter-based
@@ -484,16 +510,41 @@ __device__ __forceinline__ uint32_t child_bucket(const ExpandArgs* A, uint32_t k
     return order_bucket(A->P, draw_steps(A->P, ds.x));
 }
 
+// advance = 1 (the single-slot form): block 0 first runs the iteration's
+// advance (k_advance, one launch fewer per iteration), and every block takes
+// the iteration record from the slot's `nxt`, which this launch does not
+// change, instead of the argument block's copy, which block 0 rewrites.
 template<bool MULTI>
-__global__ __launch_bounds__(kOrderThreads) void k_order_count(const ExpandArgs* __restrict__ A, uint32_t* hist)
+__global__ __launch_bounds__(kOrderThreads) void k_order_count(const ExpandArgs* __restrict__ A, uint32_t* hist,
+                                                               AdvanceArgs V, int advance)
 {
     __shared__ uint32_t h[kOrderBuckets];
+    if constexpr(!MULTI)
+        if(advance && blockIdx.x == 0) advance_block(V);
     if(threadIdx.x < kOrderBuckets) h[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t total = order_total<MULTI>(A);
-    const uint32_t per   = (total + gridDim.x - 1) / gridDim.x;
-    const uint32_t b0 = blockIdx.x * per, b1 = b0 + per < total ? b0 + per : total;
-    for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x) atomicAdd(&h[child_bucket<MULTI>(A, k)], 1u);
+    uint32_t total;
+    if(!MULTI && advance)
+    {
+        const Iter n = A->ctl[0].nxt;
+        total        = n.status == kRunning ? n.M : 0u;
+        const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+        const uint32_t b0 = blockIdx.x * per, b1 = b0 + per < total ? b0 + per : total;
+        const uint64_t seed = A->qd[0].seed;
+        const uint32_t query = A->qd[0].query;
+        for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x)
+        {
+            const uint4 ds = philox(k, n.t, query, PURPOSE_STEPS_ADMIT, (uint32_t)seed, (uint32_t)(seed >> 32));
+            atomicAdd(&h[order_bucket(A->P, draw_steps(A->P, ds.x))], 1u);
+        }
+    }
+    else
+    {
+        total              = order_total<MULTI>(A);
+        const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+        const uint32_t b0 = blockIdx.x * per, b1 = b0 + per < total ? b0 + per : total;
+        for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x) atomicAdd(&h[child_bucket<MULTI>(A, k)], 1u);
+    }
     __syncthreads();
     if(threadIdx.x < kOrderBuckets) hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
@@ -625,44 +676,56 @@ __global__ __launch_bounds__(kOrderThreads) void k_prepare(const ExpandArgs* __r
 // valid child.  One block of 1024 threads per slot; with ctl == nullptr one
 // block scores the arrays as given (parity operator).
 // ---------------------------------------------------------------------------
+// The scores of one slot's regions by one block of NT threads (NT divides
+// 1024): raw scores into sc (LDS or global), the 1024 strided partial sums
+// and their pairwise tree in part (LDS, 1024 floats), then the normalised
+// scores in place.  Returns the total.  Contains barriers.
+template<int NT>
+__device__ __forceinline__ float block_scores(const int32_t* va, const int32_t* to, const int32_t* co, float* sc,
+                                              float* part, const RegionParams& RP, float eps)
+{
+    const int t = threadIdx.x;
+    for(int pi = t; pi < 1024; pi += NT)
+    {
+        float s = 0.0f;
+        for(int r = pi; r < RP.r1_count; r += 1024)
+        {
+            float raw   = 0.0f;
+            const int v = va[r];
+            if(v > 0)
+            {
+                const int n0    = to[r];
+                const float cov = (float)co[r] / (float)RP.r2_per;
+                const float ev  = eps + (float)v;
+                const float fv  = ev / (ev + (float)(n0 - v)) * RP.vol;
+                const float n   = (float)n0;
+                raw             = (fv * fv) * (fv * fv) / ((1.0f + cov) * (1.0f + n * n));
+            }
+            sc[r] = raw;
+            s += raw;
+        }
+        part[pi] = s;
+    }
+    __syncthreads();
+    for(int off = 512; off >= 1; off >>= 1)
+    {
+        for(int pi = t; pi < off; pi += NT) part[pi] = part[pi] + part[pi + off];
+        __syncthreads();
+    }
+    const float tot = part[0];
+    for(int r = t; r < RP.r1_count; r += NT) sc[r] = va[r] == 0 ? 1.0f : eps + sc[r] / tot;
+    __syncthreads();
+    return tot;
+}
+
 __global__ __launch_bounds__(1024) void k_score(Ctl* ctl, RegionView rv, RegionParams RP, float eps)
 {
     __shared__ float part[1024];
     const uint32_t q = blockIdx.x;
     if(ctl && ctl[q].cur.status != kRunning) return;
-    const size_t o    = (size_t)q * RP.r1_count;
-    const int32_t* va = rv.valid + o;
-    const int32_t* to = rv.total + o;
-    const int32_t* co = rv.covered + o;
-    float* sc         = rv.score + o;
-    const int t       = threadIdx.x;
-    float s           = 0.0f;
-    for(int r = t; r < RP.r1_count; r += 1024)
-    {
-        float raw   = 0.0f;
-        const int v = va[r];
-        if(v > 0)
-        {
-            const int n0    = to[r];
-            const float cov = (float)co[r] / (float)RP.r2_per;
-            const float ev  = eps + (float)v;
-            const float fv  = ev / (ev + (float)(n0 - v)) * RP.vol;
-            const float n   = (float)n0;
-            raw             = (fv * fv) * (fv * fv) / ((1.0f + cov) * (1.0f + n * n));
-        }
-        sc[r] = raw;
-        s += raw;
-    }
-    part[t] = s;
-    __syncthreads();
-    for(int off = 512; off >= 1; off >>= 1)
-    {
-        if(t < off) part[t] = part[t] + part[t + off];
-        __syncthreads();
-    }
-    const float tot = part[0];
-    for(int r = t; r < RP.r1_count; r += 1024) sc[r] = va[r] == 0 ? 1.0f : eps + sc[r] / tot;
-    if(t == 0 && ctl) ctl[q].score_total = tot;
+    const size_t o  = (size_t)q * RP.r1_count;
+    const float tot = block_scores<1024>(rv.valid + o, rv.total + o, rv.covered + o, rv.score + o, part, RP, eps);
+    if(threadIdx.x == 0 && ctl) ctl[q].score_total = tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -826,17 +889,34 @@ __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, Bat
 // run-length expanded into the repeat list (Planner.cu:52-80).  A slot whose
 // goal was reached ends here instead.
 // ---------------------------------------------------------------------------
-template<int NT>
+// SCORE (the single-slot form, R1 <= kLdsRegions): the scores of this
+// iteration (Graph::updateVertices) are computed here, by every working block
+// into LDS for its own re-admission draws, and written out by block 0 for the
+// next expansion, instead of by k_score (one launch fewer per iteration).
+template<int NT, bool SCORE>
 __global__ __launch_bounds__(NT) void k_frontier(Ctl* ctl, const QueryDev* qd, BatchCtl* bc,
                                                  const uint32_t* ftile_off, int32_t* slot_status, uint32_t Q,
                                                  TreeView tree, uint32_t* R, RegionView rv, uint64_t* status,
-                                                 PlanParams PP, uint32_t R1)
+                                                 PlanParams PP, uint32_t R1, RegionParams RP)
 {
     constexpr uint32_t kT = 4 * NT;  // nodes per tile
     __shared__ uint32_t lds[NT / 64];
     __shared__ uint32_t s_tile, s_pre;
+    HIP_DYNAMIC_SHARED(float, s_dyn)
     const uint32_t ntiles = bc->ftiles, epoch = bc->epoch;
     if(blockIdx.x >= ntiles) return;  // no ticket to take: leave the counter to the blocks with work
+    const float* lscore = nullptr;
+    if constexpr(SCORE)
+    {
+        // s_dyn: R1 scores then 1024 partial sums
+        const float tot = block_scores<NT>(rv.valid, rv.total, rv.covered, s_dyn, s_dyn + R1, RP, PP.eps);
+        if(blockIdx.x == 0)
+        {
+            for(uint32_t r = threadIdx.x; r < R1; r += NT) rv.score[r] = s_dyn[r];
+            if(threadIdx.x == 0) ctl[0].score_total = tot;
+        }
+        lscore = s_dyn;
+    }
     for(;;)
     {
         const uint32_t tile = next_tile(&bc->tile_frontier, &s_tile);
@@ -865,7 +945,7 @@ __global__ __launch_bounds__(NT) void k_frontier(Ctl* ctl, const QueryDev* qd, B
         const float fa         = ctl[q].f_accept;
         const size_t base      = d.base;
         const uint32_t k0      = (uint32_t)d.seed, k1 = (uint32_t)(d.seed >> 32);
-        const float* score     = rv.score + (size_t)q * R1;
+        const float* score     = SCORE ? lscore : rv.score + (size_t)q * R1;
         const int32_t* valid   = rv.valid + (size_t)q * R1;
         const uint32_t nbase   = lt * kT + threadIdx.x * 4;
         uint32_t w[4], cnt = 0;
@@ -1038,20 +1118,57 @@ void launch_expand_model(kpax_handle* h)
 
 // The children's records for this iteration (k_prepare), in longest-first
 // order after the bucket counts of k_order_count, or in child order.
+// Work-groups of the ordering and preparation passes: 256 (one per CU on
+// MI355X), fewer on a device with fewer CUs.
+uint32_t order_blocks(const kpax_handle* h)
+{
+    return std::min<uint32_t>(kOrderBlocks, std::max<uint32_t>(1u, (uint32_t)h->cu_count));
+}
+
+// The advance runs inside the ordering pass (single-slot handles with the
+// longest-first order) instead of as its own launch.
+bool fused_advance(const kpax_handle* h) { return h->Qmax == 1 && h->order_children; }
+
+AdvanceArgs advance_args(const kpax_handle* h)
+{
+    AdvanceArgs V;
+    V.ctl         = h->ctl;
+    V.qd          = h->qd;
+    V.bc          = h->bc;
+    V.child_off   = h->child_off;
+    V.atile_off   = h->atile_off;
+    V.ftile_off   = h->ftile_off;
+    V.slot_status = h->slot_status;
+    V.Q           = h->Qmax;
+    V.st_append   = h->st_append;
+    V.st_frontier = h->st_frontier;
+    V.n_status    = h->n_status;
+    V.PP          = h->PP;
+    V.envs        = h->envs_dev;
+    V.s0          = &((ExpandArgs*)h->expand_args)->s0;
+    V.tile        = h->scan_tile;
+    V.tickets     = h->tickets;
+    return V;
+}
+
 template<int MODEL>
 void launch_prepare(kpax_handle* h)
 {
     const ExpandArgs* A  = (const ExpandArgs*)h->expand_args;
     const uint32_t* hist = h->order_children ? h->order_hist : nullptr;
+    const dim3 ob(order_blocks(h));
+    const AdvanceArgs V = advance_args(h);
     if(h->Qmax == 1)
     {
-        if(hist) hipLaunchKernelGGL((k_order_count<false>), dim3(kOrderBlocks), dim3(kOrderThreads), 0, h->stream, A, h->order_hist);
-        hipLaunchKernelGGL((k_prepare<MODEL, false>), dim3(kOrderBlocks), dim3(kOrderThreads), 0, h->stream, A, hist);
+        if(hist)
+            hipLaunchKernelGGL((k_order_count<false>), ob, dim3(kOrderThreads), 0, h->stream, A, h->order_hist, V,
+                               fused_advance(h) ? 1 : 0);
+        hipLaunchKernelGGL((k_prepare<MODEL, false>), ob, dim3(kOrderThreads), 0, h->stream, A, hist);
     }
     else
     {
-        if(hist) hipLaunchKernelGGL((k_order_count<true>), dim3(kOrderBlocks), dim3(kOrderThreads), 0, h->stream, A, h->order_hist);
-        hipLaunchKernelGGL((k_prepare<MODEL, true>), dim3(kOrderBlocks), dim3(kOrderThreads), 0, h->stream, A, hist);
+        if(hist) hipLaunchKernelGGL((k_order_count<true>), ob, dim3(kOrderThreads), 0, h->stream, A, h->order_hist, V, 0);
+        hipLaunchKernelGGL((k_prepare<MODEL, true>), ob, dim3(kOrderThreads), 0, h->stream, A, hist);
     }
 }
 
@@ -1086,12 +1203,22 @@ void launch_append(kpax_handle* h)
                        (uint32_t)h->RP.r1_count, h->R2);
 }
 
+// Scores computed inside the frontier pass (single-slot handles whose R1
+// regions fit the LDS) instead of by k_score.
+bool fused_scores(const kpax_handle* h) { return h->Qmax == 1 && h->RP.r1_count <= kLdsRegions; }
+
 template<int NT>
 void launch_frontier(kpax_handle* h)
 {
-    hipLaunchKernelGGL(k_frontier<NT>, dim3(h->scan_blocks), dim3(NT), 0, h->stream, h->ctl, h->qd, h->bc,
-                       h->ftile_off, h->slot_status, h->Qmax, h->tree, h->R, h->rv, h->st_frontier, h->PP,
-                       (uint32_t)h->RP.r1_count);
+    if(fused_scores(h))
+        hipLaunchKernelGGL((k_frontier<NT, true>), dim3(h->scan_blocks), dim3(NT),
+                           ((size_t)h->RP.r1_count + 1024) * sizeof(float), h->stream, h->ctl, h->qd, h->bc,
+                           h->ftile_off, h->slot_status, h->Qmax, h->tree, h->R, h->rv, h->st_frontier, h->PP,
+                           (uint32_t)h->RP.r1_count, h->RP);
+    else
+        hipLaunchKernelGGL((k_frontier<NT, false>), dim3(h->scan_blocks), dim3(NT), 0, h->stream, h->ctl, h->qd,
+                           h->bc, h->ftile_off, h->slot_status, h->Qmax, h->tree, h->R, h->rv, h->st_frontier,
+                           h->PP, (uint32_t)h->RP.r1_count, h->RP);
 }
 
 // The five launches of one iteration and the copy of the slot statuses into
@@ -1100,9 +1227,7 @@ void launch_frontier(kpax_handle* h)
 int enqueue_iteration(kpax_handle* h, int p, bool ev)
 {
     const uint32_t Q = h->Qmax;
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, h->ctl, h->qd, h->bc, h->child_off, h->atile_off,
-                       h->ftile_off, h->slot_status, Q, h->st_append, h->st_frontier, h->n_status, h->PP, h->envs_dev,
-                       &((ExpandArgs*)h->expand_args)->s0, h->scan_tile, h->tickets);
+    if(!fused_advance(h)) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, advance_args(h));
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
     int rc = launch_expand(h, ev ? h->ph_ev[p][1] : nullptr);
     if(rc != KPAX_OK) return rc;
@@ -1116,7 +1241,7 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][3], h->stream));
     if(h->mix > 0.0f && (rc = launch_propose(h, 0)) != KPAX_OK) return rc;
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
-    hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
+    if(!fused_scores(h)) hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][5], h->stream));
     switch(h->scan_tile)
     {
