@@ -24,7 +24,7 @@ PKG = os.path.join(ROOT, "kino-pax-fork_amd")
 
 def _example_source():
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
-    m = re.search(r"`examples/gpu/main\.cu` becomes:\s*```cpp\n(.*?)```", text, re.S)
+    m = re.search(r"`examples/gpu/main\.cu` becomes.*?```cpp\n(.*?)```", text, re.S)
     assert m, "INTEGRATION.md holds no main.cpp example"
     return m.group(1)
 
