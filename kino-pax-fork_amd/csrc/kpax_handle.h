@@ -60,9 +60,12 @@ struct Env {
     uint32_t* occ   = nullptr;  // occupancy bitmap of the learned proposal's features
     uint4* lds_img  = nullptr;  // the grid as a work-group stages it into LDS (dev::LdsGrid)
     dev::LdsGrid lg{};
+    bool has_lds    = false;    // lds_img holds the current obstacles' image
     uint32_t n_obs  = 0;
     uint32_t k      = 0;
     std::vector<float> host;
+    // capacities (items) of the buffers above, kept across rebuilds
+    size_t lo_cap = 0, start_cap = 0, items_cap = 0, box_cap = 0, lds_cap = 0;
 };
 
 // LDS image of an environment (built on the host from its boxes): the
@@ -124,6 +127,7 @@ struct kpax_handle {
     uint32_t lds_bytes      = 0;        // dynamic LDS of the single-query expansion: the largest image in use
     bool launch_set         = false;    // the launch shape came from kpax_set_launch (kept across environments)
     uint64_t* noalloc       = nullptr;  // device address of the allocation-free operators' scratch array
+    int32_t* grid_scratch   = nullptr;  // counts, cursor, look-back words and ticket of the grid build
     bool order_children     = true;     // hand out children longest drawn edge first (default: single-query handles)
     uint32_t* tickets       = nullptr;  // kTicketGroups expansion chunk counters
     uint32_t refill_lanes   = 16;       // idle lanes that trigger a wave's service pass
@@ -208,6 +212,7 @@ inline dev::GridView grid_view(const kpax_handle* h, uint32_t env)
     g.box   = h->envs[env].box;
     g.occ   = h->envs[env].occ;
     g.lg    = h->envs[env].lg;
+    if(!h->envs[env].has_lds) g.lg.bytes = 0;
     g.lds   = nullptr;
     return g;
 }

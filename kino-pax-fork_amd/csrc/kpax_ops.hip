@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_noalloc_clear(uint64_t* words, uint32_t
 GridView operator_grid(const kpax_handle* h)
 {
     GridView G = grid_view(h, 0);
-    if(!h->use_lds || !h->envs[0].lds_img) G.lg.bytes = 0;
+    if(!h->use_lds || !h->envs[0].has_lds) G.lg.bytes = 0;
     return G;
 }
 
@@ -296,28 +296,35 @@ std::vector<uint4> build_lds_image(const kpax_config& cfg, const float* aabb6, u
         ext[d] = cfg.ws_max[d] - cfg.ws_min[d];
     }
     const size_t box_bytes = align16(24 * (size_t)n);
-    int best[3]            = {0, 0, 0};
-    double best_cost       = 0.0;
-    size_t best_bytes      = 0;
-    for(int a : cand[0])
-        for(int b : cand[1])
-            for(int c : cand[2])
+    // cells each obstacle spans per axis, for every candidate dimension
+    std::vector<uint32_t> span[3];
+    for(int d = 0; d < 3; ++d)
+    {
+        span[d].resize(cand[d].size() * (size_t)n);
+        for(size_t j = 0; j < cand[d].size(); ++j)
+        {
+            const float inv = (float)cand[d][j] / ext[d];
+            for(uint32_t i = 0; i < n; ++i)
+                span[d][j * n + i] = (uint32_t)(dev::cell_of(aabb6[6 * (size_t)i + 3 + d], mn[d], inv, cand[d][j]) -
+                                                dev::cell_of(aabb6[6 * (size_t)i + d], mn[d], inv, cand[d][j]) + 1);
+        }
+    }
+    int best[3]       = {0, 0, 0};
+    double best_cost  = 0.0;
+    size_t best_bytes = 0;
+    for(size_t ja = 0; ja < cand[0].size(); ++ja)
+        for(size_t jb = 0; jb < cand[1].size(); ++jb)
+            for(size_t jc = 0; jc < cand[2].size(); ++jc)
             {
-                const int D[3]  = {a, b, c};
+                const int a = cand[0][ja], b = cand[1][jb], c = cand[2][jc];
                 const size_t G  = (size_t)a * b * c;
                 const size_t lo = box_bytes + align16(2 * (G + 1));
                 if(lo > budget) continue;
-                float inv[3];
-                for(int d = 0; d < 3; ++d) inv[d] = (float)D[d] / ext[d];
-                size_t items = 0;
-                for(uint32_t i = 0; i < n; ++i)
-                {
-                    size_t span = 1;
-                    for(int d = 0; d < 3; ++d)
-                        span *= (size_t)(dev::cell_of(aabb6[6 * (size_t)i + 3 + d], mn[d], inv[d], D[d]) -
-                                         dev::cell_of(aabb6[6 * (size_t)i + d], mn[d], inv[d], D[d]) + 1);
-                    items += span;
-                }
+                const uint32_t* sa = &span[0][ja * n];
+                const uint32_t* sb = &span[1][jb * n];
+                const uint32_t* sc = &span[2][jc * n];
+                size_t items       = 0;
+                for(uint32_t i = 0; i < n; ++i) items += (size_t)sa[i] * sb[i] * sc[i];
                 const size_t bytes = lo + align16(2 * items);
                 if(items > 65535u || bytes > budget) continue;
                 // mean obstacles per cell (tests per query), then the smaller image
@@ -386,42 +393,71 @@ void free_env(Env& e)
     if(e.start) (void)hipFree(e.start);
     if(e.items) (void)hipFree(e.items);
     if(e.box) (void)hipFree(e.box);
-    if(e.lo) (void)hipFree(e.lo);
-    if(e.hi) (void)hipFree(e.hi);
+    if(e.lo) (void)hipFree(e.lo);  // e.hi points into the same buffer
     if(e.occ) (void)hipFree(e.occ);
     if(e.lds_img) (void)hipFree(e.lds_img);
     e = Env{};
 }
 
+// A device buffer of at least n items, reallocated only when it must grow.
+template<class T>
+int ensure(kpax_handle* h, T** p, size_t* cap, size_t n)
+{
+    n = std::max<size_t>(n, 1);
+    if(*p && *cap >= n) return KPAX_OK;
+    if(*p) (void)hipFree(*p);
+    *p   = nullptr;
+    *cap = 0;
+    HIPC(hipMalloc((void**)p, n * sizeof(T)));
+    *cap = n;
+    return KPAX_OK;
+}
+
 // Replaces environment `env` by n boxes and builds its CSR grid; the planner
 // reads the grid through envs_dev[env].  Runs on the handle's stream after
-// everything queued before it.
+// everything queued before it.  The environment's buffers and the build's
+// scratch are kept from one build to the next and grown when needed; the
+// item count is computed on the host with the same cell map, so the build is
+// queued without waiting for the device in between.
 int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n)
 {
     HIPC(hipStreamSynchronize(h->stream));
     Env& e = h->envs[env];
-    free_env(e);
     e.host.assign(aabb6, aabb6 + 6 * (size_t)n);
     e.n_obs = n;
-    std::vector<float4> lo(std::max<uint32_t>(n, 1)), hi(std::max<uint32_t>(n, 1));
+    std::vector<float4> lohi(2 * (size_t)std::max<uint32_t>(n, 1));
+    size_t K = 0;
     for(uint32_t i = 0; i < n; ++i)
     {
         const float* o = aabb6 + 6 * (size_t)i;
-        lo[i]          = make_float4(o[0], o[1], o[2], 0.0f);
-        hi[i]          = make_float4(o[3], o[4], o[5], 0.0f);
+        lohi[i]        = make_float4(o[0], o[1], o[2], 0.0f);
+        lohi[n + i]    = make_float4(o[3], o[4], o[5], 0.0f);
+        size_t span    = 1;  // cells of the obstacle, as k_grid_count counts them
+        for(int d = 0; d < 3; ++d)
+            span *= (size_t)(cell_of(o[3 + d], h->GPm.mn[d], h->GPm.inv[d], h->GPm.D[d]) -
+                             cell_of(o[d], h->GPm.mn[d], h->GPm.inv[d], h->GPm.D[d]) + 1);
+        K += span;
     }
+    if(K > (size_t)INT32_MAX) return fail(h, KPAX_INVALID_ARG, "the obstacles fill more than 2^31 grid cells");
     const uint32_t G = h->G;
-    HIPC(hipMalloc(&e.lo, lo.size() * sizeof(float4)));
-    HIPC(hipMalloc(&e.hi, hi.size() * sizeof(float4)));
-    HIPC(hipMalloc(&e.start, ((size_t)G + 1) * sizeof(int32_t)));
-    HIPC(hipMemcpy(e.lo, lo.data(), lo.size() * sizeof(float4), hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(e.hi, hi.data(), hi.size() * sizeof(float4), hipMemcpyHostToDevice));
-    Scratch sc;
-    int32_t* counts  = sc.get<int32_t>(G);
-    int32_t* cursor  = sc.get<int32_t>(G);
-    uint64_t* status = sc.get<uint64_t>(G / kTile + 2);
-    uint32_t* ticket = sc.get<uint32_t>(1);
-    if(!counts || !cursor || !status || !ticket) return fail(h, KPAX_NO_MEMORY, "grid scratch allocation failed");
+    int rc;
+    if((rc = ensure(h, &e.lo, &e.lo_cap, 2 * (size_t)std::max<uint32_t>(n, 1))) != KPAX_OK) return rc;
+    e.hi = e.lo + std::max<uint32_t>(n, 1);
+    if((rc = ensure(h, &e.start, &e.start_cap, (size_t)G + 1)) != KPAX_OK) return rc;
+    if((rc = ensure(h, &e.items, &e.items_cap, K)) != KPAX_OK) return rc;
+    if((rc = ensure(h, &e.box, &e.box_cap, 2 * K)) != KPAX_OK) return rc;
+    HIPC(hipMemcpyAsync(e.lo, lohi.data(), lohi.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    size_t cap = 0;
+    if(!h->grid_scratch)
+    {
+        // counts, cursor (G each), look-back status words, ticket
+        const size_t words = 2 * (size_t)G + 2 * ((size_t)G / kTile + 2) + 2;
+        if((rc = ensure(h, &h->grid_scratch, &cap, words)) != KPAX_OK) return rc;
+    }
+    int32_t* counts  = h->grid_scratch;
+    int32_t* cursor  = counts + G;
+    uint64_t* status = (uint64_t*)(cursor + G);  // 8 G bytes from the base: aligned
+    uint32_t* ticket = (uint32_t*)(status + G / kTile + 2);
     HIPC(hipMemsetAsync(counts, 0, (size_t)G * 4, h->stream));
     HIPC(hipMemsetAsync(cursor, 0, (size_t)G * 4, h->stream));
     HIPC(hipMemsetAsync(status, 0, (G / kTile + 2) * 8, h->stream));
@@ -431,25 +467,16 @@ int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n)
                            counts);
     hipLaunchKernelGGL(k_scan_offsets, dim3(std::min<uint32_t>(blocks_for(G, kTile), 4 * h->cu_count)), dim3(256), 0,
                        h->stream, counts, G, e.start, status, ticket);
-    HIPC(hipGetLastError());
-    int32_t K = 0;
-    HIPC(hipMemcpyAsync(&K, e.start + G, 4, hipMemcpyDeviceToHost, h->stream));
-    HIPC(hipStreamSynchronize(h->stream));
     e.k = (uint32_t)K;
-    HIPC(hipMalloc(&e.items, std::max<size_t>(K, 1) * sizeof(int32_t)));
     if(n)
         hipLaunchKernelGGL(k_grid_fill, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->GPm, e.lo, e.hi, n,
                            e.start, cursor, e.items);
     hipLaunchKernelGGL(k_grid_sort, dim3(blocks_for(G, 256)), dim3(256), 0, h->stream, e.start, e.items, G);
-    HIPC(hipMalloc(&e.box, 2 * std::max<size_t>(K, 1) * sizeof(float4)));
     if(K)
         hipLaunchKernelGGL(k_grid_boxes, dim3(blocks_for(K, 256)), dim3(256), 0, h->stream, e.items, e.lo, e.hi,
                            (uint32_t)K, e.box);
     HIPC(hipGetLastError());
-    {
-        const int rc = build_occupancy(h, env);
-        if(rc != KPAX_OK) return rc;
-    }
+    if((rc = build_occupancy(h, env)) != KPAX_OK) return rc;
     {
         // an image only where its cells list about as few obstacles as the
         // CSR grid's: a coarser grid tests more boxes per query
@@ -457,12 +484,13 @@ int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n)
         const std::vector<uint4> img = build_lds_image(h->cfg, aabb6, n, h->lds_budget, 1.5 * fine_mean + 0.05, &e.lg);
         if(!img.empty())
         {
-            HIPC(hipMalloc(&e.lds_img, img.size() * sizeof(uint4)));
-            HIPC(hipMemcpy(e.lds_img, img.data(), img.size() * sizeof(uint4), hipMemcpyHostToDevice));
+            if((rc = ensure(h, &e.lds_img, &e.lds_cap, img.size())) != KPAX_OK) return rc;
+            HIPC(hipMemcpyAsync(e.lds_img, img.data(), img.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                                h->stream));
             e.lg.img = e.lds_img;
         }
-        const int rc = refresh_lds_launch(h);
-        if(rc != KPAX_OK) return rc;
+        e.has_lds = !img.empty();
+        if((rc = refresh_lds_launch(h)) != KPAX_OK) return rc;
     }
     const GridView g = grid_view(h, env);
     HIPC(hipMemcpyAsync(h->envs_dev + env, &g, sizeof(g), hipMemcpyHostToDevice, h->stream));
@@ -496,8 +524,8 @@ int kpax_get_lds_grid(kpax_handle* h, uint32_t env, int32_t* dims, uint32_t* byt
     if(!h) return KPAX_INVALID_ARG;
     if(env >= (uint32_t)kMaxEnvs) return fail(h, KPAX_INVALID_ARG, "environment index out of range");
     const Env& e = h->envs[env];
-    for(int d = 0; dims && d < 3; ++d) dims[d] = e.lds_img ? e.lg.D[d] : 0;
-    if(bytes) *bytes = e.lds_img ? e.lg.bytes : 0;
+    for(int d = 0; dims && d < 3; ++d) dims[d] = e.has_lds ? e.lg.D[d] : 0;
+    if(bytes) *bytes = e.has_lds ? e.lg.bytes : 0;
     return KPAX_OK;
 }
 

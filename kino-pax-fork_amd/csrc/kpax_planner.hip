@@ -1691,7 +1691,7 @@ int kpax::refresh_lds_launch(kpax_handle* h)
     uint32_t bytes = 0;
     if(h->Qmax == 1)
         for(const Env& e : h->envs)
-            if(e.lds_img) bytes = std::max(bytes, e.lg.bytes);
+            if(e.has_lds) bytes = std::max(bytes, e.lg.bytes);
     h->lds_bytes = bytes;
     if(!h->launch_set)
     {
@@ -1924,6 +1924,7 @@ int kpax_destroy(kpax_handle* h)
     if(h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
     for(void* p : h->allocs) (void)hipFree(p);
+    if(h->grid_scratch) (void)hipFree(h->grid_scratch);
     for(Env& e : h->envs) free_env(e);
     void* pinned[] = {h->h_status,     h->h_ctl,       h->h_qd,         h->h_init_slots, h->h_init_starts,
                       h->h_path_slots, h->h_path_rows, h->h_path_steps, h->h_path_idx};

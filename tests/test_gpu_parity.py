@@ -369,3 +369,23 @@ def test_failed_run_leaves_the_handle_usable(kpax, tmp_path):
     assert (r.status, r.iterations, r.tree_size, r.expansions) == (fresh.status, fresh.iterations,
                                                                    fresh.tree_size, fresh.expansions)
     assert np.array_equal(r.rows.view(np.uint32), fresh.rows.view(np.uint32))
+
+
+def test_environment_rebuilds_reuse_buffers(kpax, oracle):
+    """Replacing a handle's obstacles (more, then fewer, then none, then the
+    first set again) rebuilds the CSR grid and the LDS image in the kept
+    buffers: each build equals the oracle's counts, and a plan after it equals
+    the same plan on a fresh handle."""
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=1 << 16, max_iter=30)
+    big = kpax.generate_environment(3, 7, 2000, cfg.ws_min, cfg.ws_max, start[:3], list(goal[:3]) + [cfg.goal_radius])
+    with kpax.KPAX(cfg) as p:
+        for o in (obs, big, np.zeros((0, 6), np.float32), obs[:50], obs):
+            p.set_obstacles(o)
+            st, items = p.grid()
+            assert np.array_equal(np.diff(st), oracle.grid_counts(cfg, o)) and st[-1] == items.size
+            r = p.plan(start, goal, seed=12)
+            with kpax.KPAX(cfg) as q:
+                f = q.plan(start, goal, o, seed=12)
+            assert (r.status, r.tree_size, r.expansions) == (f.status, f.tree_size, f.expansions), o.shape
+            print(f"{o.shape[0]} obstacles: LDS grid {p.lds_grid()}, status {r.status}, tree {r.tree_size}",
+                  flush=True)
