@@ -49,10 +49,14 @@ __device__ __forceinline__ void wave_lds_sync()
 // Occupancy bitmap of an environment: cell (a, b, c) of the 16^3 lattice over
 // the workspace is set when an obstacle overlaps its open box.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kOccChunk = 32;  // obstacles per work-group row of the grid
+
+// Grid (cells / 256, obstacle chunks): a lane tests its cell against chunks
+// blockIdx.y, blockIdx.y + gridDim.y, ... of the obstacles and ORs the wave's
+// verdicts into the zeroed bitmap.
 __global__ __launch_bounds__(256) void k_occupancy(FeatParams F, const float4* lo, const float4* hi, uint32_t n,
                                                    uint32_t* bits)
 {
-    // one cell per lane; a wave's 64 verdicts are two bitmap words
     constexpr int kCells = kOccDim * kOccDim * kOccDim;
     const int cidx       = blockIdx.x * blockDim.x + threadIdx.x;
     bool hit             = false;
@@ -64,15 +68,17 @@ __global__ __launch_bounds__(256) void k_occupancy(FeatParams F, const float4* l
         const float ch[3] = {F.occ_mn[0] + (float)(a + 1) * F.occ_cell[0],
                              F.occ_mn[1] + (float)(b + 1) * F.occ_cell[1],
                              F.occ_mn[2] + (float)(c + 1) * F.occ_cell[2]};
-        for(uint32_t o = 0; o < n && !hit; ++o)
-        {
-            const float4 p = lo[o], q = hi[o];
-            hit = p.x < ch[0] && q.x > cl[0] && p.y < ch[1] && q.y > cl[1] && p.z < ch[2] && q.z > cl[2];
-        }
+        for(uint32_t o0 = blockIdx.y * kOccChunk; o0 < n && !hit; o0 += gridDim.y * kOccChunk)
+            for(uint32_t o = o0, oe = n - o0 < kOccChunk ? n : o0 + kOccChunk; o < oe && !hit; ++o)
+            {
+                const float4 p = lo[o], q = hi[o];
+                hit = p.x < ch[0] && q.x > cl[0] && p.y < ch[1] && q.y > cl[1] && p.z < ch[2] && q.z > cl[2];
+            }
     }
     const uint64_t m = __ballot(hit);
     const int lane   = threadIdx.x & 63;
-    if(cidx < kCells && (lane == 0 || lane == 32)) bits[cidx / 32] = (uint32_t)(lane == 0 ? m : m >> 32);
+    const uint32_t w = (uint32_t)(lane == 0 ? m : m >> 32);
+    if(cidx < kCells && (lane == 0 || lane == 32) && w) atomicOr(&bits[cidx / 32], w);
 }
 
 // fp32 parameters (flat layout of kpax_mlp.cuh) -> bf16 device layouts.
@@ -700,8 +706,13 @@ int build_occupancy(kpax_handle* h, uint32_t env)
     Env& e = h->envs[env];
     if(!e.occ) HIPC(hipMalloc(&e.occ, kOccDim * kOccDim * kOccDim / 8));
     const FeatParams F = make_feat_params(h->cfg);
-    hipLaunchKernelGGL(k_occupancy, dim3(kOccDim * kOccDim * kOccDim / 256), dim3(256), 0, h->stream, F, e.lo, e.hi,
-                       e.n_obs, e.occ);
+    HIPC(hipMemsetAsync(e.occ, 0, kOccDim * kOccDim * kOccDim / 8, h->stream));
+    if(e.n_obs)
+    {
+        const uint32_t chunks = std::min<uint32_t>((e.n_obs + kOccChunk - 1) / kOccChunk, 1024);
+        hipLaunchKernelGGL(k_occupancy, dim3(kOccDim * kOccDim * kOccDim / 256, chunks), dim3(256), 0, h->stream, F,
+                           e.lo, e.hi, e.n_obs, e.occ);
+    }
     HIPC(hipGetLastError());
     return KPAX_OK;
 }

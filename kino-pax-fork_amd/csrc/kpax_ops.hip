@@ -171,31 +171,44 @@ __device__ __forceinline__ void obstacle_cells(const GridParams& gp, float4 a, f
     }
 }
 
-__global__ void k_grid_count(GridParams gp, const float4* lo, const float4* hi, uint32_t n, int32_t* counts)
+// One wave per obstacle: its lanes walk the obstacle's cells (a column of the
+// forest spans a few hundred), so no lane runs a long serial loop of atomics.
+__device__ __forceinline__ uint32_t obstacle_span(const int a[3], const int b[3], uint32_t* nx, uint32_t* nxy)
 {
-    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-    if(o >= n) return;
-    int a[3], b[3];
-    obstacle_cells(gp, lo[o], hi[o], a, b);
-    for(int z = a[2]; z <= b[2]; ++z)
-        for(int y = a[1]; y <= b[1]; ++y)
-            for(int x = a[0]; x <= b[0]; ++x) atomicAdd(&counts[x + gp.D[0] * (y + gp.D[1] * z)], 1);
+    *nx  = (uint32_t)(b[0] - a[0] + 1);
+    *nxy = *nx * (uint32_t)(b[1] - a[1] + 1);
+    return *nxy * (uint32_t)(b[2] - a[2] + 1);
 }
 
-__global__ void k_grid_fill(GridParams gp, const float4* lo, const float4* hi, uint32_t n, const int32_t* start,
-                            int32_t* cursor, int32_t* items)
+__device__ __forceinline__ int span_cell(const GridParams& gp, const int a[3], uint32_t k, uint32_t nx, uint32_t nxy)
 {
-    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-    if(o >= n) return;
+    const int x = a[0] + (int)(k % nx), y = a[1] + (int)((k % nxy) / nx), z = a[2] + (int)(k / nxy);
+    return x + gp.D[0] * (y + gp.D[1] * z);
+}
+
+__global__ __launch_bounds__(64) void k_grid_count(GridParams gp, const float4* lo, const float4* hi, int32_t* counts)
+{
+    const uint32_t o = blockIdx.x;
     int a[3], b[3];
     obstacle_cells(gp, lo[o], hi[o], a, b);
-    for(int z = a[2]; z <= b[2]; ++z)
-        for(int y = a[1]; y <= b[1]; ++y)
-            for(int x = a[0]; x <= b[0]; ++x)
-            {
-                const int c = x + gp.D[0] * (y + gp.D[1] * z);
-                items[start[c] + atomicAdd(&cursor[c], 1)] = (int32_t)o;
-            }
+    uint32_t nx, nxy;
+    const uint32_t span = obstacle_span(a, b, &nx, &nxy);
+    for(uint32_t k = threadIdx.x; k < span; k += 64) atomicAdd(&counts[span_cell(gp, a, k, nx, nxy)], 1);
+}
+
+__global__ __launch_bounds__(64) void k_grid_fill(GridParams gp, const float4* lo, const float4* hi,
+                                                  const int32_t* start, int32_t* cursor, int32_t* items)
+{
+    const uint32_t o = blockIdx.x;
+    int a[3], b[3];
+    obstacle_cells(gp, lo[o], hi[o], a, b);
+    uint32_t nx, nxy;
+    const uint32_t span = obstacle_span(a, b, &nx, &nxy);
+    for(uint32_t k = threadIdx.x; k < span; k += 64)
+    {
+        const int c = span_cell(gp, a, k, nx, nxy);
+        items[start[c] + atomicAdd(&cursor[c], 1)] = (int32_t)o;
+    }
 }
 
 __global__ void k_grid_sort(const int32_t* start, int32_t* items, uint32_t G)
@@ -463,35 +476,31 @@ int build_grid(kpax_handle* h, uint32_t env, const float* aabb6, uint32_t n)
     HIPC(hipMemsetAsync(status, 0, (G / kTile + 2) * 8, h->stream));
     HIPC(hipMemsetAsync(ticket, 0, 4, h->stream));
     if(n)
-        hipLaunchKernelGGL(k_grid_count, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->GPm, e.lo, e.hi, n,
-                           counts);
+        hipLaunchKernelGGL(k_grid_count, dim3(n), dim3(64), 0, h->stream, h->GPm, e.lo, e.hi, counts);
     hipLaunchKernelGGL(k_scan_offsets, dim3(std::min<uint32_t>(blocks_for(G, kTile), 4 * h->cu_count)), dim3(256), 0,
                        h->stream, counts, G, e.start, status, ticket);
     e.k = (uint32_t)K;
     if(n)
-        hipLaunchKernelGGL(k_grid_fill, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->GPm, e.lo, e.hi, n,
-                           e.start, cursor, e.items);
+        hipLaunchKernelGGL(k_grid_fill, dim3(n), dim3(64), 0, h->stream, h->GPm, e.lo, e.hi, e.start, cursor,
+                           e.items);
     hipLaunchKernelGGL(k_grid_sort, dim3(blocks_for(G, 256)), dim3(256), 0, h->stream, e.start, e.items, G);
     if(K)
         hipLaunchKernelGGL(k_grid_boxes, dim3(blocks_for(K, 256)), dim3(256), 0, h->stream, e.items, e.lo, e.hi,
                            (uint32_t)K, e.box);
     HIPC(hipGetLastError());
     if((rc = build_occupancy(h, env)) != KPAX_OK) return rc;
+    // an image only where its cells list about as few obstacles as the CSR
+    // grid's: a coarser grid tests more boxes per query
+    const double fine_mean       = (double)K / (double)std::max<uint32_t>(G, 1);
+    const std::vector<uint4> img = build_lds_image(h->cfg, aabb6, n, h->lds_budget, 1.5 * fine_mean + 0.05, &e.lg);
+    if(!img.empty())
     {
-        // an image only where its cells list about as few obstacles as the
-        // CSR grid's: a coarser grid tests more boxes per query
-        const double fine_mean       = (double)K / (double)std::max<uint32_t>(G, 1);
-        const std::vector<uint4> img = build_lds_image(h->cfg, aabb6, n, h->lds_budget, 1.5 * fine_mean + 0.05, &e.lg);
-        if(!img.empty())
-        {
-            if((rc = ensure(h, &e.lds_img, &e.lds_cap, img.size())) != KPAX_OK) return rc;
-            HIPC(hipMemcpyAsync(e.lds_img, img.data(), img.size() * sizeof(uint4), hipMemcpyHostToDevice,
-                                h->stream));
-            e.lg.img = e.lds_img;
-        }
-        e.has_lds = !img.empty();
-        if((rc = refresh_lds_launch(h)) != KPAX_OK) return rc;
+        if((rc = ensure(h, &e.lds_img, &e.lds_cap, img.size())) != KPAX_OK) return rc;
+        HIPC(hipMemcpyAsync(e.lds_img, img.data(), img.size() * sizeof(uint4), hipMemcpyHostToDevice, h->stream));
+        e.lg.img = e.lds_img;
     }
+    e.has_lds = !img.empty();
+    if((rc = refresh_lds_launch(h)) != KPAX_OK) return rc;
     const GridView g = grid_view(h, env);
     HIPC(hipMemcpyAsync(h->envs_dev + env, &g, sizeof(g), hipMemcpyHostToDevice, h->stream));
     HIPC(hipStreamSynchronize(h->stream));

@@ -136,6 +136,7 @@ inline uint64_t __ballot(int pred)
 inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 
 template<class T> inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+template<class T> inline T atomicOr(T* p, T v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 template<class T> inline T atomicMin(T* p, T v)
 {
     T cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
@@ -364,9 +365,10 @@ void emu_launch(void (*k)(KArgs...), dim3 grid, dim3 block, size_t shmem, Args..
 {
     std::vector<char> dyn(shmem ? shmem : 1);
     emu_dyn_shared = dyn.data();
-    gridDim  = {grid.x, 1, 1};
+    gridDim  = {grid.x, grid.y, 1};
     blockDim = {block.x, 1, 1};
     const unsigned nw = (block.x + 63) / 64;
+    for(unsigned by = 0; by < grid.y; ++by)
     for(unsigned b = 0; b < grid.x; ++b)
     {
         emu_group g;
@@ -378,7 +380,7 @@ void emu_launch(void (*k)(KArgs...), dim3 grid, dim3 block, size_t shmem, Args..
         emu_g = &g;
         emu_workers().run(block.x, [&, b](unsigned t) {
             threadIdx = {t, 0, 0};
-            blockIdx  = {b, 0, 0};
+            blockIdx  = {b, by, 0};
             k(args...);
             // a work-item that returns early no longer takes part in barriers
             g.wave_bar[t / 64]->arrive_and_drop();

@@ -34,13 +34,19 @@ def _states(cfg, n, rng):
 
 def test_features_bit_exact(kpax, oracle, rng):
     p, cfg, obs, _, goal = _planner(kpax, max_tree=4096)
+    # the forest, then on the same handle 1,000 small boxes (the occupancy
+    # bits are ORed over 32-obstacle chunks: 31 full chunks and a partial one),
+    # a single box and none
+    small = rng.uniform(0, 9.5, (1000, 3)).astype(np.float32)
+    small = np.concatenate([small, small + rng.uniform(0.05, 0.5, (1000, 3)).astype(np.float32)], 1)
     with p:
-        x0 = _states(cfg, 5000, rng)
-        g = rng.uniform(0, 10, (3, 5000)).astype(np.float32)
-        f = p.mlp_features(0, x0, g)
-    want = oracle.mlp_features(cfg, obs, x0, g)
-    assert np.array_equal(f.view(np.uint32), want.view(np.uint32))
-    assert f[:, 15:79].mean() > 0.01  # the forest is seen
+        for o in (obs, small, small[:1], np.zeros((0, 6), np.float32)):
+            p.set_obstacles(o)
+            x0 = _states(cfg, 5000, rng)
+            g = rng.uniform(0, 10, (3, 5000)).astype(np.float32)
+            f = p.mlp_features(0, x0, g)
+            want = oracle.mlp_features(cfg, o, x0, g)
+            assert np.array_equal(f.view(np.uint32), want.view(np.uint32)), o.shape
 
 
 def test_sampler_bit_exact(kpax, oracle, rng):
