@@ -268,6 +268,9 @@ typedef struct kpax_launch {
                                   fits (kpax_get_lds_grid); 0: they query the grid in global memory    */
     uint32_t expand_threads;   /* work-group size of the single-query expansion: a multiple of 64, at most
                                   640 (the batched expansion always runs 256)                          */
+    int32_t  wave_priority;    /* 1: an expansion wave raises its issue priority with the steps its
+                                  longest live edge has left (s_setprio), so the waves on a launch's
+                                  critical path issue first; 0: every wave at the same priority   */
 } kpax_launch;
 
 /* Current launch shape of the handle. */

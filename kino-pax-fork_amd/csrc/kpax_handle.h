@@ -131,6 +131,7 @@ struct kpax_handle {
     bool order_children     = true;     // hand out children longest drawn edge first (default: single-query handles)
     uint32_t* tickets       = nullptr;  // kTicketGroups expansion chunk counters
     uint32_t refill_lanes   = 16;       // idle lanes that trigger a wave's service pass
+    bool wave_priority      = false;    // expansion waves set their issue priority by their longest live edge
     uint32_t scan_tile      = 4096;     // items per append/frontier tile: 1024, 2048 or 4096 (4 per thread)
     uint4* prep             = nullptr;  // per processing position: the child's prepared draws (2 x uint4)
     uint32_t* order_hist    = nullptr;  // buckets x order blocks, scanned in place

@@ -151,6 +151,7 @@ struct ExpandArgs {
     uint32_t* tickets;      // kTicketGroups chunk counters, kTicketStride words apart
     uint32_t refill;        // idle lanes that trigger a wave's service pass (1..64)
     uint32_t use_lds;       // single-slot form: stage the environment's grid image in LDS
+    int32_t prio_q;         // issue-priority step quantum (a quarter of the longest edge); 0: priorities off
     const float* prop;
     SlotUniform s0;
 };
@@ -282,6 +283,23 @@ __global__ __launch_bounds__(1024) void k_advance(AdvanceArgs V) { advance_block
 // environment's grid image in LDS (work-groups of 640 threads: two per CU
 // hold the five waves per SIMD and two images); the batched form's lanes
 // query the grids of several environments in global memory (256 threads).
+// Issue priority of a wave from the steps its longest live edge has left, in
+// quarters q of the longest drawn edge (0..3).  A launch lasts as long as its
+// longest chain of steps; the waves holding the long edges issue first and
+// the waves with short edges fill their stalls, instead of every wave
+// sharing the SIMD evenly until the long edges are all that is left.
+__device__ __forceinline__ void set_issue_priority(int left, int q)
+{
+    if(__ballot(left > 3 * q))
+        __builtin_amdgcn_s_setprio(3);
+    else if(__ballot(left > 2 * q))
+        __builtin_amdgcn_s_setprio(2);
+    else if(__ballot(left > q))
+        __builtin_amdgcn_s_setprio(1);
+    else
+        __builtin_amdgcn_s_setprio(0);
+}
+
 template<int MODEL, bool MULTI>
 __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 : 5))) void k_expand(
   const ExpandArgs* __restrict__ A)
@@ -343,6 +361,7 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
     bool drained = false;
     bool live = false, pend = false, valid = false;  // integrating / finished, not yet written / its verdict
     const uint32_t refill = A->refill;
+    const int prio_q      = A->prio_q;
     for(;;)
     {
         // Service pass, once `refill` lanes are idle (or all are): write out the
@@ -444,6 +463,7 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
             }
             if(__ballot(live) == 0) break;  // drained, and every child written out
         }
+        if(prio_q) set_issue_priority(live ? n - i : 0, prio_q);
         if(live)
         {
             valid = step<MODEL>(P, G, x, u);
@@ -1098,6 +1118,7 @@ int upload_expand_args(kpax_handle* h)
     a.tickets   = h->tickets;
     a.refill    = h->refill_lanes;
     a.use_lds   = h->Qmax == 1 && h->use_lds && h->lds_bytes > 0 ? 1u : 0u;
+    a.prio_q    = h->wave_priority ? std::max(1, (h->P.steps_min + h->P.span - 1 + 3) / 4) : 0;
     a.prop      = h->prop;
     memset(&a.s0, 0, sizeof(a.s0));
     a.s0.cur.status = kNotStarted;
@@ -1709,7 +1730,7 @@ int apply_launch(kpax_handle* h, const kpax_launch& l)
        l.expand_chunk % 64 != 0 || l.scan_blocks < 1 || l.scan_blocks > (1u << 20) || l.refill_lanes < 1 ||
        l.refill_lanes > 64 || (l.scan_tile != 1024 && l.scan_tile != 2048 && l.scan_tile != 4096) ||
        (l.lds_grid != 0 && l.lds_grid != 1) || l.expand_threads < 64 || l.expand_threads > 640 ||
-       l.expand_threads % 64 != 0)
+       l.expand_threads % 64 != 0 || (l.wave_priority != 0 && l.wave_priority != 1))
         return fail(h, KPAX_INVALID_ARG, "launch shape out of range");
     HIPC(hipStreamSynchronize(h->stream));
     h->launch_set    = true;
@@ -1722,6 +1743,7 @@ int apply_launch(kpax_handle* h, const kpax_launch& l)
     h->order_children = l.order_children != 0;
     h->refill_lanes   = l.refill_lanes;
     h->scan_tile      = l.scan_tile;
+    h->wave_priority  = l.wave_priority != 0;
     drop_graphs(h);
     const int rc = upload_expand_args(h);
     if(rc == KPAX_OK && !l.use_graph) h->graph_tried = true;  // stay on direct launches
@@ -1740,6 +1762,7 @@ kpax_launch current_launch(const kpax_handle* h)
     l.scan_tile      = h->scan_tile;
     l.lds_grid       = h->use_lds ? 1 : 0;
     l.expand_threads = h->expand_threads;
+    l.wave_priority  = h->wave_priority ? 1 : 0;
     return l;
 }
 
@@ -1790,6 +1813,11 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     // order keeps a wave's lanes on one query: the serving batch ran at 2,750
     // plans/s without it and 2,270 with it.
     h->order_children = max_queries == 1;
+    // Issue priority by the longest live edge pays where a step is long
+    // (quadcopter forest: expansion 2.77 -> 2.64 ms per plan); the double
+    // integrator's steps are short and its iterations small, and the ballots
+    // per trip cost more than the ordering gains (0.547 -> 0.560 ms).
+    h->wave_priority = cfg->model != KPAX_MODEL_DOUBLE_INTEGRATOR;
     h->G     = (uint32_t)h->GPm.D[0] * h->GPm.D[1] * h->GPm.D[2];
     h->R2    = (uint32_t)h->RP.r1_count * h->RP.r2_per;
     h->envs.resize(kMaxEnvs);
@@ -2258,7 +2286,7 @@ int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_
     cur.expand_blocks = occ * cu;
     double cur_ms     = 0.0;
     int rc            = run(cur, &cur_ms, ref);
-    std::vector<std::vector<kpax_launch>> knobs(8);
+    std::vector<std::vector<kpax_launch>> knobs(9);
     for(uint32_t k = 1; k <= occ; ++k) knobs[0].push_back(kpax_launch{k * cu, 0, 0, 0, 0, 0, 0});
     for(uint32_t c : {64u, 128u, 256u, 512u}) knobs[1].push_back(kpax_launch{0, c, 0, 0, 0, 0, 0});
     for(uint32_t t : {4096u, 2048u, 1024u}) knobs[2].push_back(kpax_launch{0, 0, 0, 0, 0, 0, t});
@@ -2267,7 +2295,8 @@ int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_
     for(int o : {1, 0}) knobs[5].push_back(kpax_launch{0, 0, 0, 0, o, 0, 0});
     for(uint32_t r : {1u, 8u, 16u, 24u, 32u}) knobs[6].push_back(kpax_launch{0, 0, 0, 0, 0, r, 0});
     for(int g : {1, 0}) knobs[7].push_back(kpax_launch{0, 0, 0, 0, 0, 0, 0, g});
-    for(int k = 0; rc == KPAX_OK && k < 8; ++k)
+    for(int w : {1, 0}) knobs[8].push_back(kpax_launch{0, 0, 0, 0, 0, 0, 0, 0, 0, w});
+    for(int k = 0; rc == KPAX_OK && k < 9; ++k)
         for(const kpax_launch& v : knobs[k])
         {
             kpax_launch cand = cur;
@@ -2279,6 +2308,7 @@ int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_
             if(k == 5) cand.order_children = v.order_children;
             if(k == 6) cand.refill_lanes = v.refill_lanes;
             if(k == 7) cand.lds_grid = v.lds_grid;
+            if(k == 8) cand.wave_priority = v.wave_priority;
             if(memcmp(&cand, &cur, sizeof(cand)) == 0) continue;
             double ms = 0.0;
             if((rc = run(cand, &ms, got)) != KPAX_OK) break;

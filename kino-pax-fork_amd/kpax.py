@@ -91,7 +91,7 @@ class _Launch(ctypes.Structure):
     _fields_ = [("expand_blocks", ctypes.c_uint32), ("expand_chunk", ctypes.c_uint32),
                 ("scan_blocks", ctypes.c_uint32), ("use_graph", ctypes.c_int32), ("order_children", ctypes.c_int32),
                 ("refill_lanes", ctypes.c_uint32), ("scan_tile", ctypes.c_uint32), ("lds_grid", ctypes.c_int32),
-                ("expand_threads", ctypes.c_uint32)]
+                ("expand_threads", ctypes.c_uint32), ("wave_priority", ctypes.c_int32)]
 
 
 class _Query(ctypes.Structure):
@@ -526,7 +526,7 @@ class KPAX:
 
     def launch(self):
         """Launch shape: expand_blocks, expand_chunk, scan_blocks, use_graph, order_children, refill_lanes, scan_tile,
-        lds_grid, expand_threads."""
+        lds_grid, expand_threads, wave_priority."""
         l = _Launch()
         self._check(lib().kpax_get_launch(self._h, ctypes.byref(l)), "get_launch")
         return {k: getattr(l, k) for k, _ in _Launch._fields_}

@@ -119,6 +119,9 @@ template<class T> inline T __shfl(T x, int src, int = 64)
 
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 
+// a wave's issue priority only orders instruction issue on the GPU
+inline void __builtin_amdgcn_s_setprio(int) {}
+
 inline uint64_t __ballot(int pred)
 {
     const unsigned t = threadIdx.x, w = t / 64, lane = t % 64;

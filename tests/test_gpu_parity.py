@@ -141,6 +141,24 @@ def test_plan_identical_with_and_without_lds_grid(kpax):
 
 
 @pytest.mark.parametrize("k", MODELS)
+def test_plan_identical_with_and_without_wave_priority(kpax, k):
+    """Issue priorities reorder the expansion's instruction issue only: the
+    tree is the same bit for bit with them on and off, for every model."""
+    cfg, obs, start, goal = workload(k, seed=0, max_tree=1 << 17, max_iter=12)
+    out = {}
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        for w in (1, 0):
+            p.set_launch(wave_priority=w)
+            assert p.launch()["wave_priority"] == w
+            r = p.plan(start, goal, seed=11)
+            out[w] = ((r.status, r.tree_size, r.iterations, r.expansions, r.integrated_steps), p.tree())
+    assert out[1][0] == out[0][0]
+    assert np.array_equal(out[1][1]["states"].view(np.uint32), out[0][1]["states"].view(np.uint32))
+    assert np.array_equal(out[1][1]["parents"], out[0][1]["parents"])
+
+
+@pytest.mark.parametrize("k", MODELS)
 def test_regions_match_oracle(kpax, oracle, rng, k):
     p, cfg, _, _, _ = _planner(kpax, k, max_tree=4096)
     lo = np.array(cfg.ws_min, np.float32)[:, None]
