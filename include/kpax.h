@@ -253,7 +253,7 @@ int kpax_reset_stats(kpax_handle* h);
 
 typedef struct kpax_launch {
     uint32_t expand_blocks;    /* work-groups of the expansion kernel (its waves are persistent) */
-    uint32_t expand_chunk;     /* children a wave takes per work ticket (multiple of 64, <= 4096) */
+    uint32_t expand_chunk;     /* children a wave takes per work ticket (multiple of 16, 16..4096) */
     uint32_t scan_blocks;      /* work-groups of the append and frontier scans                   */
     int32_t  use_graph;        /* 1: replay each iteration as one captured HIP graph (default 0:
                                   direct launches, measured faster)                             */

@@ -135,7 +135,7 @@ def test_autotune_keeps_results_and_applies_the_shape(kpax):
         before = p.plan_batch(qs)
         best = p.autotune(qs, repeats=1)
         assert p.launch() == best
-        assert best["expand_chunk"] % 64 == 0 and best["expand_blocks"] > 0 and best["scan_blocks"] > 0
+        assert best["expand_chunk"] % 16 == 0 and best["expand_blocks"] > 0 and best["scan_blocks"] > 0
         after = p.plan_batch(qs)
     for a, b in zip(before, after):
         assert _same(a, b)
@@ -146,23 +146,25 @@ def test_launch_knobs_leave_results_unchanged(kpax):
     `refill_lanes` of its lanes are idle, the append and frontier scans run
     over tiles of `scan_tile` items, and children are handed out in index
     order or longest edge first (`order_children`, off by default for a
-    batch handle); every child is a function of its index and the scans are
-    exact, so the knobs change only speed.  Checked on a single plan's tree
-    and on a batch."""
+    batch handle), in chunks of `expand_chunk` children per work ticket;
+    every child is a function of its index and the scans are exact, so the
+    knobs change only speed.  Checked on a single plan's tree and on a
+    batch."""
     cfg, obs, start, goal = workload(2, seed=0, max_iter=25)
     qs = [kpax.Query(start, goal, seed=70 + i, budget=50000 + 9000 * i) for i in range(6)]
-    shapes = [(1, 1024, 0), (24, 4096, 1), (64, 2048, 1), (16, 4096, 0)]
+    shapes = [(1, 1024, 0, 64), (24, 4096, 1, 16), (64, 2048, 1, 32), (16, 4096, 0, 48)]
     out = {}
-    for refill, tile, order in shapes:
+    for refill, tile, order, chunk in shapes:
         with kpax.KPAX(cfg, max_queries=3, node_capacity=300000) as p:
             p.set_obstacles(obs)
             assert p.launch()["order_children"] == 0
-            p.set_launch(refill_lanes=refill, scan_tile=tile, order_children=order)
+            p.set_launch(refill_lanes=refill, scan_tile=tile, order_children=order, expand_chunk=chunk)
             assert p.launch()["refill_lanes"] == refill and p.launch()["scan_tile"] == tile
+            assert p.launch()["expand_chunk"] == chunk
             res = p.plan_batch(qs)
             one = p.plan(start, goal, seed=5)
             tree = p.tree()
-        out[(refill, tile, order)] = (res, one, tree)
+        out[(refill, tile, order, chunk)] = (res, one, tree)
     ref_res, ref_one, ref_tree = out[shapes[0]]
     for shape in shapes[1:]:
         res, one, tree = out[shape]
