@@ -134,7 +134,6 @@ struct kpax_handle {
     bool wave_priority      = false;    // expansion waves set their issue priority by their longest live edge
     uint32_t scan_tile      = 4096;     // items per append/frontier tile: 1024, 2048 or 4096 (4 per thread)
     uint4* prep             = nullptr;  // per processing position: the child's prepared draws (2 x uint4)
-    uint32_t* order_hist    = nullptr;  // buckets x order blocks, scanned in place
     uint32_t scan_blocks    = 1;
     uint32_t path_cap       = 0;        // rows per slot
     float* path_rows        = nullptr;  // Qmax x path_cap x (S + C + 1), goal -> root

@@ -11,10 +11,11 @@
 // KPAX.cu:303-543):
 //   k_advance   1 block   publish each slot's iteration record; scan the
 //                         slots' child / tile counts into offsets
-//   k_order_count         bucket counts for the longest-drawn-edge-first order
-//                         (optional: on by default for a single-query handle)
 //   k_prepare   coalesced each child's parent, Philox control and step draws,
-//                         written at its processing position
+//                         written at its processing position (longest drawn
+//                         edge first within each work-group's segment: on by
+//                         default for a single-query handle, whose advance
+//                         runs in block 0 of this pass)
 //   k_expand    hot       one lane per child: RK4 with checks, region and
 //                         sub-region of the result, admission
 //                         (propagateFrontier_kernel1/2)
@@ -152,6 +153,8 @@ struct ExpandArgs {
     uint32_t refill;        // idle lanes that trigger a wave's service pass (1..64)
     uint32_t use_lds;       // single-slot form: stage the environment's grid image in LDS
     int32_t prio_q;         // issue-priority step quantum (a quarter of the longest edge); 0: priorities off
+    uint32_t ordered;       // children in longest-first order per preparation segment (else child order)
+    uint32_t order_blocks;  // work-groups of the preparation pass (its segments)
     const float* prop;
     SlotUniform s0;
 };
@@ -261,6 +264,77 @@ __device__ __forceinline__ void advance_block(const AdvanceArgs& V)
 __global__ __launch_bounds__(1024) void k_advance(AdvanceArgs V) { advance_block(V); }
 
 // ---------------------------------------------------------------------------
+// Processing order of the children, longest drawn edge first, so the last
+// children a launch starts are short ones and the launch does not wait on a
+// long edge begun at its end.  Results do not depend on the order (every
+// child is a function of its index and of the tree before the iteration).
+// The preparation pass's work-group b takes segment b of the child index
+// space, [b per, (b + 1) per) with per = ceil(children / work-groups), and
+// sorts it in place by drawn step count (a counting sort in LDS); the
+// expansion's chunk k is the (k / segments)-th chunk of segment
+// k % segments, so the launch starts every segment's longest children first,
+// then their next longest, ...: the global longest-first order to within a
+// chunk, with no pass that counts the buckets of all segments first.  In
+// child order there is one segment and the position is the child index.
+// ---------------------------------------------------------------------------
+// 256 work-groups of 1024 threads: four waves per SIMD hide the Philox chains
+// and the repeat-list loads (one wave per SIMD left both exposed).
+constexpr int kOrderBlocks = 256, kOrderThreads = 1024, kOrderBuckets = 32;
+
+__device__ __forceinline__ uint32_t order_bucket(const ModelParams& P, int n)
+{
+    const uint32_t back = (uint32_t)(P.steps_min + P.span - 1 - n);  // 0 for the longest edge
+    return P.span <= kOrderBuckets ? back : (uint32_t)((uint64_t)back * kOrderBuckets / (uint32_t)P.span);
+}
+
+// Children per segment: the children over the preparation pass's `nb`
+// work-groups for the longest-first order, but at least kMinSegment (a small
+// iteration's segments would otherwise hold a child or two each, and a wave
+// would take a ticket per child); one segment in child order.
+constexpr uint32_t kMinSegment = 1024;
+__device__ __forceinline__ uint32_t segment_len(uint32_t total, uint32_t nb, bool ordered)
+{
+    if(!ordered) return total;
+    const uint32_t per = (total + nb - 1) / nb;
+    return per > kMinSegment ? per : kMinSegment;
+}
+
+// Positions [*b, *e) of work chunk k (empty when k lands past the end of the
+// last, shorter segment).  per >= 1, nseg = ceil(total / per).  Row j holds
+// the j-th chunk of every segment, rotated by j: a ticket group takes every
+// 64th chunk index, and without the rotation it would keep to the same few
+// segments, whose children (consecutive parents, often in one part of the
+// workspace) can run long or short together.
+__device__ __forceinline__ void chunk_range(uint64_t k, uint32_t total, uint32_t per, uint32_t nseg, uint32_t chunk,
+                                            uint32_t* b, uint32_t* e)
+{
+    const uint64_t j = k / nseg, s = (k % nseg + j) % nseg;
+    const uint64_t s0 = s * per, lo = s0 + j * chunk;
+    uint64_t hi = s0 + (j + 1) * chunk;
+    if(hi > s0 + per) hi = s0 + per;
+    if(hi > total) hi = total;
+    *b = (uint32_t)(lo < total ? lo : total);
+    *e = (uint32_t)(hi > *b ? hi : *b);
+}
+
+// Issue priority of a wave from the steps its longest live edge has left, in
+// quarters q of the longest drawn edge (0..3).  A launch lasts as long as its
+// longest chain of steps; the waves holding the long edges issue first and
+// the waves with short edges fill their stalls, instead of every wave
+// sharing the SIMD evenly until the long edges are all that is left.
+__device__ __forceinline__ void set_issue_priority(int left, int q)
+{
+    if(__ballot(left > 3 * q))
+        __builtin_amdgcn_s_setprio(3);
+    else if(__ballot(left > 2 * q))
+        __builtin_amdgcn_s_setprio(2);
+    else if(__ballot(left > q))
+        __builtin_amdgcn_s_setprio(1);
+    else
+        __builtin_amdgcn_s_setprio(0);
+}
+
+// ---------------------------------------------------------------------------
 // Expansion: propagateFrontier_kernel1/2 (KPAX.cu:358-439) with counter-based
 // draws.  Waves are persistent and advance their 64 children one integration
 // step per trip through the loop; a lane whose child has finished (all its
@@ -283,23 +357,6 @@ __global__ __launch_bounds__(1024) void k_advance(AdvanceArgs V) { advance_block
 // environment's grid image in LDS (work-groups of 640 threads: two per CU
 // hold the five waves per SIMD and two images); the batched form's lanes
 // query the grids of several environments in global memory (256 threads).
-// Issue priority of a wave from the steps its longest live edge has left, in
-// quarters q of the longest drawn edge (0..3).  A launch lasts as long as its
-// longest chain of steps; the waves holding the long edges issue first and
-// the waves with short edges fill their stalls, instead of every wave
-// sharing the SIMD evenly until the long edges are all that is left.
-__device__ __forceinline__ void set_issue_priority(int left, int q)
-{
-    if(__ballot(left > 3 * q))
-        __builtin_amdgcn_s_setprio(3);
-    else if(__ballot(left > 2 * q))
-        __builtin_amdgcn_s_setprio(2);
-    else if(__ballot(left > q))
-        __builtin_amdgcn_s_setprio(1);
-    else
-        __builtin_amdgcn_s_setprio(0);
-}
-
 template<int MODEL, bool MULTI>
 __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 : 5))) void k_expand(
   const ExpandArgs* __restrict__ A)
@@ -328,9 +385,13 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
     const uint32_t group   = blockIdx.x % groups;
     const uint32_t rank0   = (blockIdx.x / groups) * (blockDim.x >> 6);  // the work-group's first wave
     const uint32_t rank    = rank0 + (threadIdx.x >> 6);
+    // work chunks: segments of the preparation pass, interleaved chunk by chunk
+    const uint32_t per    = segment_len(total, A->order_blocks, A->ordered != 0);
+    const uint32_t nseg   = (total + per - 1) / per;
+    const uint64_t chunks = (uint64_t)nseg * ((per + A->chunk - 1) / A->chunk);
     // a work-group none of whose waves has a first chunk leaves at once
     // (before the LDS staging, whose barrier needs every thread)
-    if(((uint64_t)rank0 * groups + group) * A->chunk >= total) return;
+    if((uint64_t)rank0 * groups + group >= chunks) return;
     const int R1 = RP.r1_count;
     // slot-uniform in the single-slot form, per lane in the batched form
     uint32_t t = 0, slot = 0, base = 0;
@@ -350,7 +411,7 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
         G.gp  = A->GP;  // grid parameters are shared; pointers are set per child
         G.lds = nullptr;
     }
-    if(((uint64_t)rank * groups + group) * A->chunk >= total) return;
+    if((uint64_t)rank * groups + group >= chunks) return;
     uint32_t* const ticket = A->tickets + group * kTicketStride;
     const int lane       = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -407,21 +468,30 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
                 {
                     if(q_next >= q_end)
                     {
-                        const uint32_t chunk = A->chunk;
-                        uint32_t b           = 0;
+                        // the next non-empty chunk of the group (chunks past the
+                        // end of the last, shorter segment are empty)
+                        uint32_t b = total, e = total;
                         if(lane == 0)
-                        {
-                            const uint64_t c = ((uint64_t)atomicAdd(ticket, 1u) * groups + group) * chunk;
-                            b                = c < total ? (uint32_t)c : total;
-                        }
+                            for(;;)
+                            {
+                                const uint64_t k = (uint64_t)atomicAdd(ticket, 1u) * groups + group;
+                                if(k >= chunks)
+                                {
+                                    b = e = total;
+                                    break;
+                                }
+                                chunk_range(k, total, per, nseg, A->chunk, &b, &e);
+                                if(b < e) break;
+                            }
                         b = __shfl(b, 0, 64);
+                        e = __shfl(e, 0, 64);
                         if(b >= total)
                         {
                             drained = true;
                             break;
                         }
                         q_next = b;
-                        q_end  = total - b > chunk ? b + chunk : total;
+                        q_end  = e;
                     }
                     const uint32_t avail = q_end - q_next;
                     const uint32_t rank  = (uint32_t)__popcll(need & below);
@@ -477,184 +547,103 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
     }
 }
 
-// ---------------------------------------------------------------------------
-// Processing order of the children: a counting sort by drawn step count,
-// longest first, so the last children a launch starts are short ones and the
-// launch does not wait on a long edge begun at its end.  Results do not depend
-// on the order (every child is a function of its index and of the tree
-// before the iteration).  Two launches: bucket counts per block, then the
-// scatter, each of whose blocks sums the counts it needs.
-// ---------------------------------------------------------------------------
-// 256 blocks of 1024 threads: four waves per SIMD hide the Philox chains and
-// the repeat-list loads (one wave per SIMD left both exposed), and the bucket
-// counts stay 256 per bucket.
-constexpr int kOrderBlocks = 256, kOrderThreads = 1024, kOrderBuckets = 32;
-static_assert(kOrderThreads % kOrderBuckets == 0, "k_prepare splits its threads evenly over the buckets");
-
-__device__ __forceinline__ uint32_t order_bucket(const ModelParams& P, int n)
-{
-    const uint32_t back = (uint32_t)(P.steps_min + P.span - 1 - n);  // 0 for the longest edge
-    return P.span <= kOrderBuckets ? back : (uint32_t)((uint64_t)back * kOrderBuckets / (uint32_t)P.span);
-}
-
-template<bool MULTI>
-__device__ __forceinline__ uint32_t order_total(const ExpandArgs* A)
-{
-    if constexpr(MULTI)
-        return A->bc->children;
-    else
-        return A->s0.cur.status == kRunning ? A->s0.cur.M : 0u;
-}
-
-template<bool MULTI>
-__device__ __forceinline__ uint32_t child_bucket(const ExpandArgs* A, uint32_t k)
-{
-    uint32_t c = k, t;
-    uint64_t seed;
-    uint32_t query;
-    if constexpr(MULTI)
-    {
-        const uint32_t s = segment_of(A->child_off, A->Q, k);
-        c                = k - A->child_off[s];
-        t                = A->ctl[s].cur.t;
-        seed             = A->qd[s].seed;
-        query            = A->qd[s].query;
-    }
-    else
-    {
-        t     = A->s0.cur.t;
-        seed  = A->s0.qd.seed;
-        query = A->s0.qd.query;
-    }
-    const uint4 ds = philox(c, t, query, PURPOSE_STEPS_ADMIT, (uint32_t)seed, (uint32_t)(seed >> 32));
-    return order_bucket(A->P, draw_steps(A->P, ds.x));
-}
-
-// advance = 1 (the single-slot form): block 0 first runs the iteration's
-// advance (k_advance, one launch fewer per iteration), and every block takes
-// the iteration record from the slot's `nxt`, which this launch does not
-// change, instead of the argument block's copy, which block 0 rewrites.
-template<bool MULTI>
-__global__ __launch_bounds__(kOrderThreads) void k_order_count(const ExpandArgs* __restrict__ A, uint32_t* hist,
-                                                               AdvanceArgs V, int advance)
-{
-    __shared__ uint32_t h[kOrderBuckets];
-    if constexpr(!MULTI)
-        if(advance && blockIdx.x == 0) advance_block(V);
-    if(threadIdx.x < kOrderBuckets) h[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t total;
-    if(!MULTI && advance)
-    {
-        const Iter n = A->ctl[0].nxt;
-        total        = n.status == kRunning ? n.M : 0u;
-        const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
-        const uint32_t b0 = blockIdx.x * per, b1 = b0 + per < total ? b0 + per : total;
-        const uint64_t seed = A->qd[0].seed;
-        const uint32_t query = A->qd[0].query;
-        for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x)
-        {
-            const uint4 ds = philox(k, n.t, query, PURPOSE_STEPS_ADMIT, (uint32_t)seed, (uint32_t)(seed >> 32));
-            atomicAdd(&h[order_bucket(A->P, draw_steps(A->P, ds.x))], 1u);
-        }
-    }
-    else
-    {
-        total              = order_total<MULTI>(A);
-        const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
-        const uint32_t b0 = blockIdx.x * per, b1 = b0 + per < total ? b0 + per : total;
-        for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x) atomicAdd(&h[child_bucket<MULTI>(A, k)], 1u);
-    }
-    __syncthreads();
-    if(threadIdx.x < kOrderBuckets) hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
-}
-
 // Preparation of the iteration's children, in a coalesced pass before the
 // expansion: each child's parent (repeat list), control draw (uniform, or the
 // parent's Gaussian with the query's proposal weight), step count and
 // admission draw, stored as a 32-byte record at its processing position, so
-// the expansion's refill is two vector loads and the parent state.  With
-// `hist` (the per-block bucket counts of k_order_count) the position follows
-// the longest-first order; without it, the child index.  Each ordering block
-// finds its own first position per bucket from the raw counts: the children
-// of all earlier buckets plus those of this bucket counted by earlier blocks.
-// Thread (bucket b, part j) sums 1/kOrderParts of b's block counts; one
-// thread per bucket adds up its parts, then each bucket's position sums the
-// totals of the buckets before it.
-constexpr int kOrderParts = kOrderThreads / kOrderBuckets;
-
+// the expansion's refill is two vector loads and the parent state.
+//
+// advance = 1 (the single-slot form): block 0 first runs the iteration's
+// advance (k_advance, one launch fewer per iteration), and every block takes
+// the iteration record and query from the slot's `nxt` and `qd`, which this
+// launch does not change, instead of the argument block's copies, which
+// block 0 rewrites.
 template<int MODEL, bool MULTI>
-__global__ __launch_bounds__(kOrderThreads) void k_prepare(const ExpandArgs* __restrict__ A, const uint32_t* hist)
+__global__ __launch_bounds__(kOrderThreads) void k_prepare(const ExpandArgs* __restrict__ A, int ordered,
+                                                           AdvanceArgs V, int advance)
 {
     constexpr int C = Dims<MODEL>::C;
-    __shared__ uint32_t pos[kOrderBuckets], s_tot[kOrderBuckets];
-    __shared__ uint32_t s_all[kOrderBuckets][kOrderParts], s_before[kOrderBuckets][kOrderParts];
-    if(hist)
+    __shared__ uint32_t pos[kOrderBuckets];
+    if constexpr(!MULTI)
+        if(advance && blockIdx.x == 0) advance_block(V);
+    uint32_t total;
+    Iter it0{};
+    QueryDev q0{};
+    if constexpr(MULTI)
+        total = A->bc->children;
+    else
     {
-        {
-            const uint32_t b = threadIdx.x / kOrderParts, part = threadIdx.x % kOrderParts;
-            const uint32_t per = (gridDim.x + kOrderParts - 1) / kOrderParts;
-            const uint32_t j0 = part * per, j1 = j0 + per < gridDim.x ? j0 + per : gridDim.x;
-            uint32_t all = 0, before = 0;
-            for(uint32_t j = j0; j < j1; ++j)
-            {
-                const uint32_t v = hist[b * gridDim.x + j];
-                all += v;
-                before += j < blockIdx.x ? v : 0u;
-            }
-            s_all[b][part]    = all;
-            s_before[b][part] = before;
-        }
-        __syncthreads();
-        if(threadIdx.x < kOrderBuckets)
-        {
-            uint32_t all = 0, before = 0;
-            for(int j = 0; j < kOrderParts; ++j)
-            {
-                all += s_all[threadIdx.x][j];
-                before += s_before[threadIdx.x][j];
-            }
-            s_tot[threadIdx.x] = all;
-            pos[threadIdx.x]   = before;
-        }
-        __syncthreads();
-        if(threadIdx.x < kOrderBuckets)
-        {
-            uint32_t p = pos[threadIdx.x];
-            for(uint32_t b = 0; b < threadIdx.x; ++b) p += s_tot[b];
-            pos[threadIdx.x] = p;
-        }
-        __syncthreads();
+        it0   = advance ? A->ctl[0].nxt : A->s0.cur;
+        q0    = advance ? A->qd[0] : A->s0.qd;
+        total = it0.status == kRunning ? it0.M : 0u;
     }
     const ModelParams& P = A->P;
-    const uint32_t total = order_total<MULTI>(A);
-    const uint32_t per   = (total + gridDim.x - 1) / gridDim.x;
+    // ordered: this work-group's segment; child order: an even share
+    const uint32_t per   = ordered ? segment_len(total, gridDim.x, true) : (total + gridDim.x - 1) / gridDim.x;
     const uint32_t b0 = blockIdx.x * per, b1 = b0 + per < total ? b0 + per : total;
-    for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x)
-    {
-        uint32_t c = k, t, fan, base, query, slot = 0;
-        uint64_t seed;
-        float mix;
+    // the draws of child c (index c - child_off[slot] within its slot)
+    auto draws = [&](uint32_t k, uint32_t* c, uint32_t* t, uint32_t* slot, uint64_t* seed, uint32_t* query) {
         if constexpr(MULTI)
         {
-            slot              = segment_of(A->child_off, A->Q, k);
-            c                 = k - A->child_off[slot];
-            t                 = A->ctl[slot].cur.t;
-            fan               = A->ctl[slot].cur.fan;
-            const QueryDev& d = A->qd[slot];
-            base              = d.base;
-            seed              = d.seed;
-            query             = d.query;
-            mix               = d.mix;
+            *slot  = segment_of(A->child_off, A->Q, k);
+            *c     = k - A->child_off[*slot];
+            *t     = A->ctl[*slot].cur.t;
+            *seed  = A->qd[*slot].seed;
+            *query = A->qd[*slot].query;
         }
         else
         {
-            t     = A->s0.cur.t;
-            fan   = A->s0.cur.fan;
-            base  = A->s0.qd.base;
-            seed  = A->s0.qd.seed;
-            query = A->s0.qd.query;
-            mix   = A->s0.qd.mix;
+            *slot  = 0;
+            *c     = k;
+            *t     = it0.t;
+            *seed  = q0.seed;
+            *query = q0.query;
+        }
+    };
+    if(ordered)
+    {
+        // counting sort of the segment: bucket counts, then their exclusive
+        // scan (longest bucket first) from the segment's start
+        if(threadIdx.x < kOrderBuckets) pos[threadIdx.x] = 0;
+        __syncthreads();
+        for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x)
+        {
+            uint32_t c, t, slot, query;
+            uint64_t seed;
+            draws(k, &c, &t, &slot, &seed, &query);
+            const uint4 ds = philox(c, t, query, PURPOSE_STEPS_ADMIT, (uint32_t)seed, (uint32_t)(seed >> 32));
+            atomicAdd(&pos[order_bucket(P, draw_steps(P, ds.x))], 1u);
+        }
+        __syncthreads();
+        if(threadIdx.x == 0)
+        {
+            uint32_t run = b0;
+            for(int b = 0; b < kOrderBuckets; ++b)
+            {
+                const uint32_t n = pos[b];
+                pos[b]           = run;
+                run += n;
+            }
+        }
+        __syncthreads();
+    }
+    for(uint32_t k = b0 + threadIdx.x; k < b1; k += blockDim.x)
+    {
+        uint32_t c, t, slot, query, fan, base;
+        uint64_t seed;
+        float mix;
+        draws(k, &c, &t, &slot, &seed, &query);
+        if constexpr(MULTI)
+        {
+            fan  = A->ctl[slot].cur.fan;
+            base = A->qd[slot].base;
+            mix  = A->qd[slot].mix;
+        }
+        else
+        {
+            fan  = it0.fan;
+            base = q0.base;
+            mix  = q0.mix;
         }
         const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
         const uint32_t parent = A->R[base + c / fan];
@@ -679,7 +668,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_prepare(const ExpandArgs* __r
             propose_control<C>(mean, sg, dp, P, u);
         }
         const int n      = draw_steps(P, ds.x);
-        const uint32_t p = hist ? atomicAdd(&pos[order_bucket(P, n)], 1u) : k;
+        const uint32_t p = ordered ? atomicAdd(&pos[order_bucket(P, n)], 1u) : k;
         uint32_t uw[4]   = {0u, 0u, 0u, 0u};
 #pragma unroll
         for(int j = 0; j < C; ++j) uw[j] = __builtin_bit_cast(uint32_t, u[j]);
@@ -1094,6 +1083,17 @@ uint32_t expand_blocks(const kpax_handle* h)
     }
 }
 
+// Work-groups of the preparation pass (its segments of the longest-first
+// order): 256 (one per CU on MI355X), fewer on a device with fewer CUs.
+uint32_t order_blocks(const kpax_handle* h)
+{
+    return std::min<uint32_t>(kOrderBlocks, std::max<uint32_t>(1u, (uint32_t)h->cu_count));
+}
+
+// The advance runs inside the preparation pass of a single-slot handle
+// instead of as its own launch.
+bool fused_advance(const kpax_handle* h) { return h->Qmax == 1; }
+
 int upload_expand_args(kpax_handle* h)
 {
     ExpandArgs a;
@@ -1119,6 +1119,8 @@ int upload_expand_args(kpax_handle* h)
     a.refill    = h->refill_lanes;
     a.use_lds   = h->Qmax == 1 && h->use_lds && h->lds_bytes > 0 ? 1u : 0u;
     a.prio_q    = h->wave_priority ? std::max(1, (h->P.steps_min + h->P.span - 1 + 3) / 4) : 0;
+    a.ordered   = h->order_children ? 1u : 0u;
+    a.order_blocks = order_blocks(h);
     a.prop      = h->prop;
     memset(&a.s0, 0, sizeof(a.s0));
     a.s0.cur.status = kNotStarted;
@@ -1137,18 +1139,6 @@ void launch_expand_model(kpax_handle* h)
         hipLaunchKernelGGL((k_expand<MODEL, true>), dim3(h->expand_blocks), dim3(h->expand_threads), 0, h->stream, A);
 }
 
-// The children's records for this iteration (k_prepare), in longest-first
-// order after the bucket counts of k_order_count, or in child order.
-// Work-groups of the ordering and preparation passes: 256 (one per CU on
-// MI355X), fewer on a device with fewer CUs.
-uint32_t order_blocks(const kpax_handle* h)
-{
-    return std::min<uint32_t>(kOrderBlocks, std::max<uint32_t>(1u, (uint32_t)h->cu_count));
-}
-
-// The advance runs inside the ordering pass (single-slot handles with the
-// longest-first order) instead of as its own launch.
-bool fused_advance(const kpax_handle* h) { return h->Qmax == 1 && h->order_children; }
 
 AdvanceArgs advance_args(const kpax_handle* h)
 {
@@ -1172,25 +1162,20 @@ AdvanceArgs advance_args(const kpax_handle* h)
     return V;
 }
 
+// The children's records for this iteration (k_prepare), each segment in
+// longest-first order or all in child order.
 template<int MODEL>
 void launch_prepare(kpax_handle* h)
 {
-    const ExpandArgs* A  = (const ExpandArgs*)h->expand_args;
-    const uint32_t* hist = h->order_children ? h->order_hist : nullptr;
+    const ExpandArgs* A = (const ExpandArgs*)h->expand_args;
     const dim3 ob(order_blocks(h));
     const AdvanceArgs V = advance_args(h);
+    const int ordered   = h->order_children ? 1 : 0;
     if(h->Qmax == 1)
-    {
-        if(hist)
-            hipLaunchKernelGGL((k_order_count<false>), ob, dim3(kOrderThreads), 0, h->stream, A, h->order_hist, V,
-                               fused_advance(h) ? 1 : 0);
-        hipLaunchKernelGGL((k_prepare<MODEL, false>), ob, dim3(kOrderThreads), 0, h->stream, A, hist);
-    }
+        hipLaunchKernelGGL((k_prepare<MODEL, false>), ob, dim3(kOrderThreads), 0, h->stream, A, ordered, V,
+                           fused_advance(h) ? 1 : 0);
     else
-    {
-        if(hist) hipLaunchKernelGGL((k_order_count<true>), ob, dim3(kOrderThreads), 0, h->stream, A, h->order_hist, V, 0);
-        hipLaunchKernelGGL((k_prepare<MODEL, true>), ob, dim3(kOrderThreads), 0, h->stream, A, hist);
-    }
+        hipLaunchKernelGGL((k_prepare<MODEL, true>), ob, dim3(kOrderThreads), 0, h->stream, A, ordered, V, 0);
 }
 
 // The preparation passes, then the expansion; `between` (profiling) is
@@ -1807,12 +1792,14 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     h->PP    = make_plan_params(*cfg);
     h->T     = (uint32_t)node_capacity;
     h->Qmax  = max_queries;
-    // Longest-edge-first ordering pays in the single-query form.  In the
-    // batched form it mixes the children of all running queries (their
-    // environments, grids and node ranges) within each wave, where child
-    // order keeps a wave's lanes on one query: the serving batch ran at 2,750
-    // plans/s without it and 2,270 with it.
-    h->order_children = max_queries == 1;
+    // Longest-edge-first ordering pays in the single-query form of the
+    // quadcopter and the Dubins airplane.  The double integrator's short
+    // steps leave too little to gain for the extra counting pass (plan 1.146
+    // ms with it, 1.128 without).  In the batched form it mixes the children
+    // of all running queries (their environments, grids and node ranges)
+    // within each wave, where child order keeps a wave's lanes on one query:
+    // the serving batch ran at 2,750 plans/s without it and 2,270 with it.
+    h->order_children = max_queries == 1 && cfg->model != KPAX_MODEL_DOUBLE_INTEGRATOR;
     // Issue priority by the longest live edge pays where a step is long
     // (quadcopter forest: expansion 2.77 -> 2.64 ms per plan); the double
     // integrator's steps are short and its iterations small, and the ballots
@@ -1882,7 +1869,6 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->atile_off, Q + 1));
     TRY(dalloc(h, &h->ftile_off, Q + 1));
     TRY(dalloc(h, &h->prep, 2 * T));
-    TRY(dalloc(h, &h->order_hist, (size_t)kOrderBlocks * kOrderBuckets));
     TRY(dalloc(h, &h->slot_status, Q + 1));
     h->n_status = (uint32_t)(T / kTile + Q + 2);
     TRY(dalloc(h, &h->st_append, h->n_status));

@@ -237,7 +237,9 @@ inline hipError_t hipGetSymbolAddress(void** p, const void* symbol)
 }
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int)
 {
-    p->multiProcessorCount              = 1;
+    // KPAX_EMU_CUS: the compute units the emulated device reports (default 1)
+    const char* cus                     = std::getenv("KPAX_EMU_CUS");
+    p->multiProcessorCount              = cus ? std::max(1, std::atoi(cus)) : 1;
     p->sharedMemPerBlock                = 160 << 10;
     p->maxSharedMemoryPerMultiProcessor = 160 << 10;
     return hipSuccess;
