@@ -560,6 +560,33 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* agg, u
     return before + inc - v;
 }
 
+// block_excl_scan of v together with the block total of w, in the same two
+// barriers; lds must hold 2 NW words.
+template<int NW>
+__device__ __forceinline__ uint32_t block_excl_scan_sum(uint32_t v, uint32_t w, uint32_t* agg, uint32_t* wtot,
+                                                        uint32_t* lds)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    const uint32_t ws  = wave_sum(w);
+    if(lane == 63) lds[wid] = inc;
+    if(lane == 0) lds[NW + wid] = ws;
+    __syncthreads();
+    uint32_t before = 0, total = 0, wt = 0;
+#pragma unroll
+    for(int k = 0; k < NW; ++k)
+    {
+        const uint32_t s = lds[k];
+        if(k < wid) before += s;
+        total += s;
+        wt += lds[NW + k];
+    }
+    *agg  = total;
+    *wtot = wt;
+    __syncthreads();
+    return before + inc - v;
+}
+
 __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* agg, uint32_t* lds)
 {
     return block_excl_scan<4>(v, agg, lds);
@@ -613,6 +640,53 @@ __device__ __forceinline__ uint32_t lookback(uint64_t* status, uint32_t tile, ui
     }
     if(lane == 0) status_store(&status[tile], status_word(kFlagPrefix, epoch, excl + agg));
     return excl;
+}
+
+// The look-back in two halves, so a block can work between publishing its
+// aggregate and waiting on its predecessors: tile_publish (one thread) makes
+// the tile's aggregate visible (or its prefix, for a segment's first tile);
+// tile_prefix_published (block-wide, two barriers) then returns the tile's
+// exclusive prefix and publishes its inclusive one.
+__device__ __forceinline__ void tile_publish(uint64_t* status, uint32_t tile, uint32_t agg, uint32_t first,
+                                             uint32_t epoch)
+{
+    status_store(&status[tile], status_word(tile == first ? kFlagPrefix : kFlagAgg, epoch, agg));
+}
+
+__device__ __forceinline__ uint32_t tile_prefix_published(uint64_t* status, uint32_t tile, uint32_t agg,
+                                                          uint32_t first, uint32_t epoch, uint32_t* lds_word)
+{
+    if(threadIdx.x < 64)
+    {
+        const int lane = threadIdx.x & 63;
+        uint32_t excl  = 0;
+        if(tile != first)
+        {
+            const uint32_t ep = epoch & kEpochMask;
+            int64_t base      = (int64_t)tile - 1;
+            while(true)
+            {
+                const int64_t j = base - lane;
+                uint64_t s      = kFlagPrefix;  // before the segment: an empty prefix
+                if(j >= (int64_t)first)
+                {
+                    s = status_load(&status[j]);
+                    while((s >> 62) == 0 || (uint32_t)((s >> 32) & kEpochMask) != ep) s = status_load(&status[j]);
+                }
+                const uint64_t pm = __ballot((s >> 62) == 2);
+                const int stop    = pm ? __builtin_ctzll(pm) : 64;
+                excl += wave_sum(lane <= stop ? (uint32_t)s : 0u);
+                if(pm) break;
+                base -= 64;
+            }
+            if(lane == 0) status_store(&status[tile], status_word(kFlagPrefix, epoch, excl + agg));
+        }
+        if(threadIdx.x == 0) *lds_word = excl;
+    }
+    __syncthreads();
+    const uint32_t e = *lds_word;
+    __syncthreads();
+    return e;
 }
 
 // Block-wide: exclusive prefix of this tile given its aggregate.

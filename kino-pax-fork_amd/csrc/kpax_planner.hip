@@ -694,35 +694,67 @@ __device__ __forceinline__ float block_scores(const int32_t* va, const int32_t* 
                                               float* part, const RegionParams& RP, float eps)
 {
     const int t = threadIdx.x;
+    const int R1 = RP.r1_count;
     for(int pi = t; pi < 1024; pi += NT)
     {
         float s = 0.0f;
-        for(int r = pi; r < RP.r1_count; r += 1024)
+        // four regions per round, their counters loaded together (clamped
+        // indices: the loads are unconditional, so they issue as one round trip)
+        for(int r0 = pi; r0 < R1; r0 += 4 * 1024)
         {
-            float raw   = 0.0f;
-            const int v = va[r];
-            if(v > 0)
+            int v[4], n0[4], cv[4];
+#pragma unroll
+            for(int k = 0; k < 4; ++k)
             {
-                const int n0    = to[r];
-                const float cov = (float)co[r] / (float)RP.r2_per;
-                const float ev  = eps + (float)v;
-                const float fv  = ev / (ev + (float)(n0 - v)) * RP.vol;
-                const float n   = (float)n0;
-                raw             = (fv * fv) * (fv * fv) / ((1.0f + cov) * (1.0f + n * n));
+                const int r  = r0 + 1024 * k;
+                const int rr = r < R1 ? r : R1 - 1;
+                v[k]         = va[rr];
+                n0[k]        = to[rr];
+                cv[k]        = co[rr];
             }
-            sc[r] = raw;
-            s += raw;
+#pragma unroll
+            for(int k = 0; k < 4; ++k)
+            {
+                const int r = r0 + 1024 * k;
+                if(r >= R1) break;
+                float raw = 0.0f;
+                if(v[k] > 0)
+                {
+                    const float cov = (float)cv[k] / (float)RP.r2_per;
+                    const float ev  = eps + (float)v[k];
+                    const float fv  = ev / (ev + (float)(n0[k] - v[k])) * RP.vol;
+                    const float n   = (float)n0[k];
+                    raw             = (fv * fv) * (fv * fv) / ((1.0f + cov) * (1.0f + n * n));
+                }
+                sc[r] = raw;
+                s += raw;
+            }
         }
         part[pi] = s;
     }
     __syncthreads();
-    for(int off = 512; off >= 1; off >>= 1)
+    // the pairwise tree part[i] += part[i + off], off = 512 .. 1: the levels
+    // down to 64 partials across the block, the last six within wave 0 by
+    // shuffles (the same additions in the same order, without barriers)
+    for(int off = 512; off >= 64; off >>= 1)
     {
         for(int pi = t; pi < off; pi += NT) part[pi] = part[pi] + part[pi + off];
         __syncthreads();
     }
+    if(t < 64)
+    {
+        float x = part[t];
+#pragma unroll
+        for(int off = 32; off >= 1; off >>= 1)
+        {
+            const float y = __shfl_down(x, off, 64);
+            if(t < off) x = x + y;
+        }
+        if(t == 0) part[0] = x;
+    }
+    __syncthreads();
     const float tot = part[0];
-    for(int r = t; r < RP.r1_count; r += NT) sc[r] = va[r] == 0 ? 1.0f : eps + sc[r] / tot;
+    for(int r = t; r < R1; r += NT) sc[r] = va[r] == 0 ? 1.0f : eps + sc[r] / tot;
     __syncthreads();
     return tot;
 }
@@ -748,6 +780,12 @@ __global__ __launch_bounds__(1024) void k_score(Ctl* ctl, RegionView rv, RegionP
 // regions it touched reach HBM: siblings land in the same few regions, and
 // per-child device-scope atomics would serialise on them.
 // ---------------------------------------------------------------------------
+// The tile's critical path is a chain of dependent memory round trips, so
+// the work that does not depend on the scan runs while it waits: the tile's
+// ticket is taken while the LDS counters are cleared; the first admitted
+// child's record is loaded before the scan; the aggregate is published
+// before the counters are flushed, and the flush (fire-and-forget atomics)
+// runs before the wait on the predecessor tiles.
 template<int NT>
 __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, BatchCtl* bc, const uint32_t* atile_off,
                                                uint32_t Q, ChildView ch, TreeView tree, RegionView rv,
@@ -755,7 +793,7 @@ __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, Bat
                                                uint32_t R1, uint32_t R2)
 {
     constexpr uint32_t kT = 4 * NT;  // children per tile
-    __shared__ uint32_t lds[NT / 64];
+    __shared__ uint32_t lds[2 * (NT / 64)];
     __shared__ uint32_t s_tile, s_pre;
     HIP_DYNAMIC_SHARED(int32_t, s_hist)
     const uint32_t ntiles = bc->atiles, epoch = bc->epoch;
@@ -763,13 +801,14 @@ __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, Bat
     const bool local      = R1 <= (uint32_t)kLdsRegions;
     for(;;)
     {
-        const uint32_t tile = next_tile(&bc->tile_append, &s_tile);
-        if(tile >= ntiles) break;
+        // the ticket's round trip overlaps the clearing of the counters (the
+        // previous tile's flush ended before its look-back's barriers)
+        if(threadIdx.x == 0) s_tile = atomicAdd(&bc->tile_append, 1u);
         if(local)
-        {
             for(uint32_t r = threadIdx.x; r < 2 * R1; r += blockDim.x) s_hist[r] = 0;
-            __syncthreads();
-        }
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if(tile >= ntiles) break;
         const uint32_t q      = segment_of(atile_off, Q, tile);
         const uint32_t first  = atile_off[q], lt = tile - first, nt = atile_off[q + 1] - first;
         const Iter it         = ctl[q].cur;
@@ -782,17 +821,31 @@ __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, Bat
         int32_t* val          = rv.valid + (size_t)q * R1;
         const uint32_t cbase  = lt * kT + threadIdx.x * 4;
         uint32_t f[4], cnt = 0, steps = 0;
+        // The four meta words, then the cell words of the candidates, each as
+        // one round trip: the loads are unconditional (past the last child,
+        // the last child's word; for a non-candidate, cell 0's word) so they
+        // issue together instead of one wait per child.
+        uint4 mj[4];  // the children's meta words: steps | flags, parent, region, R2 cell
+        uint64_t key[4];
 #pragma unroll
         for(int j = 0; j < 4; ++j)
         {
             const uint32_t c = cbase + j;
-            uint4 meta       = make_uint4(0u, 0u, 0u, 0u);  // steps | flags, parent, region, R2 cell
-            uint32_t a       = 0u;
-            if(c < M)
-            {
-                meta = ch.meta[base + c];
-                a    = meta.x >> kMetaFlagShift;
-            }
+            mj[j]            = ch.meta[base + (c < M ? c : M - 1)];
+        }
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            const bool cand = cbase + j < M && ((mj[j].x >> kMetaFlagShift) & kCandidate);
+            key[j]          = r2s[cand ? mj[j].w : 0u];
+        }
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            const uint32_t c    = cbase + j;
+            if(c >= M) mj[j] = make_uint4(0u, 0u, 0u, 0u);
+            const uint4 meta    = mj[j];
+            const uint32_t a    = meta.x >> kMetaFlagShift;
             if(c < M)
             {
                 steps += meta.x & ((1u << kMetaFlagShift) - 1u);
@@ -813,7 +866,7 @@ __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, Bat
             bool first_in_cell = false;
             if(a & kCandidate)
             {
-                first_in_cell = r2s[meta.w] == cell_key(it.t, c);
+                first_in_cell = key[j] == cell_key(it.t, c);
                 if(first_in_cell) atomicAdd(&cov[meta.z], 1);
             }
             f[j] = (a & kValid) && ((a & kScoreAdmit) || first_in_cell) ? 1u : 0u;
@@ -831,12 +884,33 @@ __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, Bat
             }
             cnt += f[j];
         }
+        // the first admitted child's record, loaded while the tile scans
+        int jf = -1;
+#pragma unroll
+        for(int j = 3; j >= 0; --j)
+            if(f[j]) jf = j;
+        uint4 rf[kChildWords / 4];
+        if(jf >= 0)
+        {
+            const uint4* rec = ch.rec + (base + cbase + (uint32_t)jf) * (kChildWords / 4);
+#pragma unroll
+            for(int v = 0; v < kChildWords / 4; ++v) rf[v] = rec[v];
+        }
         uint32_t agg, steps_tile;
-        (void)block_excl_scan<NT / 64>(steps, &steps_tile, lds);
-        if(threadIdx.x == 0 && steps_tile)
-            atomicAdd((unsigned long long*)&ctl[q].steps_done, (unsigned long long)steps_tile);
-        const uint32_t loc = block_excl_scan<NT / 64>(cnt, &agg, lds);
-        const uint32_t pre = tile_prefix(status, tile, agg, first, epoch, &s_pre);
+        const uint32_t loc = block_excl_scan_sum<NT / 64>(cnt, steps, &agg, &steps_tile, lds);
+        if(threadIdx.x == 0)
+        {
+            tile_publish(status, tile, agg, first, epoch);
+            if(steps_tile) atomicAdd((unsigned long long*)&ctl[q].steps_done, (unsigned long long)steps_tile);
+        }
+        if(local)
+            for(uint32_t r = threadIdx.x; r < R1; r += blockDim.x)
+            {
+                const int32_t a = s_hist[r], b = s_hist[R1 + r];
+                if(a) atomicAdd(&tot[r], a);
+                if(b) atomicAdd(&val[r], b);
+            }
+        const uint32_t pre = tile_prefix_published(status, tile, agg, first, epoch, &s_pre);
         uint32_t dst       = ts + pre + loc;
 #pragma unroll
         for(int j = 0; j < 4; ++j)
@@ -848,18 +922,28 @@ __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, Bat
                 ctl[q].overflow = 1;  // cannot happen while M <= cap - ts; never write past the range
                 break;
             }
-            const size_t o   = base + dst;
-            const uint4* rec = ch.rec + c * (kChildWords / 4);
-            const uint4 m4   = ch.meta[c];
+            const size_t o = base + dst;
+            uint4 r4[kChildWords / 4];
+            if(j == jf)
+            {
+#pragma unroll
+                for(int v = 0; v < kChildWords / 4; ++v) r4[v] = rf[v];
+            }
+            else
+            {
+                const uint4* rec = ch.rec + c * (kChildWords / 4);
+#pragma unroll
+                for(int v = 0; v < kChildWords / 4; ++v) r4[v] = rec[v];
+            }
+            const uint4 m4 = mj[j];
             uint32_t w[kChildWords];
 #pragma unroll
             for(int v = 0; v < kChildWords / 4; ++v)
             {
-                const uint4 q4 = rec[v];
-                w[4 * v]       = q4.x;
-                w[4 * v + 1]   = q4.y;
-                w[4 * v + 2]   = q4.z;
-                w[4 * v + 3]   = q4.w;
+                w[4 * v]     = r4[v].x;
+                w[4 * v + 1] = r4[v].y;
+                w[4 * v + 2] = r4[v].z;
+                w[4 * v + 3] = r4[v].w;
             }
             for(int dd = 0; dd < S; ++dd) tree.st[(size_t)dd * T + o] = __builtin_bit_cast(float, w[dd]);
             for(int dd = 0; dd < C; ++dd) tree.ct[(size_t)dd * T + o] = __builtin_bit_cast(float, w[12 + dd]);
@@ -877,16 +961,6 @@ __global__ __launch_bounds__(NT) void k_append(Ctl* ctl, const QueryDev* qd, Bat
             const uint32_t Fn = pre + agg;
             ctl[q].Fn         = Fn;
             ctl[q].f_accept   = f_accept(it.t, ts + Fn, cap, PP.eps);
-        }
-        if(local)
-        {
-            __syncthreads();
-            for(uint32_t r = threadIdx.x; r < R1; r += blockDim.x)
-            {
-                const int32_t a = s_hist[r], b = s_hist[R1 + r];
-                if(a) atomicAdd(&tot[r], a);
-                if(b) atomicAdd(&val[r], b);
-            }
         }
     }
 }
@@ -914,6 +988,8 @@ __global__ __launch_bounds__(NT) void k_frontier(Ctl* ctl, const QueryDev* qd, B
     HIP_DYNAMIC_SHARED(float, s_dyn)
     const uint32_t ntiles = bc->ftiles, epoch = bc->epoch;
     if(blockIdx.x >= ntiles) return;  // no ticket to take: leave the counter to the blocks with work
+    // the first ticket's round trip overlaps the scores
+    if(threadIdx.x == 0) s_tile = atomicAdd(&bc->tile_frontier, 1u);
     const float* lscore = nullptr;
     if constexpr(SCORE)
     {
@@ -926,9 +1002,11 @@ __global__ __launch_bounds__(NT) void k_frontier(Ctl* ctl, const QueryDev* qd, B
         }
         lscore = s_dyn;
     }
-    for(;;)
+    __syncthreads();
+    uint32_t tile = s_tile;
+    __syncthreads();  // every thread has the first ticket before the next is taken
+    for(;; tile = next_tile(&bc->tile_frontier, &s_tile))
     {
-        const uint32_t tile = next_tile(&bc->tile_frontier, &s_tile);
         if(tile >= ntiles) break;
         const uint32_t q     = segment_of(ftile_off, Q, tile);
         const uint32_t first = ftile_off[q], lt = tile - first;
@@ -957,6 +1035,17 @@ __global__ __launch_bounds__(NT) void k_frontier(Ctl* ctl, const QueryDev* qd, B
         const float* score     = SCORE ? lscore : rv.score + (size_t)q * R1;
         const int32_t* valid   = rv.valid + (size_t)q * R1;
         const uint32_t nbase   = lt * kT + threadIdx.x * 4;
+        // the nodes' regions, then the new nodes' valid counts, each as one
+        // round trip (unconditional loads at clamped indices; tsn >= 1)
+        int rg[4], vc[4];
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            const uint32_t node = nbase + j;
+            rg[j]               = tree.reg[base + (node < tsn ? node : tsn - 1)];
+        }
+#pragma unroll
+        for(int j = 0; j < 4; ++j) vc[j] = valid[rg[j]];
         uint32_t w[4], cnt = 0;
 #pragma unroll
         for(int j = 0; j < 4; ++j)
@@ -965,14 +1054,14 @@ __global__ __launch_bounds__(NT) void k_frontier(Ctl* ctl, const QueryDev* qd, B
             w[j]                = 0;
             if(node < tsn)
             {
-                const int r = tree.reg[base + node];
+                const int r = rg[j];
                 if(node < it.ts)
                 {
                     const uint4 dr = philox(node, it.t, d.query, PURPOSE_READMIT, k0, k1);
                     w[j]           = u01(dr.x) <= score[r] + fa ? 1u : 0u;
                 }
                 else
-                    w[j] = valid[r] < PP.sparse_valid_below ? PP.sparse_repeat : PP.dense_repeat;
+                    w[j] = vc[j] < PP.sparse_valid_below ? PP.sparse_repeat : PP.dense_repeat;
             }
             cnt += w[j];
         }

@@ -107,6 +107,17 @@ template<class T> inline T __shfl_xor(T x, unsigned m, int = 64)
     return y;
 }
 
+template<class T> inline T __shfl_down(T x, unsigned off, int = 64)
+{
+    const unsigned lane = threadIdx.x % 64;
+    uint64_t bits       = 0;
+    std::memcpy(&bits, &x, sizeof(T));
+    const uint64_t r = emu_exchange(bits, lane + off < 64 ? lane + off : lane);
+    T y;
+    std::memcpy(&y, &r, sizeof(T));
+    return y;
+}
+
 template<class T> inline T __shfl(T x, int src, int = 64)
 {
     uint64_t bits = 0;
