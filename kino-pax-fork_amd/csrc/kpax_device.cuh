@@ -87,11 +87,12 @@ __device__ __forceinline__ void sincos_f32(float x, float* s, float* c)
 
 // Truncation toward zero clamped to [0, len-1], NaN -> 0; identical to the
 // reference's (int) cast + clamp for finite inputs (Graph.cu:107-109).
+// Written without branches: max(f, 0) is 0 for NaN, negatives and -0, and
+// min(., len - 1) truncates to len - 1 for every f >= len - 1, so this is
+// `f >= 0 ? (f >= len ? len - 1 : (int)f) : 0` for every float f.
 __host__ __device__ __forceinline__ int clamp_index(float f, int len)
 {
-    if(!(f >= 0.0f)) return 0;
-    if(f >= (float)len) return len - 1;
-    return (int)f;
+    return (int)fminf(fmaxf(f, 0.0f), (float)(len - 1));
 }
 
 __device__ __forceinline__ bool inb(float v, float lo, float hi) { return v >= lo && v <= hi; }
@@ -198,9 +199,8 @@ __device__ __forceinline__ void stage_lds_grid(GridView& g, uint4* lds)
     g.lds = (const uint8_t*)lds;
 }
 
-__device__ __forceinline__ bool seg_free(const GridView& g, const float bmin[3], const float bmax[3])
+__device__ __forceinline__ bool seg_free_csr(const GridView& g, const float bmin[3], const float bmax[3])
 {
-    if(g.lds) return seg_free_lds(g, bmin, bmax);
     int lo[3], hi[3];
 #pragma unroll
     for(int d = 0; d < 3; ++d)
@@ -242,6 +242,23 @@ __device__ __forceinline__ bool seg_free(const GridView& g, const float bmin[3],
     return true;
 }
 
+// Grid query path, fixed at compile time where a kernel knows it (so the
+// other path's code and registers stay out of it): kGridAny takes the staged
+// LDS image when there is one, else the CSR grid.
+enum : int { kGridAny = 0, kGridLds = 1, kGridCsr = 2 };
+
+template<int GQ = kGridAny>
+__device__ __forceinline__ bool seg_free(const GridView& g, const float bmin[3], const float bmax[3])
+{
+    if constexpr(GQ == kGridLds)
+        return seg_free_lds(g, bmin, bmax);
+    else if constexpr(GQ == kGridCsr)
+        return seg_free_csr(g, bmin, bmax);
+    else
+        return g.lds ? seg_free_lds(g, bmin, bmax) : seg_free_csr(g, bmin, bmax);
+}
+
+template<int GQ = kGridAny>
 __device__ __forceinline__ bool ws_and_collision(const ModelParams& P, const GridView& G, const float p0[3],
                                                  const float p1[3])
 {
@@ -256,7 +273,7 @@ __device__ __forceinline__ bool ws_and_collision(const ModelParams& P, const Gri
         bmin[d]       = sw ? p1[d] : p0[d];
         bmax[d]       = sw ? p0[d] : p1[d];
     }
-    return seg_free(G, bmin, bmax);
+    return seg_free<GQ>(G, bmin, bmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -281,6 +298,7 @@ template<> struct Dims<KPAX_MODEL_QUADCOPTER> {
 // float result does not depend on how steps are scheduled.
 
 // Double integrator: statePropagatorSpatialHash.cu:90-128.
+template<int GQ = kGridAny>
 __device__ __forceinline__ bool step_di(const ModelParams& P, const GridView& G, float* s, const float* u)
 {
     const float dt = P.dt;
@@ -308,10 +326,11 @@ __device__ __forceinline__ bool step_di(const ModelParams& P, const GridView& G,
     s[0] = x, s[1] = y, s[2] = z, s[3] = vx, s[4] = vy, s[5] = vz;
     if(!inb(vx, P.vmin, P.vmax) || !inb(vy, P.vmin, P.vmax) || !inb(vz, P.vmin, P.vmax)) return false;
     const float p1[3] = {x, y, z};
-    return ws_and_collision(P, G, p0, p1);
+    return ws_and_collision<GQ>(P, G, p0, p1);
 }
 
 // Dubins airplane: statePropagatorSpatialHash.cu:158-214.
+template<int GQ = kGridAny>
 __device__ __forceinline__ bool step_dubins(const ModelParams& P, const GridView& G, float* s, const float* u)
 {
     const float dt = P.dt, h = P.half, d6 = P.d6;
@@ -339,7 +358,7 @@ __device__ __forceinline__ bool step_dubins(const ModelParams& P, const GridView
     s[0] = x, s[1] = y, s[2] = z, s[3] = yaw, s[4] = pitch, s[5] = v;
     if(!inb(v, P.vmin, P.vmax) || !inb(pitch, P.pmin, P.pmax)) return false;
     const float p1[3] = {x, y, z};
-    return ws_and_collision(P, G, p0, p1);
+    return ws_and_collision<GQ>(P, G, p0, p1);
 }
 
 // 12D quadcopter ode: statePropagator.cu:325-370.  The reference's terms,
@@ -403,6 +422,7 @@ __device__ __forceinline__ void quad_ode(float* out, const float* x, const float
 // Quadcopter RK4: statePropagatorSpatialHash.cu:256-296.  The stage sum
 // k1 + 2 k2 + 2 k3 + k4 is accumulated as each stage finishes (the same
 // operations in the same order), so only two stage vectors are live.
+template<int GQ = kGridAny>
 __device__ __forceinline__ bool step_quad(const ModelParams& P, const GridView& G, float* h0, const float* u)
 {
     const float dt = P.dt, half = P.half, d6 = P.d6;
@@ -430,18 +450,18 @@ __device__ __forceinline__ bool step_quad(const ModelParams& P, const GridView& 
     for(int j = 0; j < 12; ++j) h0[j] = fma_(d6, acc[j] + kb[j], h0[j]);
     if(!inb(h0[6], P.vmin, P.vmax) || !inb(h0[7], P.vmin, P.vmax) || !inb(h0[8], P.vmin, P.vmax)) return false;
     const float p1[3] = {h0[0], h0[1], h0[2]};
-    return ws_and_collision(P, G, p0, p1);
+    return ws_and_collision<GQ>(P, G, p0, p1);
 }
 
-template<int MODEL>
+template<int MODEL, int GQ = kGridAny>
 __device__ __forceinline__ bool step(const ModelParams& P, const GridView& G, float* s, const float* u)
 {
     if constexpr(MODEL == KPAX_MODEL_DOUBLE_INTEGRATOR)
-        return step_di(P, G, s, u);
+        return step_di<GQ>(P, G, s, u);
     else if constexpr(MODEL == KPAX_MODEL_DUBINS_AIRPLANE)
-        return step_dubins(P, G, s, u);
+        return step_dubins<GQ>(P, G, s, u);
     else
-        return step_quad(P, G, s, u);
+        return step_quad<GQ>(P, G, s, u);
 }
 
 // A whole edge: `steps` steps, stopping at the first invalid one.  Returns the

@@ -155,7 +155,8 @@ struct kpax_handle {
                                         // after advance, expand, append, propose, score, frontier
     hipEvent_t init_ev[2]   = {nullptr, nullptr};
     // one captured iteration per status-buffer parity, replayed while profiling is off
-    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [LDS expansion][status parity]
+    bool expand_lds         = false;    // single-slot: the running query's environment has a staged LDS image
     bool graph_tried        = false;
     bool use_graph          = false;
     // learned proposal (kpax_set_proposal)

@@ -151,7 +151,6 @@ struct ExpandArgs {
     uint4* prep;            // per processing position: {child, parent, steps | slot << 17, admit draw}, {controls}
     uint32_t* tickets;      // kTicketGroups chunk counters, kTicketStride words apart
     uint32_t refill;        // idle lanes that trigger a wave's service pass (1..64)
-    uint32_t use_lds;       // single-slot form: stage the environment's grid image in LDS
     int32_t prio_q;         // issue-priority step quantum (a quarter of the longest edge); 0: priorities off
     uint32_t ordered;       // children in longest-first order per preparation segment (else child order)
     uint32_t order_blocks;  // work-groups of the preparation pass (its segments)
@@ -357,7 +356,9 @@ __device__ __forceinline__ void set_issue_priority(int left, int q)
 // environment's grid image in LDS (work-groups of 640 threads: two per CU
 // hold the five waves per SIMD and two images); the batched form's lanes
 // query the grids of several environments in global memory (256 threads).
-template<int MODEL, bool MULTI>
+// GQ: the grid query path (kGridLds: the single-slot form with its
+// environment's image staged in LDS; kGridCsr: the CSR grid in HBM).
+template<int MODEL, bool MULTI, int GQ>
 __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_eu(MULTI ? 4 : 5))) void k_expand(
   const ExpandArgs* __restrict__ A)
 {
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
         t    = A->s0.cur.t;
         base = A->s0.qd.base;
         G    = A->s0.G;
-        if(A->use_lds)
+        if constexpr(GQ == kGridLds)
             stage_lds_grid(G, s_grid);
         else
             G.lds = nullptr;
@@ -536,7 +537,7 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
         if(prio_q) set_issue_priority(live ? n - i : 0, prio_q);
         if(live)
         {
-            valid = step<MODEL>(P, G, x, u);
+            valid = step<MODEL, GQ>(P, G, x, u);
             ++i;
             if(!valid || i == n)
             {
@@ -1145,12 +1146,16 @@ size_t append_lds_bytes(const kpax_handle* h)
 
 // Work-groups of k_expand that fit on the device at once: its waves are
 // persistent, so more would only wait for the first ones to drain the queue.
-template<int MODEL, bool MULTI>
+// The single-slot expansion stages its environment's grid image in LDS when
+// the handle has one in use (the image's bytes are its dynamic LDS).
+bool lds_expand(const kpax_handle* h) { return h->Qmax == 1 && h->use_lds && h->lds_bytes > 0; }
+
+template<int MODEL, bool MULTI, int GQ>
 uint32_t expand_blocks_for(const kpax_handle* h)
 {
     int per_cu = 0;
-    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand<MODEL, MULTI>, (int)h->expand_threads,
-                                                    MULTI ? 0 : h->lds_bytes) != hipSuccess ||
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_expand<MODEL, MULTI, GQ>, (int)h->expand_threads,
+                                                    GQ == kGridLds ? h->lds_bytes : 0) != hipSuccess ||
        per_cu < 1)
         per_cu = 1;
     return (uint32_t)per_cu * (uint32_t)h->cu_count;
@@ -1159,7 +1164,8 @@ uint32_t expand_blocks_for(const kpax_handle* h)
 template<int MODEL>
 uint32_t expand_blocks_model(const kpax_handle* h)
 {
-    return h->Qmax == 1 ? expand_blocks_for<MODEL, false>(h) : expand_blocks_for<MODEL, true>(h);
+    if(h->Qmax != 1) return expand_blocks_for<MODEL, true, kGridCsr>(h);
+    return lds_expand(h) ? expand_blocks_for<MODEL, false, kGridLds>(h) : expand_blocks_for<MODEL, false, kGridCsr>(h);
 }
 
 uint32_t expand_blocks(const kpax_handle* h)
@@ -1206,7 +1212,6 @@ int upload_expand_args(kpax_handle* h)
     a.prep      = h->prep;
     a.tickets   = h->tickets;
     a.refill    = h->refill_lanes;
-    a.use_lds   = h->Qmax == 1 && h->use_lds && h->lds_bytes > 0 ? 1u : 0u;
     a.prio_q    = h->wave_priority ? std::max(1, (h->P.steps_min + h->P.span - 1 + 3) / 4) : 0;
     a.ordered   = h->order_children ? 1u : 0u;
     a.order_blocks = order_blocks(h);
@@ -1221,11 +1226,15 @@ template<int MODEL>
 void launch_expand_model(kpax_handle* h)
 {
     const ExpandArgs* A = (const ExpandArgs*)h->expand_args;
-    if(h->Qmax == 1)
-        hipLaunchKernelGGL((k_expand<MODEL, false>), dim3(h->expand_blocks), dim3(h->expand_threads),
-                           h->use_lds ? h->lds_bytes : 0u, h->stream, A);
+    if(h->Qmax != 1)
+        hipLaunchKernelGGL((k_expand<MODEL, true, kGridCsr>), dim3(h->expand_blocks), dim3(h->expand_threads), 0,
+                           h->stream, A);
+    else if(h->expand_lds)
+        hipLaunchKernelGGL((k_expand<MODEL, false, kGridLds>), dim3(h->expand_blocks), dim3(h->expand_threads),
+                           h->lds_bytes, h->stream, A);
     else
-        hipLaunchKernelGGL((k_expand<MODEL, true>), dim3(h->expand_blocks), dim3(h->expand_threads), 0, h->stream, A);
+        hipLaunchKernelGGL((k_expand<MODEL, false, kGridCsr>), dim3(h->expand_blocks), dim3(h->expand_threads), 0,
+                           h->stream, A);
 }
 
 
@@ -1351,32 +1360,40 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
     return KPAX_OK;
 }
 
-// Captures one iteration per status parity as a HIP graph.  A capture the
-// runtime refuses leaves the handle on direct launches of the same kernels.
+// Captures one iteration per status parity as a HIP graph, for each
+// expansion a single-slot handle may launch (with the LDS grid image, and
+// without it for environments that have none).  A capture the runtime
+// refuses leaves the handle on direct launches of the same kernels.
 void capture_graphs(kpax_handle* h)
 {
     h->graph_tried = true;
-    for(int p = 0; p < 2; ++p)
-    {
-        hipGraph_t g = nullptr;
-        if(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+    const bool keep = h->expand_lds;
+    for(int v = 0; v < (lds_expand(h) ? 2 : 1); ++v)
+        for(int p = 0; p < 2; ++p)
         {
-            (void)hipGetLastError();
-            h->use_graph = false;
-            return;
+            h->expand_lds = v == 1;
+            hipGraph_t g  = nullptr;
+            if(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                h->use_graph  = false;
+                h->expand_lds = keep;
+                return;
+            }
+            const int rc       = enqueue_iteration(h, p, false);
+            const hipError_t e = hipStreamEndCapture(h->stream, &g);
+            if(rc != KPAX_OK || e != hipSuccess || !g ||
+               hipGraphInstantiate(&h->graph_exec[v][p], g, nullptr, nullptr, 0) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                if(g) (void)hipGraphDestroy(g);
+                h->use_graph  = false;
+                h->expand_lds = keep;
+                return;
+            }
+            (void)hipGraphDestroy(g);
         }
-        const int rc       = enqueue_iteration(h, p, false);
-        const hipError_t e = hipStreamEndCapture(h->stream, &g);
-        if(rc != KPAX_OK || e != hipSuccess || !g ||
-           hipGraphInstantiate(&h->graph_exec[p], g, nullptr, nullptr, 0) != hipSuccess)
-        {
-            (void)hipGetLastError();
-            if(g) (void)hipGraphDestroy(g);
-            h->use_graph = false;
-            return;
-        }
-        (void)hipGraphDestroy(g);
-    }
+    h->expand_lds = keep;
 }
 
 // First-fit allocator of node ranges in [0, T).
@@ -1505,6 +1522,9 @@ int run_queries_impl(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result*
             d.query = 0;  // the same draws as a plan of this query alone
             d.seed  = qs[qi].seed;
             d.mix   = d.cap >= h->prop_min_budget ? h->mix : 0.0f;  // proposal policy by tree budget
+            // a single-slot handle expands with the LDS grid when this
+            // query's environment has an image (else the CSR grid)
+            if(Q == 1) h->expand_lds = lds_expand(h) && h->envs[d.env].has_lds;
             for(int j = 0; j < 3; ++j) d.goal[j] = qs[qi].goal[j];
             h->h_init_slots[n_init] = s;
             for(int j = 0; j < 16; ++j) h->h_init_starts[16 * (size_t)n_init + j] = j < S ? qs[qi].start[j] : 0.0f;
@@ -1625,7 +1645,7 @@ int run_queries_impl(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result*
         const int p = (int)(it & 1);
         if(graph)
         {
-            HIPC(hipGraphLaunch(h->graph_exec[p], h->stream));
+            HIPC(hipGraphLaunch(h->graph_exec[h->expand_lds ? 1 : 0][p], h->stream));
             h->stats.graph_launches += 1;
         }
         else if((rc = enqueue_iteration(h, p, prof)) != KPAX_OK)
@@ -1742,12 +1762,13 @@ int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
 
 void drop_graphs(kpax_handle* h)
 {
-    for(auto& e : h->graph_exec)
-        if(e)
-        {
-            (void)hipGraphExecDestroy(e);
-            e = nullptr;
-        }
+    for(auto& ev : h->graph_exec)
+        for(auto& e : ev)
+            if(e)
+            {
+                (void)hipGraphExecDestroy(e);
+                e = nullptr;
+            }
     h->graph_tried = false;
 }
 
