@@ -1952,6 +1952,12 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     // per plan at 64, 2.54 at 32 and 2.59 at 16; the batch and the Dubins
     // maze ran best at 64.
     h->expand_chunk  = max_queries == 1 && cfg->model == KPAX_MODEL_QUADCOPTER ? 32u : 64u;
+    // With 32-child chunks a wave's lanes idle sooner after a refill, and a
+    // service pass at 8 idle lanes measured best for single quadcopter plans
+    // (expansion 2.39 against 2.43 ms per plan at 16); the Dubins maze (41.8
+    // against 42.7 ms) and the serving batch (2,754-2,770 against 2,661-2,719
+    // plans/s) stay at 16, the double integrator is the same at either.
+    h->refill_lanes  = max_queries == 1 && cfg->model == KPAX_MODEL_QUADCOPTER ? 8u : 16u;
     const uint32_t Q = h->Qmax;
     const size_t T   = h->T;
     // tiles of 4096 items run as 1024-thread blocks, two resident per CU
