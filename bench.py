@@ -58,11 +58,11 @@ def bytes_per_expansion(S, C):
 # HBM bytes per expansion that k_expand<quadcopter> moves on this workload,
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (average per
 # dispatch over the expansions per dispatch; FETCH scaled by 2, WRITE by 1,
-# the factors that make a 1 GiB device copy read and write 1 GiB): 158 B read
+# the factors that make a 1 GiB device copy read and write 1 GiB): 175 B read
 # (the child's 32-byte prepared record; parents read in longest-edge-first
 # order, a cache line per state word; each work-group's 22 KB grid image) and
-# 95 B written (the 80-byte child and its share of partial lines).
-MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 158 + 95
+# 96 B written (the 80-byte child and its share of partial lines).
+MEASURED_EXPAND_TRAFFIC_PER_EXPANSION = 175 + 96
 
 
 # Algorithmic FP32 operations per integration step: the reference's step
