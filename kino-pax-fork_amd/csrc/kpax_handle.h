@@ -130,6 +130,7 @@ struct kpax_handle {
     int32_t* grid_scratch   = nullptr;  // counts, cursor, look-back words and ticket of the grid build
     bool order_children     = true;     // hand out children longest drawn edge first (default: single-query handles)
     uint32_t* tickets       = nullptr;  // kTicketGroups expansion chunk counters
+    void* slot_run          = nullptr;  // Qmax 32-byte records: what the batched refill reads of a slot
     uint32_t refill_lanes   = 16;       // idle lanes that trigger a wave's service pass
     bool wave_priority      = false;    // expansion waves set their issue priority by their longest live edge
     uint32_t scan_tile      = 4096;     // items per append/frontier tile: 1024, 2048 or 4096 (4 per thread)

@@ -132,11 +132,24 @@ struct SlotUniform {
     GridView G;
 };
 
+// What the batched expansion's refill needs of a running slot, gathered by
+// the advance into one 32-byte record: the iteration, the node range's base
+// and the environment's CSR grid, one load instead of the chain slot ->
+// query -> environment.
+struct SlotRun {
+    uint32_t t, base;
+    const int32_t* start;
+    const float4* box;
+    uint32_t pad[2];
+};
+static_assert(sizeof(SlotRun) == 32, "a slot's run record is two 16-byte words");
+
 struct ExpandArgs {
     Ctl* ctl;
     const QueryDev* qd;
     BatchCtl* bc;
     const uint32_t* child_off;
+    const SlotRun* run;  // per slot, written by the advance
     const GridView* envs;
     TreeView tree;
     ChildView ch;
@@ -178,6 +191,7 @@ struct AdvanceArgs {
     SlotUniform* s0;
     uint32_t tile;
     uint32_t* tickets;
+    SlotRun* run;
 };
 
 // The advance, by one block of 1024 threads (see k_advance).
@@ -212,6 +226,16 @@ __device__ __forceinline__ void advance_block(const AdvanceArgs& V)
         }
         if(n.status == kRunning)
         {
+            if(V.run)
+            {
+                const QueryDev& d = qd[q];
+                SlotRun r{};
+                r.t     = n.t;
+                r.base  = d.base;
+                r.start = envs[d.env].start;
+                r.box   = envs[d.env].box;
+                V.run[q] = r;
+            }
             c->expansions += n.M;
             c->Fn       = 0;
             c->f_accept = f_accept(n.t, n.ts, qd[q].cap, PP.eps);  // kept when no child is admitted
@@ -498,27 +522,25 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
                     const uint32_t rank  = (uint32_t)__popcll(need & below);
                     if(c == kNone && rank < avail)
                     {
-                        // the child's draws and parent, prepared by k_prepare
+                        // the child's draws and parent (its node index in the
+                        // pool: the state loads need no slot record), prepared
+                        // by k_prepare
                         const uint4* pr = A->prep + 2 * (size_t)(q_next + rank);
                         const uint4 r0 = pr[0], r1 = pr[1];
-                        uint32_t ci    = r0.x;
-                        if constexpr(MULTI)
-                        {
-                            const uint32_t s = r0.z >> kPrepSlotShift;
-                            ci               = r0.x - A->child_off[s];
-                            t                = A->ctl[s].cur.t;
-                            const QueryDev& d = A->qd[s];
-                            base             = d.base;
-                            const GridView& g = A->envs[d.env];
-                            G.start          = g.start;
-                            G.box            = g.box;
-                            slot             = s;
-                        }
-                        c                 = ci;
-                        parent            = r0.y;
-                        const uint32_t pn = base + parent;
+                        const uint32_t pn = r0.y;
 #pragma unroll
                         for(int d = 0; d < S; ++d) x[d] = tree.st[(size_t)d * T + pn];
+                        if constexpr(MULTI)
+                        {
+                            slot            = r0.z >> kPrepSlotShift;
+                            const SlotRun r = A->run[slot];
+                            t               = r.t;
+                            base            = r.base;
+                            G.start         = r.start;
+                            G.box           = r.box;
+                        }
+                        c                 = r0.x;  // index within the slot's children
+                        parent            = pn - base;
                         const uint32_t uw[4] = {r1.x, r1.y, r1.z, r1.w};
 #pragma unroll
                         for(int j = 0; j < C; ++j) u[j] = __builtin_bit_cast(float, uw[j]);
@@ -674,7 +696,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_prepare(const ExpandArgs* __r
 #pragma unroll
         for(int j = 0; j < C; ++j) uw[j] = __builtin_bit_cast(uint32_t, u[j]);
         uint4* pr = A->prep + 2 * (size_t)p;
-        pr[0]     = make_uint4(k, parent, (uint32_t)n | (slot << kPrepSlotShift), ds.y);
+        pr[0]     = make_uint4(c, base + parent, (uint32_t)n | (slot << kPrepSlotShift), ds.y);
         pr[1]     = make_uint4(uw[0], uw[1], uw[2], uw[3]);
     }
 }
@@ -1196,6 +1218,7 @@ int upload_expand_args(kpax_handle* h)
     a.qd        = h->qd;
     a.bc        = h->bc;
     a.child_off = h->child_off;
+    a.run       = (const SlotRun*)h->slot_run;
     a.envs      = h->envs_dev;
     a.tree      = h->tree;
     a.ch        = h->ch;
@@ -1257,6 +1280,7 @@ AdvanceArgs advance_args(const kpax_handle* h)
     V.s0          = &((ExpandArgs*)h->expand_args)->s0;
     V.tile        = h->scan_tile;
     V.tickets     = h->tickets;
+    V.run         = h->Qmax > 1 ? (SlotRun*)h->slot_run : nullptr;  // only the batched expansion reads it
     return V;
 }
 
@@ -1980,6 +2004,11 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->qd, Q));
     TRY(dalloc(h, &h->bc, 1));
     TRY(dalloc(h, &h->tickets, (size_t)kTicketGroups * kTicketStride));
+    {
+        SlotRun* run = nullptr;
+        TRY(dalloc(h, &run, (size_t)Q));
+        h->slot_run = run;
+    }
     if(hipMemset(h->tickets, 0, (size_t)kTicketGroups * kTicketStride * 4) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
     TRY(dalloc(h, &h->child_off, Q + 1));
     TRY(dalloc(h, &h->atile_off, Q + 1));
