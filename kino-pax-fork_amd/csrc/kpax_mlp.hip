@@ -113,6 +113,28 @@ __global__ __launch_bounds__(256) void k_pack_weights(const float* w, MlpDev* d)
         d->b2[i] = w[kMlpB2 + i];
     }
     for(int i = t; i < 32; i += stride) d->b3[i] = i < kMlpOut ? w[kMlpB3 + i] : 0.0f;
+    // inference fragments: element e of lane l = j + 16 g of fragment [tile][step]
+    uint16_t* f1 = (uint16_t*)d->frag.w1;
+    for(int i = t; i < kFragT * kFragS1 * 64 * 8; i += stride)
+    {
+        const int e = i & 7, l = (i >> 3) & 63, st = i >> 9;
+        const int n = 16 * (st / kFragS1) + (l & 15), k = 32 * (st % kFragS1) + 8 * (l >> 4) + e;
+        f1[i] = k < kMlpIn ? f2bf(w[kMlpW1 + k * kMlpHidden + n]) : (uint16_t)0;
+    }
+    uint16_t* f2 = (uint16_t*)d->frag.w2;
+    for(int i = t; i < kFragT * kFragS2 * 64 * 8; i += stride)
+    {
+        const int e = i & 7, l = (i >> 3) & 63, st = i >> 9;
+        const int n = 16 * (st / kFragS2) + (l & 15), k = frag_in_hidden(st % kFragS2, l >> 4, e);
+        f2[i] = f2bf(w[kMlpW2 + k * kMlpHidden + n]);
+    }
+    uint16_t* f3 = (uint16_t*)d->frag.w3;
+    for(int i = t; i < kFragS2 * 64 * 8; i += stride)
+    {
+        const int e = i & 7, l = (i >> 3) & 63, s = i >> 9;
+        const int n = l & 15, k = frag_in_hidden(s, l >> 4, e);
+        f3[i] = n < kMlpOut ? f2bf(w[kMlpW3 + k * kMlpOut + n]) : (uint16_t)0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -230,34 +252,33 @@ __device__ __forceinline__ void wave_tiles(char* lds, uint16_t (**xa)[kLdH], uin
     *xb = (uint16_t (*)[kLdH])(lds + kMlpLdsWeights + (2 * w + 1) * kMlpTileBytes);
 }
 
-// Raw network outputs of n given feature rows (fp32 [n][80]) -> out [n][8].
-__global__ __launch_bounds__(256) void k_mlp_infer(const MlpDev* Wd, const float* X, uint32_t n, float* out)
+// Raw network outputs of n given feature rows (fp32 [n][80]) -> out [n][8],
+// 16 rows per wave through the same inference code as the planner's pass.
+__global__ __launch_bounds__(256) void k_mlp_infer(const MlpDev* W, const float* X, uint32_t n, float* out)
 {
-    HIP_DYNAMIC_SHARED(char, lds)
-    load_weights_lds(Wd, lds);
-    const MlpDev& W = *(const MlpDev*)lds;
-    uint16_t(*xa)[kLdH];
-    uint16_t(*xb)[kLdH];
-    wave_tiles(lds, &xa, &xb);
-    const int lane = threadIdx.x & 63, r = lane & 31;
-    const uint32_t ntiles = (n + 31) / 32;
+    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const uint32_t ntiles = (n + 15) / 16;
     for(uint32_t tile = blockIdx.x * kMlpWaves + (threadIdx.x >> 6); tile < ntiles; tile += gridDim.x * kMlpWaves)
     {
-        const uint32_t row0 = tile * 32;
-        for(int k = lane; k < 32 * kMlpIn; k += 64)
+        const uint32_t row = tile * 16 + j, rr = row < n ? row : n - 1;
+        uint4 xb[kFragS1];
+#pragma unroll
+        for(int s = 0; s < kFragS1; ++s)
         {
-            const int rr = k / kMlpIn, cc = k % kMlpIn;
-            xa[rr][cc] = row0 + rr < n ? f2bf(X[(size_t)(row0 + rr) * kMlpIn + cc]) : (uint16_t)0;
-        }
-        wave_lds_sync();
-        float o[16];
-        mlp_tile<false>(W, xa, xb, o, Keep{});
-        if(r < kMlpOut)
-            for(int i = 0; i < 16; ++i)
+            uint16_t v[8];
+#pragma unroll
+            for(int e = 0; e < 8; ++e)
             {
-                const uint32_t row = row0 + mfma_row(i, lane);
-                if(row < n) out[(size_t)row * kMlpOut + r] = o[i];
+                const int k = 32 * s + 8 * g + e;
+                v[e] = k < kMlpIn ? f2bf(X[(size_t)rr * kMlpIn + k]) : (uint16_t)0;
             }
+            xb[s] = make_uint4(pack_bf16x2(v[0], v[1]), pack_bf16x2(v[2], v[3]), pack_bf16x2(v[4], v[5]),
+                               pack_bf16x2(v[6], v[7]));
+        }
+        const f32x4 o = mlp_forward16(W, xb);
+        if(g < 2 && row < n)
+#pragma unroll
+            for(int e = 0; e < 4; ++e) out[(size_t)row * kMlpOut + 4 * g + e] = o[e];
     }
 }
 
@@ -307,22 +328,15 @@ struct ProposeArgs {
     uint32_t n_init;
 };
 
-// Per-slot table of a proposal pass, in LDS after the network's buffers:
-// tile offsets (Q + 1), first node and node count of each slot.
-constexpr size_t kProposeLdsBytes = kMlpLdsBytes + (3 * (size_t)kMaxSlots + 1) * sizeof(uint32_t);
-static_assert(kProposeLdsBytes <= 160 * 1024, "proposal pass exceeds the LDS of a CU");
-
 // Gaussian of every node appended this iteration (or of every root being
-// initialised, n_init > 0): each block first lays out the slots' 32-row tiles
-// from the counts the append kernel left (ctl[q].Fn), so a block with no tile
-// exits before loading the weights and the tile loop reads only LDS.
-__global__ __launch_bounds__(256) void k_propose(const MlpDev* Wd, ProposeArgs A)
+// initialised, n_init > 0), 16 nodes per wave: each block first lays out the
+// slots' 16-node tiles from the counts the append kernel left (ctl[q].Fn), so a
+// block with no tile exits at once and the tile loop reads only LDS for it.
+// The weights are read as operand fragments straight from global memory (L2).
+__global__ __launch_bounds__(256) void k_propose(const MlpDev* W, ProposeArgs A)
 {
-    HIP_DYNAMIC_SHARED(char, lds)
     __shared__ uint32_t s_scan[4];
-    uint32_t* s_off  = (uint32_t*)(lds + kMlpLdsBytes);
-    uint32_t* s_node = s_off + kMaxSlots + 1;
-    uint32_t* s_nv   = s_node + kMaxSlots;
+    __shared__ uint32_t s_off[kMaxSlots + 1], s_node[kMaxSlots], s_nv[kMaxSlots];
     const uint32_t n = A.n_init ? A.n_init : A.Q;  // table entries (n <= kMaxSlots = 4 x 256)
     uint32_t cnt[4], mine = 0;
 #pragma unroll
@@ -346,7 +360,7 @@ __global__ __launch_bounds__(256) void k_propose(const MlpDev* Wd, ProposeArgs A
             s_node[k] = node0;
             s_nv[k]   = nv;
         }
-        cnt[j] = (nv + 31) / 32;
+        cnt[j] = (nv + 15) / 16;
         mine += cnt[j];
     }
     uint32_t ntiles;
@@ -361,45 +375,30 @@ __global__ __launch_bounds__(256) void k_propose(const MlpDev* Wd, ProposeArgs A
     if(threadIdx.x == 0) s_off[n] = ntiles;
     __syncthreads();
     if(blockIdx.x * kMlpWaves >= ntiles) return;  // the same for every thread of the block
-    load_weights_lds(Wd, lds);
-    const MlpDev& W = *(const MlpDev*)lds;
-    uint16_t(*xa)[kLdH];
-    uint16_t(*xb)[kLdH];
-    wave_tiles(lds, &xa, &xb);
-    const int lane = threadIdx.x & 63, r = lane & 31;
+    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
     for(uint32_t tile = blockIdx.x * kMlpWaves + (threadIdx.x >> 6); tile < ntiles; tile += gridDim.x * kMlpWaves)
     {
         const uint32_t k     = segment_of(s_off, n, tile);
         const uint32_t lt    = tile - s_off[k];
-        const uint32_t left  = s_nv[k] - lt * 32;
-        const uint32_t nv    = left < 32 ? left : 32u;
-        const uint32_t node0 = s_node[k] + lt * 32;
+        const uint32_t left  = s_nv[k] - lt * 16;
+        const uint32_t nv    = left < 16 ? left : 16u;
+        const uint32_t node0 = s_node[k] + lt * 16;
         const uint32_t q     = A.n_init ? A.init_slots[k] : k;
-        if(lane < 32)
-        {
-            float f[kMlpIn];
-            if((uint32_t)r < nv)
+        const uint32_t node  = node0 + ((uint32_t)j < nv ? (uint32_t)j : nv - 1);  // rows past the last repeat it
+        float x[12];
+#pragma unroll
+        for(int d = 0; d < 12; ++d) x[d] = d < A.F.S ? A.tree_st[(size_t)d * A.T + node] : 0.0f;
+        const QueryDev& d = A.qd[q];
+        uint4 xb[kFragS1];
+        features16(A.F, A.envs[d.env].occ, x, d.goal, xb);
+        const f32x4 o = mlp_forward16(W, xb);
+        if(g < 2 && (uint32_t)j < nv)
+#pragma unroll
+            for(int e = 0; e < 4; ++e)
             {
-                float x[12];
-                for(int d = 0; d < 12; ++d) x[d] = d < A.F.S ? A.tree_st[(size_t)d * A.T + node0 + r] : 0.0f;
-                const QueryDev& d = A.qd[q];
-                features(A.F, A.envs[d.env].occ, x, d.goal, f);
-            }
-            else
-                for(int kk = 0; kk < kMlpIn; ++kk) f[kk] = 0.0f;
-            for(int kk = 0; kk < kMlpIn; ++kk) xa[r][kk] = f2bf(f[kk]);
-        }
-        wave_lds_sync();
-        float o[16];
-        mlp_tile<false>(W, xa, xb, o, Keep{});
-        if(r < kMlpOut)
-            for(int i = 0; i < 16; ++i)
-            {
-                const int row = mfma_row(i, lane);
-                if((uint32_t)row >= nv) continue;
-                float v = o[i];
-                if(r >= 4) v = expf(v < -4.0f ? -4.0f : (v > 0.5f ? 0.5f : v));
-                A.prop[(size_t)r * A.T + node0 + row] = v;
+                float v = o[e];
+                if(g == 1) v = expf(v < -4.0f ? -4.0f : (v > 0.5f ? 0.5f : v));
+                A.prop[(size_t)(4 * g + e) * A.T + node0 + j] = v;
             }
     }
 }
@@ -733,8 +732,8 @@ int launch_propose(kpax_handle* h, uint32_t n_init)
     A.init_slots = h->d_init_slots;
     A.n_init     = n_init;
     const uint32_t blocks = n_init ? std::max<uint32_t>(1, (n_init + kMlpWaves - 1) / kMlpWaves)
-                                   : (uint32_t)h->cu_count;
-    hipLaunchKernelGGL(k_propose, dim3(blocks), dim3(256), kProposeLdsBytes, h->stream, (const MlpDev*)h->mlp, A);
+                                   : 2 * (uint32_t)h->cu_count;
+    hipLaunchKernelGGL(k_propose, dim3(blocks), dim3(256), 0, h->stream, (const MlpDev*)h->mlp, A);
     HIPC(hipGetLastError());
     return KPAX_OK;
 }
@@ -922,9 +921,9 @@ int kpax_mlp_infer(kpax_handle* h, const float* params, const float* features, u
     HIPC(hipMemcpy(w, params, kMlpParams * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(x, features, (size_t)kMlpIn * n * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_pack_weights, dim3(64), dim3(256), 0, h->stream, w, d);
-    const uint32_t tiles = (n + 31) / 32;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(std::min<uint32_t>((tiles + kMlpWaves - 1) / kMlpWaves, h->cu_count)),
-                       dim3(256), kMlpLdsBytes, h->stream, d, x, n, o);
+    const uint32_t tiles = (n + 15) / 16;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(std::min<uint32_t>((tiles + kMlpWaves - 1) / kMlpWaves, 2 * h->cu_count)),
+                       dim3(256), 0, h->stream, d, x, n, o);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     HIPC(hipMemcpy(out, o, (size_t)kMlpOut * n * 4, hipMemcpyDeviceToHost));

@@ -175,6 +175,12 @@ struct f32x16 {
     float operator[](int i) const { return v[i]; }
 };
 
+struct f32x4 {
+    float v[4];
+    float& operator[](int i) { return v[i]; }
+    float operator[](int i) const { return v[i]; }
+};
+
 inline void emu_wave_barrier() { emu_g->wave_bar[threadIdx.x / 64]->arrive_and_wait(); }
 
 inline float emu_bf16(uint32_t word, unsigned half)
@@ -207,6 +213,34 @@ inline f32x16 mfma_32x32x16(uint4 a, uint4 b, f32x16 c)
         for(unsigned k = 0; k < 16; ++k)
         {
             const unsigned la = row + 32 * (k / 8), lb = col + 32 * (k / 8), e = k % 8;
+            s += emu_bf16(buf[la * 8 + e / 2], e % 2) * emu_bf16(buf[lb * 8 + 4 + e / 2], e % 2);
+        }
+        d[i] = s;
+    }
+    bar.arrive_and_wait();
+    return d;
+}
+
+// D(16x16) = A(16x32) B(32x16) + C: lane l = j + 16 g supplies A row j and
+// B column j for the k-slots (g, 0..7) and receives column j, rows 4 g + e.
+inline f32x4 mfma_16x16x32(uint4 a, uint4 b, f32x4 c)
+{
+    const unsigned t = threadIdx.x, w = t / 64, lane = t % 64;
+    auto& bar        = *emu_g->wave_bar[w];
+    uint32_t* buf    = emu_g->mbuf.data() + (size_t)w * 64 * 8;
+    bar.arrive_and_wait();
+    const uint32_t in[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    for(int j = 0; j < 8; ++j) buf[lane * 8 + j] = in[j];
+    bar.arrive_and_wait();
+    f32x4 d;
+    const unsigned col = lane % 16;
+    for(int i = 0; i < 4; ++i)
+    {
+        const unsigned row = 4 * (lane / 16) + i;
+        float s            = c[i];
+        for(unsigned k = 0; k < 32; ++k)
+        {
+            const unsigned la = row + 16 * (k / 8), lb = col + 16 * (k / 8), e = k % 8;
             s += emu_bf16(buf[la * 8 + e / 2], e % 2) * emu_bf16(buf[lb * 8 + 4 + e / 2], e % 2);
         }
         d[i] = s;

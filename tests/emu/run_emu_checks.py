@@ -165,6 +165,13 @@ def mlp_checks():
         p.set_proposal(params, 0.5)
         r1 = p.plan(start, goal, seed=9)
         r2 = p.plan(start, goal, seed=9)
+        tree, table = p.tree(), p.proposal_table()
+        used = np.unique(tree["parents"][1:])  # nodes the plan expanded: their Gaussians were read
+        gg = np.tile(np.asarray(goal[:3], np.float32).reshape(3, 1), (1, used.size))
+        raw = p.mlp_infer(params, p.mlp_features(0, tree["states"][:, used], gg)).T
+        check(np.array_equal(table[:4, used].view(np.uint32), raw[:4].view(np.uint32))
+              and np.allclose(table[4:, used], np.exp(np.clip(raw[4:], -4.0, 0.5)), rtol=1e-6, atol=0),
+              f"proposal pass equals the network on the features of {used.size} expanded nodes")
         p.set_proposal(None)
         r0 = p.plan(start, goal, seed=9)
     o = oracle.plan(cfg, obs, start, goal, seed=9, threads=4)

@@ -216,6 +216,28 @@ def test_proposal_plan_matches_oracle_with_injected_gaussians(kpax, oracle, budg
     assert (plain["tree_size"], plain["expansions"]) != (o["tree_size"], o["expansions"])
 
 
+def test_proposal_pass_is_the_network_on_each_node(kpax):
+    """The planner's proposal pass (features in registers, network on the
+    matrix cores, 16 nodes per wave) gives every expanded node of a plan the
+    Gaussian that the feature operator and the inference operator give for that
+    node's state: means bit for bit, sigmas = exp(clamp(log sigma, -4, 1/2))
+    to float rounding."""
+    cfg, obs, start, goal = workload(2, seed=0, max_tree=1 << 17)
+    params = np.load(PARAMS)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        p.set_proposal(params, 0.5)
+        r = p.plan(start, goal, seed=23)
+        tree, table = p.tree(), p.proposal_table()
+        used = np.unique(tree["parents"][1:])  # every node the plan expanded
+        gg = np.tile(np.asarray(goal[:3], np.float32).reshape(3, 1), (1, used.size))
+        raw = p.mlp_infer(params, p.mlp_features(0, tree["states"][:, used], gg)).T
+    print(f"plan {r.status} it {r.iterations} tree {r.tree_size}: {used.size} expanded nodes", flush=True)
+    assert used.size > 10000
+    assert np.array_equal(table[:4, used].view(np.uint32), raw[:4].view(np.uint32))
+    assert np.allclose(table[4:, used], np.exp(np.clip(raw[4:], -4.0, 0.5)), rtol=1e-6, atol=0)
+
+
 def test_trained_proposal_gains_over_uniform_sampling(kpax):
     """The shipped network (trained on plain-planner paths in other forests),
     mixed 50/50 with uniform sampling, on 96 held-out queries in 8 held-out
