@@ -337,28 +337,44 @@ __global__ __launch_bounds__(256) void k_propose(const MlpDev* W, ProposeArgs A)
 {
     __shared__ uint32_t s_scan[4];
     __shared__ uint32_t s_off[kMaxSlots + 1], s_node[kMaxSlots], s_nv[kMaxSlots];
-    const uint32_t n = A.n_init ? A.n_init : A.Q;  // table entries (n <= kMaxSlots = 4 x 256)
+    __shared__ const uint32_t* s_occ[kMaxSlots];
+    __shared__ float s_goal[kMaxSlots][3];
+    const uint32_t n = A.n_init ? A.n_init : A.Q;  // table entries (1 <= n <= kMaxSlots = 4 x 256)
     uint32_t cnt[4], mine = 0;
 #pragma unroll
     for(int j = 0; j < 4; ++j)
     {
-        const uint32_t k = threadIdx.x * 4 + j;
-        uint32_t nv = 0, node0 = 0;
+        // every entry's loads go out together (indices clamped), then the selects
+        const uint32_t k  = threadIdx.x * 4 + j;
+        const uint32_t kc = k < n ? k : 0u;
+        const uint32_t q  = A.n_init ? A.init_slots[kc] : kc;
+        const QueryDev& d = A.qd[q];
+        const float mix = d.mix;
+        const uint32_t base = d.base, env = d.env;
+        const float g0 = d.goal[0], g1 = d.goal[1], g2 = d.goal[2];
+        uint32_t nv, node0;
+        if(A.n_init)
+        {
+            nv    = mix > 0.0f ? 1u : 0u;  // queries under the budget policy sample uniformly
+            node0 = base;
+        }
+        else
+        {
+            const Ctl& c = A.ctl[kc];
+            const int32_t st = c.cur.status;
+            const uint32_t fn = c.Fn, ts = c.cur.ts;
+            nv    = st == kRunning && mix > 0.0f ? fn : 0u;
+            node0 = base + ts;
+        }
+        if(k >= n) nv = 0;
         if(k < n)
         {
-            if(A.n_init)
-            {
-                const QueryDev& d = A.qd[A.init_slots[k]];
-                nv                = d.mix > 0.0f ? 1u : 0u;  // queries under the budget policy sample uniformly
-                node0             = d.base;
-            }
-            else if(A.ctl[k].cur.status == kRunning && A.qd[k].mix > 0.0f)
-            {
-                nv    = A.ctl[k].Fn;
-                node0 = A.qd[k].base + A.ctl[k].cur.ts;
-            }
-            s_node[k] = node0;
-            s_nv[k]   = nv;
+            s_node[k]    = node0;
+            s_nv[k]      = nv;
+            s_occ[k]     = nv ? A.envs[env].occ : nullptr;
+            s_goal[k][0] = g0;
+            s_goal[k][1] = g1;
+            s_goal[k][2] = g2;
         }
         cnt[j] = (nv + 15) / 16;
         mine += cnt[j];
@@ -383,14 +399,13 @@ __global__ __launch_bounds__(256) void k_propose(const MlpDev* W, ProposeArgs A)
         const uint32_t left  = s_nv[k] - lt * 16;
         const uint32_t nv    = left < 16 ? left : 16u;
         const uint32_t node0 = s_node[k] + lt * 16;
-        const uint32_t q     = A.n_init ? A.init_slots[k] : k;
         const uint32_t node  = node0 + ((uint32_t)j < nv ? (uint32_t)j : nv - 1);  // rows past the last repeat it
-        float x[12];
+        float x[12];  // the proposal is defined for the quadcopter: 12 state values
 #pragma unroll
-        for(int d = 0; d < 12; ++d) x[d] = d < A.F.S ? A.tree_st[(size_t)d * A.T + node] : 0.0f;
-        const QueryDev& d = A.qd[q];
+        for(int d = 0; d < 12; ++d) x[d] = A.tree_st[(size_t)d * A.T + node];
+        const float goal[3] = {s_goal[k][0], s_goal[k][1], s_goal[k][2]};
         uint4 xb[kFragS1];
-        features16(A.F, A.envs[d.env].occ, x, d.goal, xb);
+        features16(A.F, s_occ[k], x, goal, xb);
         const f32x4 o = mlp_forward16(W, xb);
         if(g < 2 && (uint32_t)j < nv)
 #pragma unroll
