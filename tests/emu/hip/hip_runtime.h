@@ -132,8 +132,6 @@ inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 
 // a wave's issue priority only orders instruction issue on the GPU
 inline void __builtin_amdgcn_s_setprio(int) {}
-// instruction-scheduling fences have no meaning on the host
-inline void __builtin_amdgcn_sched_barrier(int) {}
 
 inline uint64_t __ballot(int pred)
 {
