@@ -248,8 +248,9 @@ def learned_proposal(cfg):
     params = np.load(PARAMS)
     strip = lambda d: {k: v for k, v in d.items() if k != "results"}  # noqa: E731
     out = {"workload": "96 held-out queries (start/goal >= 4 m apart) over 8 held-out forests of 100-600 columns, "
-                       "max_iter 100, single-query form; parameters trained by train_proposal.py on 10,000 "
-                       "plain-planner runs in other forests; per budget, then budgets log-uniform in "
+                       "max_iter 100, single-query form; parameters trained by train_proposal.py on the 2.0M "
+                       "path edges of 180,000 plain-planner runs in other forests; per budget, then budgets "
+                       "log-uniform in "
                        "[2^15, 2^19] with and without the budget policy (uniform sampling below 2^17)"}
     for budget in (1 << 18, 1 << 17, 1 << 16):
         c = cfg.copy()

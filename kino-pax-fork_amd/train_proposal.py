@@ -1,14 +1,16 @@
 #!/usr/bin/env python3
 """Train the learned control proposal for the quadcopter forest workload.
 
-    python kino-pax-fork_amd/train_proposal.py [--runs 10000] [--epochs 40] [--out data/proposal_forest.npy]
+    python kino-pax-fork_amd/train_proposal.py [--runs 180000] [--epochs 20] [--batch 4096] [--lr 1e-3]
+                                               [--out data/proposal_forest.npy]
 
 1. Draws `runs` start/goal pairs (at least 4 m apart) in 32 seeded forests of
    100-600 columns over [0,10]^2 x [0,5].
 2. Solves them with the plain planner (uniform controls), 256 queries per
    batched call, tree budget 2^18, 100 iterations.
 3. Takes every edge of every returned path as a training example: features of
-   the parent node and the edge's control, normalised to [-1, 1].
+   the parent node and the edge's control, normalised to [-1, 1].  The
+   default 180,000 runs give about 2.0 million edges (11 per path).
 4. Fits the network with the GPU trainer (Adam, Gaussian negative
    log-likelihood) and saves the fp32 parameters (flat layout of
    include/kpax.h) with numpy.
@@ -33,11 +35,11 @@ import proposal  # noqa: E402
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--runs", type=int, default=10000)
+    ap.add_argument("--runs", type=int, default=180000)
     ap.add_argument("--envs", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=40)
-    ap.add_argument("--batch", type=int, default=1024)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=os.path.join(HERE, "data", "proposal_forest.npy"))
     args = ap.parse_args()
 
@@ -52,7 +54,8 @@ def main():
             p.set_environment(e, o)
         for b in range(0, len(qs), 256):
             res += p.plan_batch(qs[b:b + 256])
-            print(f"planned {len(res)}/{len(qs)} ({time.perf_counter() - t0:.1f} s)", flush=True)
+            if (b // 256) % 40 == 0 or len(res) == len(qs):
+                print(f"planned {len(res)}/{len(qs)} ({time.perf_counter() - t0:.1f} s)", flush=True)
         X, Y = proposal.dataset(p, cfg, qs, res)
     t_data = time.perf_counter() - t0
     print(f"{X.shape[0]} edges from {sum(r.status == kpax.OK for r in res)} solved runs", flush=True)
