@@ -165,6 +165,7 @@ struct kpax_handle {
     float* prop             = nullptr;  // per node: 4 means then 4 sigmas, column stride T
     float mix               = 0.0f;
     uint32_t prop_min_budget = 0;       // queries with a smaller tree budget sample uniformly
+    uint32_t prop_slots     = 1;        // admitted slots that sample from the proposal (0: skip its pass)
     // environments
     std::vector<kpax::Env> envs;
     // last single-query plan (for kpax_get_tree / kpax_get_regions)

@@ -1367,7 +1367,7 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
         default: launch_append<1024>(h); break;
     }
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][3], h->stream));
-    if(h->mix > 0.0f && (rc = launch_propose(h, 0)) != KPAX_OK) return rc;
+    if(h->mix > 0.0f && h->prop_slots > 0 && (rc = launch_propose(h, 0)) != KPAX_OK) return rc;
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
     if(!fused_scores(h)) hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][5], h->stream));
@@ -1404,6 +1404,7 @@ void capture_graphs(kpax_handle* h)
                 h->expand_lds = keep;
                 return;
             }
+            h->prop_slots      = 1;  // a captured iteration holds the proposal pass whatever runs
             const int rc       = enqueue_iteration(h, p, false);
             const hipError_t e = hipStreamEndCapture(h->stream, &g);
             if(rc != KPAX_OK || e != hipSuccess || !g ||
@@ -1524,9 +1525,10 @@ int run_queries_impl(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result*
     for(uint32_t s = Q; s-- > 0;) free_slots.push_back(s);  // lowest slot first
     uint32_t active = 0;
     uint64_t it     = 0;  // iterations launched by this call
+    h->prop_slots   = 0;  // direct launches skip the proposal pass while no admitted slot samples from it
 
     auto admit = [&](bool first) -> int {
-        uint32_t n_init = 0;
+        uint32_t n_init = 0, n_prop = 0;
         for(size_t k = 0; k < waiting.size() && !free_slots.empty();)
         {
             const uint32_t qi = waiting[k];
@@ -1546,6 +1548,7 @@ int run_queries_impl(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result*
             d.query = 0;  // the same draws as a plan of this query alone
             d.seed  = qs[qi].seed;
             d.mix   = d.cap >= h->prop_min_budget ? h->mix : 0.0f;  // proposal policy by tree budget
+            if(d.mix > 0.0f) ++h->prop_slots, ++n_prop;
             // a single-slot handle expands with the LDS grid when this
             // query's environment has an image (else the CSR grid)
             if(Q == 1) h->expand_lds = lds_expand(h) && h->envs[d.env].has_lds;
@@ -1574,7 +1577,7 @@ int run_queries_impl(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result*
         hipLaunchKernelGGL(k_init, dim3(n_init), dim3(256), 0, h->stream, h->ctl, h->qd, h->tree, h->R,
                            h->slot_status, h->d_init_slots, h->d_init_starts, S, C, h->T, h->RP, h->PP);
         HIPC(hipGetLastError());
-        if(h->mix > 0.0f)
+        if(h->mix > 0.0f && n_prop > 0)
         {
             const int rc = launch_propose(h, n_init);
             if(rc != KPAX_OK) return rc;
@@ -1643,6 +1646,7 @@ int run_queries_impl(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result*
             h->stats.integrated_steps += c.steps_done;
             h->stats.plans += 1;
             ranges.release(slot_base[s], slot_cap[s]);
+            if(h->h_qd[s].mix > 0.0f) --h->prop_slots;
             slot_query[s] = -1;
             free_slots.push_back(s);
             --active;
