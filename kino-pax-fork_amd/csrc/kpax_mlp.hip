@@ -138,11 +138,11 @@ __global__ __launch_bounds__(256) void k_pack_weights(const float* w, MlpDev* d)
 }
 
 // ---------------------------------------------------------------------------
-// The network on one 32-row tile held by one wave.  xa holds the tile's
-// inputs (bf16, 80 columns of kLdH); xb is scratch.  Returns, per lane, the
-// 16 raw outputs of column (lane % 32) (< 8 meaningful) at the rows
-// mfma_row(i, lane).  When keep is set, pre-activations (fp32) and
-// activations (bf16) of both hidden layers are stored for the backward pass.
+// The training forward pass on one 32-row tile held by one wave.  xa holds
+// the tile's inputs (bf16, 80 columns of kLdH); xb is scratch.  Returns, per
+// lane, the 16 raw outputs of column (lane % 32) (< 8 meaningful) at the rows
+// mfma_row(i, lane), and stores the pre-activations (fp32) and activations
+// (bf16) of both hidden layers for the backward pass.
 // ---------------------------------------------------------------------------
 struct Keep {
     float* h1;      // [rows][128] pre-activations
@@ -153,7 +153,6 @@ struct Keep {
     uint32_t nrows; // valid rows of this tile
 };
 
-template<bool KEEP>
 __device__ __forceinline__ void mlp_tile(const MlpDev& W, uint16_t (*xa)[kLdH], uint16_t (*xb)[kLdH], float* out,
                                          const Keep& keep)
 {
@@ -181,8 +180,7 @@ __device__ __forceinline__ void mlp_tile(const MlpDev& W, uint16_t (*xa)[kLdH], 
             const float h = acc[nt][i] + W.b1[col];
             const uint16_t a = f2bf(gelu(h));
             xb[row][col] = a;
-            if constexpr(KEEP)
-                if((uint32_t)row < keep.nrows)
+            if((uint32_t)row < keep.nrows)
                 {
                     keep.h1[(size_t)(keep.row0 + row) * kMlpHidden + col] = h;
                     keep.a1[(size_t)(keep.row0 + row) * kMlpHidden + col] = a;
@@ -211,8 +209,7 @@ __device__ __forceinline__ void mlp_tile(const MlpDev& W, uint16_t (*xa)[kLdH], 
             const float h = acc[nt][i] + W.b2[col];
             const uint16_t a = f2bf(gelu(h));
             xa[row][col] = a;
-            if constexpr(KEEP)
-                if((uint32_t)row < keep.nrows)
+            if((uint32_t)row < keep.nrows)
                 {
                     keep.h2[(size_t)(keep.row0 + row) * kMlpHidden + col] = h;
                     keep.a2[(size_t)(keep.row0 + row) * kMlpHidden + col] = a;
@@ -231,8 +228,8 @@ __device__ __forceinline__ void mlp_tile(const MlpDev& W, uint16_t (*xa)[kLdH], 
     wave_lds_sync();
 }
 
-// Shared memory of the network kernels: the inference weights, then two
-// 32-row activation tiles per wave.
+// Shared memory of the training forward pass: the weights it reads (the
+// part of MlpDev up to b3), then two 32-row activation tiles per wave.
 constexpr int kMlpWaves = 4;
 constexpr size_t kMlpTileBytes = 32 * kLdH * sizeof(uint16_t);
 constexpr size_t kMlpLdsBytes  = kMlpLdsWeights + 2 * kMlpWaves * kMlpTileBytes;
@@ -313,8 +310,8 @@ __global__ __launch_bounds__(256) void k_propose_sample(ModelParams P, const flo
 // Proposal pass of the planner: features of the nodes appended this
 // iteration (or of the roots of newly admitted slots), the network, and the
 // node's Gaussian (means, then sigmas = exp(clamp(log sigma, -4, 1/2))) into
-// the per-node proposal arrays.  One wave per 32-row tile; tiles of all
-// running slots are numbered by the advance kernel's offsets.
+// the per-node proposal arrays.  One wave per 16-node tile; each block lays
+// out the tiles of all running slots from the append kernel's counts.
 // ---------------------------------------------------------------------------
 struct ProposeArgs {
     const Ctl* ctl;
@@ -470,7 +467,7 @@ __global__ __launch_bounds__(256) void k_train_forward(const MlpDev* Wd, TrainBu
         wave_lds_sync();
         float o[16];
         const Keep keep{T.h1, T.h2, T.a1, T.a2, row0, B - row0 < 32 ? B - row0 : 32u};
-        mlp_tile<true>(W, xa, xb, o, keep);
+        mlp_tile(W, xa, xb, o, keep);
         // outputs to LDS (fp32 view of xb: 32 rows x 8) for the row-wise loss
         float* ot = (float*)&xb[0][0];
         if(r < kMlpOut)
@@ -730,8 +727,6 @@ int build_occupancy(kpax_handle* h, uint32_t env)
     HIPC(hipGetLastError());
     return KPAX_OK;
 }
-
-size_t mlp_lds_bytes() { return kMlpLdsBytes; }
 
 int launch_propose(kpax_handle* h, uint32_t n_init)
 {
