@@ -186,7 +186,7 @@ def test_proposal_budget_policy(kpax):
 PARAMS = os.path.join(ROOT, "kino-pax-fork_amd", "data", "proposal_forest.npy")
 
 
-@pytest.mark.parametrize("budget,seed", [(1 << 17, 21), (1 << 18, 22)])
+@pytest.mark.parametrize("budget,seed", [(1 << 17, 21), (1 << 18, 22), (1 << 20, 24)])
 def test_proposal_plan_matches_oracle_with_injected_gaussians(kpax, oracle, budget, seed):
     """A plan with the shipped network mixed in (50/50) equals the oracle's plan
     tree for tree when the oracle is given each node's Gaussian as the GPU's
