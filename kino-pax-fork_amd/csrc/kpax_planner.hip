@@ -2426,29 +2426,30 @@ int kpax_autotune(kpax_handle* h, const kpax_query* queries, uint32_t n, uint32_
     cur.expand_blocks = occ * cu;
     double cur_ms     = 0.0;
     int rc            = run(cur, &cur_ms, ref);
-    std::vector<std::vector<kpax_launch>> knobs(9);
-    for(uint32_t k = 1; k <= occ; ++k) knobs[0].push_back(kpax_launch{k * cu, 0, 0, 0, 0, 0, 0});
-    for(uint32_t c : {32u, 64u, 128u, 256u, 512u}) knobs[1].push_back(kpax_launch{0, c, 0, 0, 0, 0, 0});
-    for(uint32_t t : {4096u, 2048u, 1024u}) knobs[2].push_back(kpax_launch{0, 0, 0, 0, 0, 0, t});
-    for(uint32_t m : {1u, 2u, 4u}) knobs[3].push_back(kpax_launch{0, 0, m * cu, 0, 0, 0, 0});
-    for(int g : {1, 0}) knobs[4].push_back(kpax_launch{0, 0, 0, g, 0, 0, 0});
-    for(int o : {1, 0}) knobs[5].push_back(kpax_launch{0, 0, 0, 0, o, 0, 0});
-    for(uint32_t r : {1u, 8u, 16u, 24u, 32u}) knobs[6].push_back(kpax_launch{0, 0, 0, 0, 0, r, 0});
-    for(int g : {1, 0}) knobs[7].push_back(kpax_launch{0, 0, 0, 0, 0, 0, 0, g});
-    for(int w : {1, 0}) knobs[8].push_back(kpax_launch{0, 0, 0, 0, 0, 0, 0, 0, 0, w});
-    for(int k = 0; rc == KPAX_OK && k < 9; ++k)
-        for(const kpax_launch& v : knobs[k])
+    // one knob at a time: each candidate is the best shape so far with one
+    // field changed
+    struct Knob {
+        void (*set)(kpax_launch&, uint32_t);
+        std::vector<uint32_t> values;
+    };
+    std::vector<uint32_t> blocks;
+    for(uint32_t k = 1; k <= occ; ++k) blocks.push_back(k * cu);
+    const std::vector<Knob> knobs = {
+        {[](kpax_launch& l, uint32_t v) { l.expand_blocks = v; }, blocks},
+        {[](kpax_launch& l, uint32_t v) { l.expand_chunk = v; }, {32u, 64u, 128u, 256u, 512u}},
+        {[](kpax_launch& l, uint32_t v) { l.scan_tile = v; }, {4096u, 2048u, 1024u}},
+        {[](kpax_launch& l, uint32_t v) { l.scan_blocks = v; }, {cu, 2 * cu, 4 * cu}},
+        {[](kpax_launch& l, uint32_t v) { l.use_graph = (int32_t)v; }, {1u, 0u}},
+        {[](kpax_launch& l, uint32_t v) { l.order_children = (int32_t)v; }, {1u, 0u}},
+        {[](kpax_launch& l, uint32_t v) { l.refill_lanes = v; }, {1u, 8u, 16u, 24u, 32u}},
+        {[](kpax_launch& l, uint32_t v) { l.lds_grid = (int32_t)v; }, {1u, 0u}},
+        {[](kpax_launch& l, uint32_t v) { l.wave_priority = (int32_t)v; }, {1u, 0u}},
+    };
+    for(size_t k = 0; rc == KPAX_OK && k < knobs.size(); ++k)
+        for(const uint32_t v : knobs[k].values)
         {
             kpax_launch cand = cur;
-            if(k == 0) cand.expand_blocks = v.expand_blocks;
-            if(k == 1) cand.expand_chunk = v.expand_chunk;
-            if(k == 2) cand.scan_tile = v.scan_tile;
-            if(k == 3) cand.scan_blocks = v.scan_blocks;
-            if(k == 4) cand.use_graph = v.use_graph;
-            if(k == 5) cand.order_children = v.order_children;
-            if(k == 6) cand.refill_lanes = v.refill_lanes;
-            if(k == 7) cand.lds_grid = v.lds_grid;
-            if(k == 8) cand.wave_priority = v.wave_priority;
+            knobs[k].set(cand, v);
             if(memcmp(&cand, &cur, sizeof(cand)) == 0) continue;
             double ms = 0.0;
             if((rc = run(cand, &ms, got)) != KPAX_OK) break;
