@@ -6,6 +6,8 @@ tree size, goal node, expansion and step counters, and the path rows bit for
 bit.  Checked against the CPU oracle for small budgets and against single
 GPU plans for larger ones.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -220,3 +222,48 @@ def test_serving_batch_at_its_shape(kpax, oracle):
             (o["status"], o["iterations"], o["tree_size"], o["expansions"], o["goal_index"])
         if r.status == kpax.OK:
             assert np.array_equal(r.rows.view(np.uint32), oracle_path_rows(o, c).view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_batch_at_the_slot_and_environment_limits(kpax):
+    """A handle at its limits: 1,024 query slots (the most kpax_create_batch
+    takes) over 64 environments (the most a handle holds), every slot busy at
+    once, with and without the learned proposal (its per-slot table then has
+    all 1,024 entries).  Every query returns what it returns planned alone; a
+    query naming environment 64 is rejected while the rest run; a 1,025-slot
+    handle is refused."""
+    import proposal
+    cfg = workload(2)[0]
+    cfg.max_iter = 60
+    rng = np.random.default_rng(11)
+    envs = [kpax.generate_environment(2, 7000 + e, int(rng.integers(40, 121)), cfg.ws_min, cfg.ws_max)
+            for e in range(64)]
+    qs = proposal.serving_queries(cfg, envs, 1024, seed=13, min_dist=1.0, max_dist=4.0,
+                                  budget_range=(4096, 16384))
+    params = np.load(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "kino-pax-fork_amd",
+                                  "data", "proposal_forest.npy"))
+    T = sum(q.budget for q in qs) + 1024
+    with pytest.raises(kpax.KpaxError):
+        kpax.KPAX(cfg, max_queries=1025, node_capacity=T)
+    bad = kpax.Query(qs[0].start, qs[0].goal, seed=1, budget=4096, env=64)
+    for mix in (0.0, 0.5):
+        print(f"1024 slots, 64 environments, mix {mix}", flush=True)
+        with kpax.KPAX(cfg, max_queries=1024, node_capacity=T) as p:
+            for e, o in enumerate(envs):
+                p.set_environment(e, o)
+            if mix:
+                p.set_proposal(params, mix)
+            res = p.plan_batch(qs + [bad])
+        assert res[-1].status == kpax.INVALID_ARG
+        res = res[:-1]
+        with kpax.KPAX(cfg, max_queries=1, node_capacity=16384) as s:
+            for e, o in enumerate(envs):
+                s.set_environment(e, o)
+            if mix:
+                s.set_proposal(params, mix)
+            seq = [s.plan_batch([q])[0] for q in qs]
+        diff = [i for i, (a, b) in enumerate(zip(res, seq)) if not _same(a, b)]
+        assert not diff, diff[:10]
+        found = sum(r.status == kpax.OK for r in res)
+        print(f"  found {found} of 1024", flush=True)
+        assert found > 512 and any(r.status != kpax.OK for r in res)  # both outcomes at the limits
