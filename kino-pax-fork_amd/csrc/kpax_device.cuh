@@ -158,16 +158,10 @@ __device__ __forceinline__ bool box_hits(const GridView& g, int k, const float b
 // spatialHash.cu:145-194; broad phase collisionCheck.cu:3-10).  An obstacle
 // that overlaps the box shares a coarse cell with it, so the verdict is the
 // brute-force one, as with the CSR grid.
-__device__ __forceinline__ bool seg_free_lds(const GridView& g, const float bmin[3], const float bmax[3])
+__device__ __forceinline__ bool lds_walk(const GridView& g, const int lo[3], const int hi[3], const float bmin[3],
+                                         const float bmax[3])
 {
-    const LdsGrid& L = g.lg;
-    int lo[3], hi[3];
-#pragma unroll
-    for(int d = 0; d < 3; ++d)
-    {
-        lo[d] = cell_of(bmin[d], L.mn[d], L.inv[d], L.D[d]);
-        hi[d] = cell_of(bmax[d], L.mn[d], L.inv[d], L.D[d]);
-    }
+    const LdsGrid& L    = g.lg;
     const uint16_t* st = (const uint16_t*)(g.lds + L.off_start);
     const uint16_t* it = (const uint16_t*)(g.lds + L.off_items);
     const float2* bx   = (const float2*)g.lds;
@@ -187,6 +181,27 @@ __device__ __forceinline__ bool seg_free_lds(const GridView& g, const float bmin
             }
         }
     return true;
+}
+
+__device__ __forceinline__ bool seg_free_lds(const GridView& g, const float bmin[3], const float bmax[3])
+{
+    const LdsGrid& L = g.lg;
+    int lo[3], hi[3];
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        lo[d] = cell_of(bmin[d], L.mn[d], L.inv[d], L.D[d]);
+        hi[d] = cell_of(bmax[d], L.mn[d], L.inv[d], L.D[d]);
+    }
+    return lds_walk(g, lo, hi, bmin, bmax);
+}
+
+// Image cells of a point (the first step of a child carries them from its
+// start point, every later step from the previous step's end).
+__device__ __forceinline__ void lds_cells(const GridView& g, const float* p, int c[3])
+{
+#pragma unroll
+    for(int d = 0; d < 3; ++d) c[d] = cell_of(p[d], g.lg.mn[d], g.lg.inv[d], g.lg.D[d]);
 }
 
 // Copies g's LDS image into `lds` with every thread of the work-group and
@@ -301,13 +316,13 @@ template<> struct Dims<KPAX_MODEL_QUADCOPTER> {
 // float result does not depend on how steps are scheduled.
 
 // Double integrator: statePropagatorSpatialHash.cu:90-128.
-template<int GQ = kGridAny>
-__device__ __forceinline__ bool step_di(const ModelParams& P, const GridView& G, float* s, const float* u)
+// integrate_*: the model's step and its own bound checks (velocity, pitch);
+// step_* adds the workspace and collision checks of the step's segment.
+__device__ __forceinline__ bool integrate_di(const ModelParams& P, float* s, const float* u)
 {
     const float dt = P.dt;
     float x = s[0], y = s[1], z = s[2], vx = s[3], vy = s[4], vz = s[5];
     const float ax = u[0], ay = u[1], az = u[2];
-    const float p0[3] = {x, y, z};
     if(!P.di_textbook)
     {
         x += (vx + (vx + 2.0f * (vx + ax * dt / 2.0f) + (vx + ax * dt))) * dt / 6.0f;
@@ -327,19 +342,23 @@ __device__ __forceinline__ bool step_di(const ModelParams& P, const GridView& G,
         vz += az * dt;
     }
     s[0] = x, s[1] = y, s[2] = z, s[3] = vx, s[4] = vy, s[5] = vz;
-    if(!inb(vx, P.vmin, P.vmax) || !inb(vy, P.vmin, P.vmax) || !inb(vz, P.vmin, P.vmax)) return false;
-    const float p1[3] = {x, y, z};
-    return ws_and_collision<GQ>(P, G, p0, p1);
+    return inb(vx, P.vmin, P.vmax) && inb(vy, P.vmin, P.vmax) && inb(vz, P.vmin, P.vmax);
+}
+
+template<int GQ = kGridAny>
+__device__ __forceinline__ bool step_di(const ModelParams& P, const GridView& G, float* s, const float* u)
+{
+    const float p0[3] = {s[0], s[1], s[2]};
+    if(!integrate_di(P, s, u)) return false;
+    return ws_and_collision<GQ>(P, G, p0, s);
 }
 
 // Dubins airplane: statePropagatorSpatialHash.cu:158-214.
-template<int GQ = kGridAny>
-__device__ __forceinline__ bool step_dubins(const ModelParams& P, const GridView& G, float* s, const float* u)
+__device__ __forceinline__ bool integrate_dubins(const ModelParams& P, float* s, const float* u)
 {
     const float dt = P.dt, h = P.half, d6 = P.d6;
     float x = s[0], y = s[1], z = s[2], yaw = s[3], pitch = s[4], v = s[5];
     const float a = u[0], yr = u[1], pr = u[2];
-    const float p0[3] = {x, y, z};
     float cp, sp, cy, sy, cpm, spm, cym, sym, cpe, spe, cye, sye;
     sincos_f32(pitch, &sp, &cp);
     sincos_f32(yaw, &sy, &cy);
@@ -359,9 +378,15 @@ __device__ __forceinline__ bool step_dubins(const ModelParams& P, const GridView
     pitch += dt * pr;
     v += d6 * (a + 2.0f * (a + a) + a);
     s[0] = x, s[1] = y, s[2] = z, s[3] = yaw, s[4] = pitch, s[5] = v;
-    if(!inb(v, P.vmin, P.vmax) || !inb(pitch, P.pmin, P.pmax)) return false;
-    const float p1[3] = {x, y, z};
-    return ws_and_collision<GQ>(P, G, p0, p1);
+    return inb(v, P.vmin, P.vmax) && inb(pitch, P.pmin, P.pmax);
+}
+
+template<int GQ = kGridAny>
+__device__ __forceinline__ bool step_dubins(const ModelParams& P, const GridView& G, float* s, const float* u)
+{
+    const float p0[3] = {s[0], s[1], s[2]};
+    if(!integrate_dubins(P, s, u)) return false;
+    return ws_and_collision<GQ>(P, G, p0, s);
 }
 
 // 12D quadcopter ode: statePropagator.cu:325-370.  The reference's terms,
@@ -425,13 +450,11 @@ __device__ __forceinline__ void quad_ode(float* out, const float* x, const float
 // Quadcopter RK4: statePropagatorSpatialHash.cu:256-296.  The stage sum
 // k1 + 2 k2 + 2 k3 + k4 is accumulated as each stage finishes (the same
 // operations in the same order), so only two stage vectors are live.
-template<int GQ = kGridAny>
-__device__ __forceinline__ bool step_quad(const ModelParams& P, const GridView& G, float* h0, const float* u)
+__device__ __forceinline__ bool integrate_quad(const ModelParams& P, float* h0, const float* u)
 {
     const float dt = P.dt, half = P.half, d6 = P.d6;
     float ka[12], kb[12], acc[12];
     const float zero[12] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    const float p0[3] = {h0[0], h0[1], h0[2]};
     float sb[3], cb[3], sn[3], cn[3];
 #pragma unroll
     for(int i = 0; i < 3; ++i) sincos_f32(h0[3 + i], &sb[i], &cb[i]);
@@ -451,9 +474,56 @@ __device__ __forceinline__ bool step_quad(const ModelParams& P, const GridView& 
     quad_ode(kb, h0, ka, dt, u, P, sn, cn);  // k4
 #pragma unroll
     for(int j = 0; j < 12; ++j) h0[j] = fma_(d6, acc[j] + kb[j], h0[j]);
-    if(!inb(h0[6], P.vmin, P.vmax) || !inb(h0[7], P.vmin, P.vmax) || !inb(h0[8], P.vmin, P.vmax)) return false;
-    const float p1[3] = {h0[0], h0[1], h0[2]};
-    return ws_and_collision<GQ>(P, G, p0, p1);
+    return inb(h0[6], P.vmin, P.vmax) && inb(h0[7], P.vmin, P.vmax) && inb(h0[8], P.vmin, P.vmax);
+}
+
+template<int GQ = kGridAny>
+__device__ __forceinline__ bool step_quad(const ModelParams& P, const GridView& G, float* h0, const float* u)
+{
+    const float p0[3] = {h0[0], h0[1], h0[2]};
+    if(!integrate_quad(P, h0, u)) return false;
+    return ws_and_collision<GQ>(P, G, p0, h0);
+}
+
+template<int MODEL>
+__device__ __forceinline__ bool integrate(const ModelParams& P, float* s, const float* u)
+{
+    if constexpr(MODEL == KPAX_MODEL_DOUBLE_INTEGRATOR)
+        return integrate_di(P, s, u);
+    else if constexpr(MODEL == KPAX_MODEL_DUBINS_AIRPLANE)
+        return integrate_dubins(P, s, u);
+    else
+        return integrate_quad(P, s, u);
+}
+
+// A step through the staged LDS image with the cells of its start point
+// known (cells, carried from the previous step): the image cells of the
+// segment's box are the minimum and maximum of its two ends' cells (the cell
+// map is monotone), so only the end point's are computed, and handed back
+// for the next step.  The same verdict and state as step<MODEL, kGridLds>.
+template<int MODEL>
+__device__ __forceinline__ bool step_lds_carry(const ModelParams& P, const GridView& G, float* s, const float* u,
+                                               int cells[3])
+{
+    const float p0[3] = {s[0], s[1], s[2]};
+    if(!integrate<MODEL>(P, s, u)) return false;
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+        if(!inb(s[d], P.ws_min[d], P.ws_max[d])) return false;
+    float bmin[3], bmax[3];
+    int lo[3], hi[3];
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        const bool sw = p0[d] > s[d];
+        bmin[d]       = sw ? s[d] : p0[d];
+        bmax[d]       = sw ? p0[d] : s[d];
+        const int c1  = cell_of(s[d], G.lg.mn[d], G.lg.inv[d], G.lg.D[d]);
+        lo[d]         = cells[d] < c1 ? cells[d] : c1;
+        hi[d]         = cells[d] < c1 ? c1 : cells[d];
+        cells[d]      = c1;
+    }
+    return lds_walk(G, lo, hi, bmin, bmax);
 }
 
 template<int MODEL, int GQ = kGridAny>
