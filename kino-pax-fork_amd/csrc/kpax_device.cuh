@@ -217,15 +217,9 @@ __device__ __forceinline__ void stage_lds_grid(GridView& g, uint4* lds)
     g.lds = (const uint8_t*)lds;
 }
 
-__device__ __forceinline__ bool seg_free_csr(const GridView& g, const float bmin[3], const float bmax[3])
+__device__ __forceinline__ bool csr_walk(const GridView& g, const int lo[3], const int hi[3], const float bmin[3],
+                                         const float bmax[3])
 {
-    int lo[3], hi[3];
-#pragma unroll
-    for(int d = 0; d < 3; ++d)
-    {
-        lo[d] = cell_of(bmin[d], g.gp.mn[d], g.gp.inv[d], g.gp.D[d]);
-        hi[d] = cell_of(bmax[d], g.gp.mn[d], g.gp.inv[d], g.gp.D[d]);
-    }
     if(hi[0] - lo[0] <= 1 && hi[1] - lo[1] <= 1 && hi[2] - lo[2] <= 1)
     {
         // An integration step spans at most 2 cells per axis: the cell-start
@@ -258,6 +252,25 @@ __device__ __forceinline__ bool seg_free_csr(const GridView& g, const float bmin
                 if(box_hits(g, k, bmin, bmax)) return false;
         }
     return true;
+}
+
+__device__ __forceinline__ bool seg_free_csr(const GridView& g, const float bmin[3], const float bmax[3])
+{
+    int lo[3], hi[3];
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        lo[d] = cell_of(bmin[d], g.gp.mn[d], g.gp.inv[d], g.gp.D[d]);
+        hi[d] = cell_of(bmax[d], g.gp.mn[d], g.gp.inv[d], g.gp.D[d]);
+    }
+    return csr_walk(g, lo, hi, bmin, bmax);
+}
+
+// CSR grid cells of a point (carried from step to step like lds_cells).
+__device__ __forceinline__ void csr_cells(const GridView& g, const float* p, int c[3])
+{
+#pragma unroll
+    for(int d = 0; d < 3; ++d) c[d] = cell_of(p[d], g.gp.mn[d], g.gp.inv[d], g.gp.D[d]);
 }
 
 // Grid query path, fixed at compile time where a kernel knows it (so the
@@ -496,15 +509,17 @@ __device__ __forceinline__ bool integrate(const ModelParams& P, float* s, const 
         return integrate_quad(P, s, u);
 }
 
-// A step through the staged LDS image with the cells of its start point
-// known (cells, carried from the previous step): the image cells of the
-// segment's box are the minimum and maximum of its two ends' cells (the cell
-// map is monotone), so only the end point's are computed, and handed back
-// for the next step.  The same verdict and state as step<MODEL, kGridLds>.
-template<int MODEL>
-__device__ __forceinline__ bool step_lds_carry(const ModelParams& P, const GridView& G, float* s, const float* u,
-                                               int cells[3])
+// A step with the grid cells of its start point known (cells, carried from
+// the previous step): the cells of the segment's box are the minimum and
+// maximum of its two ends' cells (the cell map is monotone), so only the end
+// point's are computed, and handed back for the next step.  GQ: the staged
+// LDS image's cells or the CSR grid's.  The same verdict and state as
+// step<MODEL, GQ>.
+template<int MODEL, int GQ>
+__device__ __forceinline__ bool step_carry(const ModelParams& P, const GridView& G, float* s, const float* u,
+                                           int cells[3])
 {
+    static_assert(GQ == kGridLds || GQ == kGridCsr, "a carried step needs a fixed grid path");
     const float p0[3] = {s[0], s[1], s[2]};
     if(!integrate<MODEL>(P, s, u)) return false;
 #pragma unroll
@@ -518,12 +533,16 @@ __device__ __forceinline__ bool step_lds_carry(const ModelParams& P, const GridV
         const bool sw = p0[d] > s[d];
         bmin[d]       = sw ? s[d] : p0[d];
         bmax[d]       = sw ? p0[d] : s[d];
-        const int c1  = cell_of(s[d], G.lg.mn[d], G.lg.inv[d], G.lg.D[d]);
+        const int c1  = GQ == kGridLds ? cell_of(s[d], G.lg.mn[d], G.lg.inv[d], G.lg.D[d])
+                                       : cell_of(s[d], G.gp.mn[d], G.gp.inv[d], G.gp.D[d]);
         lo[d]         = cells[d] < c1 ? cells[d] : c1;
         hi[d]         = cells[d] < c1 ? c1 : cells[d];
         cells[d]      = c1;
     }
-    return lds_walk(G, lo, hi, bmin, bmax);
+    if constexpr(GQ == kGridLds)
+        return lds_walk(G, lo, hi, bmin, bmax);
+    else
+        return csr_walk(G, lo, hi, bmin, bmax);
 }
 
 template<int MODEL, int GQ = kGridAny>

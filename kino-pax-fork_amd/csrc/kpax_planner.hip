@@ -446,7 +446,7 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
     uint32_t q_next = 0, q_end = 0;  // the wave's current chunk [q_next, q_end)
     bool drained = false;
     bool live = false, pend = false, valid = false;  // integrating / finished, not yet written / its verdict
-    int cells[3] = {0, 0, 0};  // LDS grid path: image cells of the current step's start point
+    int cells[3] = {0, 0, 0};  // grid cells of the current step's start point
     const uint32_t refill = A->refill;
     const int prio_q      = A->prio_q;
     for(;;)
@@ -549,7 +549,10 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
                         i          = 0;
                         admit_draw = r0.w;
                         live       = true;
-                        if constexpr(GQ == kGridLds) lds_cells(G, x, cells);
+                        if constexpr(GQ == kGridLds)
+                            lds_cells(G, x, cells);
+                        else
+                            csr_cells(G, x, cells);
                     }
                     const uint32_t wanted = (uint32_t)__popcll(need);
                     q_next += wanted < avail ? wanted : avail;
@@ -561,10 +564,7 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
         if(prio_q) set_issue_priority(live ? n - i : 0, prio_q);
         if(live)
         {
-            if constexpr(GQ == kGridLds)
-                valid = step_lds_carry<MODEL>(P, G, x, u, cells);
-            else
-                valid = step<MODEL, GQ>(P, G, x, u);
+            valid = step_carry<MODEL, GQ>(P, G, x, u, cells);
             ++i;
             if(!valid || i == n)
             {
