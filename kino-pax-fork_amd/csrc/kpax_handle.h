@@ -111,7 +111,8 @@ struct kpax_handle {
     uint32_t* child_off     = nullptr;  // Qmax + 1 offsets, written by the advance kernel
     uint32_t* atile_off     = nullptr;
     uint32_t* ftile_off     = nullptr;
-    int32_t* slot_status    = nullptr;  // Qmax statuses + the running count
+    int32_t* slot_status    = nullptr;  // device address of h_status (2 parities x (Qmax statuses + the running count))
+    int status_parity       = 0;        // parity of the iteration being enqueued
     uint64_t* st_append     = nullptr;
     uint64_t* st_frontier   = nullptr;
     uint32_t n_status       = 0;
@@ -142,7 +143,8 @@ struct kpax_handle {
     int32_t* path_idx       = nullptr;
     uint32_t* d_path_slots  = nullptr;
     // pinned host mirrors
-    int32_t* h_status       = nullptr;  // 2 x (Qmax + 1): statuses after iterations of each parity
+    int32_t* h_status       = nullptr;  // 2 x (Qmax + 1): statuses after iterations of each parity (pinned,
+                                        // written by the kernels)
     kpax::Ctl* h_ctl        = nullptr;
     kpax::QueryDev* h_qd    = nullptr;
     uint32_t* h_init_slots  = nullptr;

@@ -27,9 +27,10 @@
 //                         (updateFrontier_kernel second half + findInd/repeatInd)
 // Work of all slots is flattened: children, append tiles and frontier tiles
 // are numbered across slots by the advance kernel's offsets, and each item
-// finds its slot by binary search.  The launches and the status copy go on
-// the stream directly, or (launch option) as one captured HIP graph per
-// status parity.
+// finds its slot by binary search.  The advance and the frontier pass write
+// the slots' statuses straight into a pinned host buffer (one per iteration
+// parity), so an iteration is its launches alone, on the stream directly or
+// (launch option) as one captured HIP graph per status parity.
 
 #include <hip/hip_runtime.h>
 
@@ -87,10 +88,13 @@ __global__ __launch_bounds__(256) void k_reset_regions(RegionView rv, const uint
 }
 
 // Root and first iteration record of each listed slot (KPAX.cu:60-88).  One
-// block per slot; starts holds 16 floats per listed slot.
+// block per slot; starts holds 16 floats per listed slot.  The slot's first
+// status goes into both parities of the host's status buffer (Qn = Qmax + 1
+// words each).
 __global__ __launch_bounds__(256) void k_init(Ctl* ctl, const QueryDev* qd, TreeView tree, uint32_t* R,
-                                              int32_t* slot_status, const uint32_t* slots, const float* starts, int S,
-                                              int C, uint32_t T, RegionParams RP, PlanParams PP)
+                                              int32_t* slot_status, uint32_t Qn, const uint32_t* slots,
+                                              const float* starts, int S, int C, uint32_t T, RegionParams RP,
+                                              PlanParams PP)
 {
     const uint32_t q   = slots[blockIdx.x];
     const QueryDev d   = qd[q];
@@ -114,8 +118,9 @@ __global__ __launch_bounds__(256) void k_init(Ctl* ctl, const QueryDev* qd, Tree
         c.cur          = c.nxt;
         c.cur.status   = kNotStarted;  // nothing runs until k_advance publishes nxt
         c.goal         = INT_MAX;
-        ctl[q]         = c;
-        slot_status[q] = c.nxt.status;
+        ctl[q]              = c;
+        slot_status[q]      = c.nxt.status;
+        slot_status[Qn + q] = c.nxt.status;
     }
 }
 
@@ -1266,6 +1271,12 @@ void launch_expand_model(kpax_handle* h)
 }
 
 
+// The slot statuses of the iteration being enqueued: the pinned host buffer
+// of its parity (Qmax statuses and the running count), written by the advance
+// and the frontier pass; the host reads it once the iteration's event is
+// done, with no copy on the stream.
+int32_t* status_of_parity(const kpax_handle* h) { return h->slot_status + (size_t)h->status_parity * (h->Qmax + 1); }
+
 AdvanceArgs advance_args(const kpax_handle* h)
 {
     AdvanceArgs V;
@@ -1275,7 +1286,7 @@ AdvanceArgs advance_args(const kpax_handle* h)
     V.child_off   = h->child_off;
     V.atile_off   = h->atile_off;
     V.ftile_off   = h->ftile_off;
-    V.slot_status = h->slot_status;
+    V.slot_status = status_of_parity(h);
     V.Q           = h->Qmax;
     V.st_append   = h->st_append;
     V.st_frontier = h->st_frontier;
@@ -1346,20 +1357,21 @@ void launch_frontier(kpax_handle* h)
     if(fused_scores(h))
         hipLaunchKernelGGL((k_frontier<NT, true>), dim3(h->scan_blocks), dim3(NT),
                            ((size_t)h->RP.r1_count + 1024) * sizeof(float), h->stream, h->ctl, h->qd, h->bc,
-                           h->ftile_off, h->slot_status, h->Qmax, h->tree, h->R, h->rv, h->st_frontier, h->PP,
+                           h->ftile_off, status_of_parity(h), h->Qmax, h->tree, h->R, h->rv, h->st_frontier, h->PP,
                            (uint32_t)h->RP.r1_count, h->RP);
     else
         hipLaunchKernelGGL((k_frontier<NT, false>), dim3(h->scan_blocks), dim3(NT), 0, h->stream, h->ctl, h->qd,
-                           h->bc, h->ftile_off, h->slot_status, h->Qmax, h->tree, h->R, h->rv, h->st_frontier,
+                           h->bc, h->ftile_off, status_of_parity(h), h->Qmax, h->tree, h->R, h->rv, h->st_frontier,
                            h->PP, (uint32_t)h->RP.r1_count, h->RP);
 }
 
-// The five launches of one iteration and the copy of the slot statuses into
-// the pinned buffer of parity p.  With `ev` set, phase boundaries are recorded
-// between the launches.
+// The launches of one iteration, whose kernels write the slot statuses into
+// the pinned buffer of parity p.  With `ev` set, phase boundaries are
+// recorded between the launches.
 int enqueue_iteration(kpax_handle* h, int p, bool ev)
 {
-    const uint32_t Q = h->Qmax;
+    const uint32_t Q     = h->Qmax;
+    h->status_parity     = p;
     if(!fused_advance(h)) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, advance_args(h));
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
     int rc = launch_expand(h, ev ? h->ph_ev[p][1] : nullptr);
@@ -1384,8 +1396,6 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
     }
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][6], h->stream));
     HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(h->h_status + (size_t)p * (Q + 1), h->slot_status, (Q + 1) * sizeof(int32_t),
-                        hipMemcpyDeviceToHost, h->stream));
     return KPAX_OK;
 }
 
@@ -1580,7 +1590,7 @@ int run_queries_impl(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result*
                            dim3(std::min<uint32_t>(blocks_for((uint64_t)n_init * h->R2, 256), 8 * h->cu_count)),
                            dim3(256), 0, h->stream, h->rv, h->d_init_slots, n_init, R1, h->R2);
         hipLaunchKernelGGL(k_init, dim3(n_init), dim3(256), 0, h->stream, h->ctl, h->qd, h->tree, h->R,
-                           h->slot_status, h->d_init_slots, h->d_init_starts, S, C, h->T, h->RP, h->PP);
+                           h->slot_status, Q + 1, h->d_init_slots, h->d_init_starts, S, C, h->T, h->RP, h->PP);
         HIPC(hipGetLastError());
         if(h->mix > 0.0f && n_prop > 0)
         {
@@ -1788,7 +1798,7 @@ int run_queries(kpax_handle* h, const QueryIn* qs, uint32_t n, kpax_result* out)
     std::vector<int32_t> s0(Q + 1, kNotStarted);
     s0[Q] = 0;
     (void)hipMemcpy(h->ctl, c0.data(), Q * sizeof(Ctl), hipMemcpyHostToDevice);
-    (void)hipMemcpy(h->slot_status, s0.data(), (Q + 1) * sizeof(int32_t), hipMemcpyHostToDevice);
+    for(int par = 0; par < 2; ++par) memcpy(h->h_status + (size_t)par * (Q + 1), s0.data(), (Q + 1) * sizeof(int32_t));
     for(uint32_t i = 0; i < n; ++i) kpax_result_free(&out[i]);
     return rc;
 }
@@ -2023,7 +2033,6 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     TRY(dalloc(h, &h->atile_off, Q + 1));
     TRY(dalloc(h, &h->ftile_off, Q + 1));
     TRY(dalloc(h, &h->prep, 2 * T));
-    TRY(dalloc(h, &h->slot_status, Q + 1));
     h->n_status = (uint32_t)(T / kTile + Q + 2);
     TRY(dalloc(h, &h->st_append, h->n_status));
     TRY(dalloc(h, &h->st_frontier, h->n_status));
@@ -2043,6 +2052,7 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
                                                                                                 : KPAX_NO_MEMORY;
     };
     TRY(pinned((void**)&h->h_status, 2 * (Q + 1) * sizeof(int32_t)));
+    if(hipHostGetDevicePointer((void**)&h->slot_status, h->h_status, 0) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
     TRY(pinned((void**)&h->h_ctl, Q * sizeof(Ctl)));
     TRY(pinned((void**)&h->h_qd, Q * sizeof(QueryDev)));
     TRY(pinned((void**)&h->h_init_slots, Q * sizeof(uint32_t)));
@@ -2070,8 +2080,7 @@ int kpax_create_batch(const kpax_config* cfg, uint32_t max_queries, uint64_t nod
     if(hipMemcpy(h->ctl, c0.data(), Q * sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
     std::vector<int32_t> s0(Q + 1, kNotStarted);
     s0[Q] = 0;
-    if(hipMemcpy(h->slot_status, s0.data(), (Q + 1) * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
-        TRY(KPAX_DEVICE_ERROR);
+    for(int par = 0; par < 2; ++par) memcpy(h->h_status + (size_t)par * (Q + 1), s0.data(), (Q + 1) * sizeof(int32_t));
     if(hipMemset(h->bc, 0, sizeof(BatchCtl)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
     if(hipMemset(h->qd, 0, Q * sizeof(QueryDev)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);
     if(hipMemset(h->st_append, 0, h->n_status * sizeof(uint64_t)) != hipSuccess) TRY(KPAX_DEVICE_ERROR);

@@ -303,6 +303,11 @@ inline hipError_t hipFree(void* p)
 }
 inline hipError_t hipHostMalloc(void** p, size_t n, int) { return hipMalloc(p, n); }
 inline hipError_t hipHostFree(void* p) { return hipFree(p); }
+inline hipError_t hipHostGetDevicePointer(void** d, void* h, unsigned)
+{
+    *d = h;
+    return hipSuccess;
+}
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind)
 {
     std::memcpy(d, s, n);
