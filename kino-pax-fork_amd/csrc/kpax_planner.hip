@@ -9,13 +9,13 @@
 // no host round trip between them (the reference makes at least eight
 // blocking device->host transfers per iteration of its single query,
 // KPAX.cu:303-543):
-//   k_advance   1 block   publish each slot's iteration record; scan the
-//                         slots' child / tile counts into offsets
-//   k_prepare   coalesced each child's parent, Philox control and step draws,
+//   k_prepare   coalesced block 0 first runs the advance: publish each slot's
+//                         iteration record, scan the slots' child / tile
+//                         counts into offsets; then each child's parent,
+//                         Philox control and step draws,
 //                         written at its processing position (longest drawn
 //                         edge first within each work-group's segment: on by
-//                         default for a single-query handle, whose advance
-//                         runs in block 0 of this pass)
+//                         default for a single-query handle)
 //   k_expand    hot       one lane per child: RK4 with checks, region and
 //                         sub-region of the result, admission
 //                         (propagateFrontier_kernel1/2)
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_init(Ctl* ctl, const QueryDev* qd, Tree
         memset(&c, 0, sizeof(c));
         c.nxt          = make_iter(1u, 1u, PP.root_repeat, d.cap, PP);
         c.cur          = c.nxt;
-        c.cur.status   = kNotStarted;  // nothing runs until k_advance publishes nxt
+        c.cur.status   = kNotStarted;  // nothing runs until the advance publishes nxt
         c.goal         = INT_MAX;
         ctl[q]              = c;
         slot_status[q]      = c.nxt.status;
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_init(Ctl* ctl, const QueryDev* qd, Tree
 // into vector lanes, costing a readlane at every use inside the step loop,
 // while values in the block can be re-read with scalar loads whenever the
 // compiler needs the registers.  `s0` is slot 0's record, query and grid,
-// copied by k_advance each iteration for the single-slot form.
+// copied by the advance each iteration for the single-slot form.
 struct SlotUniform {
     Iter cur;
     QueryDev qd;
@@ -199,7 +199,7 @@ struct AdvanceArgs {
     SlotRun* run;
 };
 
-// The advance, by one block of 1024 threads (see k_advance).
+// The advance, by block 0 (1024 threads) of the preparation pass.
 __device__ __forceinline__ void advance_block(const AdvanceArgs& V)
 {
     Ctl* ctl = V.ctl;
@@ -289,7 +289,6 @@ __device__ __forceinline__ void advance_block(const AdvanceArgs& V)
         }
 }
 
-__global__ __launch_bounds__(1024) void k_advance(AdvanceArgs V) { advance_block(V); }
 
 // ---------------------------------------------------------------------------
 // Processing order of the children, longest drawn edge first, so the last
@@ -586,24 +585,49 @@ __global__ __launch_bounds__(MULTI ? 256 : 640) __attribute__((amdgpu_waves_per_
 // admission draw, stored as a 32-byte record at its processing position, so
 // the expansion's refill is two vector loads and the parent state.
 //
-// advance = 1 (the single-slot form): block 0 first runs the iteration's
-// advance (k_advance, one launch fewer per iteration), and every block takes
-// the iteration record and query from the slot's `nxt` and `qd`, which this
-// launch does not change, instead of the argument block's copies, which
-// block 0 rewrites.
+// advance = 1: block 0 first runs the iteration's advance (no launch of its
+// own), and every block takes the iteration records from the slots' `nxt`,
+// which this launch does not change, instead of `cur` and the argument
+// block's copies, which block 0 rewrites; in the batched form each block also
+// scans the slots' child counts itself, since the offsets block 0 publishes
+// are only for the passes after this one.
 template<int MODEL, bool MULTI>
 __global__ __launch_bounds__(kOrderThreads) void k_prepare(const ExpandArgs* __restrict__ A, int ordered,
                                                            AdvanceArgs V, int advance)
 {
     constexpr int C = Dims<MODEL>::C;
     __shared__ uint32_t pos[kOrderBuckets];
-    if constexpr(!MULTI)
-        if(advance && blockIdx.x == 0) advance_block(V);
+    // the batched form's child offsets per slot (with the advance: this
+    // block's own scan of the slots' next iterations)
+    __shared__ uint32_t s_off[MULTI ? kMaxSlots + 1 : 1];
+    __shared__ uint32_t s_scan[kOrderThreads / 64];
+    if(advance && blockIdx.x == 0) advance_block(V);
     uint32_t total;
     Iter it0{};
     QueryDev q0{};
+    const uint32_t* child_off = A->child_off;
     if constexpr(MULTI)
-        total = A->bc->children;
+    {
+        if(advance)
+        {
+            // the advance publishes the same offsets, but only for the passes
+            // after this one
+            const uint32_t q = threadIdx.x;
+            uint32_t m       = 0;
+            if(q < A->Q)
+            {
+                const Iter n = A->ctl[q].nxt;
+                if(n.status == kRunning) m = n.M;
+            }
+            const uint32_t em = block_excl_scan<kOrderThreads / 64>(m, &total, s_scan);
+            if(q < A->Q) s_off[q] = em;
+            if(q == 0) s_off[A->Q] = total;
+            __syncthreads();
+            child_off = s_off;
+        }
+        else
+            total = A->bc->children;
+    }
     else
     {
         it0   = advance ? A->ctl[0].nxt : A->s0.cur;
@@ -618,9 +642,9 @@ __global__ __launch_bounds__(kOrderThreads) void k_prepare(const ExpandArgs* __r
     auto draws = [&](uint32_t k, uint32_t* c, uint32_t* t, uint32_t* slot, uint64_t* seed, uint32_t* query) {
         if constexpr(MULTI)
         {
-            *slot  = segment_of(A->child_off, A->Q, k);
-            *c     = k - A->child_off[*slot];
-            *t     = A->ctl[*slot].cur.t;
+            *slot  = segment_of(child_off, A->Q, k);
+            *c     = k - child_off[*slot];
+            *t     = advance ? A->ctl[*slot].nxt.t : A->ctl[*slot].cur.t;
             *seed  = A->qd[*slot].seed;
             *query = A->qd[*slot].query;
         }
@@ -668,7 +692,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_prepare(const ExpandArgs* __r
         draws(k, &c, &t, &slot, &seed, &query);
         if constexpr(MULTI)
         {
-            fan  = A->ctl[slot].cur.fan;
+            fan  = advance ? A->ctl[slot].nxt.fan : A->ctl[slot].cur.fan;
             base = A->qd[slot].base;
             mix  = A->qd[slot].mix;
         }
@@ -1217,10 +1241,6 @@ uint32_t order_blocks(const kpax_handle* h)
     return std::min<uint32_t>(kOrderBlocks, std::max<uint32_t>(1u, (uint32_t)h->cu_count));
 }
 
-// The advance runs inside the preparation pass of a single-slot handle
-// instead of as its own launch.
-bool fused_advance(const kpax_handle* h) { return h->Qmax == 1; }
-
 int upload_expand_args(kpax_handle* h)
 {
     ExpandArgs a;
@@ -1310,10 +1330,9 @@ void launch_prepare(kpax_handle* h)
     const AdvanceArgs V = advance_args(h);
     const int ordered   = h->order_children ? 1 : 0;
     if(h->Qmax == 1)
-        hipLaunchKernelGGL((k_prepare<MODEL, false>), ob, dim3(kOrderThreads), 0, h->stream, A, ordered, V,
-                           fused_advance(h) ? 1 : 0);
+        hipLaunchKernelGGL((k_prepare<MODEL, false>), ob, dim3(kOrderThreads), 0, h->stream, A, ordered, V, 1);
     else
-        hipLaunchKernelGGL((k_prepare<MODEL, true>), ob, dim3(kOrderThreads), 0, h->stream, A, ordered, V, 0);
+        hipLaunchKernelGGL((k_prepare<MODEL, true>), ob, dim3(kOrderThreads), 0, h->stream, A, ordered, V, 1);
 }
 
 // The preparation passes, then the expansion; `between` (profiling) is
@@ -1372,7 +1391,6 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
 {
     const uint32_t Q     = h->Qmax;
     h->status_parity     = p;
-    if(!fused_advance(h)) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1024), 0, h->stream, advance_args(h));
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][0], h->stream));
     int rc = launch_expand(h, ev ? h->ph_ev[p][1] : nullptr);
     if(rc != KPAX_OK) return rc;
