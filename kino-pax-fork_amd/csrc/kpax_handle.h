@@ -155,7 +155,9 @@ struct kpax_handle {
     int32_t* h_path_idx     = nullptr;
     hipEvent_t it_ev[2]     = {nullptr, nullptr};
     hipEvent_t ph_ev[2][7]  = {};       // phase boundaries of iterations of each parity (profiling):
-                                        // after advance, expand, append, propose, score, frontier
+                                        // before the preparation pass, then after it, expand, append,
+                                        // propose, score, frontier
+    bool ph_ran[2][2]       = {};       // per parity: the proposal pass and k_score were launched
     hipEvent_t init_ev[2]   = {nullptr, nullptr};
     // one captured iteration per status-buffer parity, replayed while profiling is off
     hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [LDS expansion][status parity]

@@ -1402,10 +1402,20 @@ int enqueue_iteration(kpax_handle* h, int p, bool ev)
         default: launch_append<1024>(h); break;
     }
     if(ev) HIPC(hipEventRecord(h->ph_ev[p][3], h->stream));
-    if(h->mix > 0.0f && h->prop_slots > 0 && (rc = launch_propose(h, 0)) != KPAX_OK) return rc;
-    if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
-    if(!fused_scores(h)) hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
-    if(ev) HIPC(hipEventRecord(h->ph_ev[p][5], h->stream));
+    // the proposal and score boundaries only around launches that run (an
+    // event pair with nothing between them would time the events themselves)
+    h->ph_ran[p][0] = h->mix > 0.0f && h->prop_slots > 0;
+    h->ph_ran[p][1] = !fused_scores(h);
+    if(h->ph_ran[p][0])
+    {
+        if((rc = launch_propose(h, 0)) != KPAX_OK) return rc;
+        if(ev) HIPC(hipEventRecord(h->ph_ev[p][4], h->stream));
+    }
+    if(h->ph_ran[p][1])
+    {
+        hipLaunchKernelGGL(k_score, dim3(Q), dim3(1024), 0, h->stream, h->ctl, h->rv, h->RP, h->PP.eps);
+        if(ev) HIPC(hipEventRecord(h->ph_ev[p][5], h->stream));
+    }
     switch(h->scan_tile)
     {
         case 1024: launch_frontier<256>(h); break;
@@ -1504,17 +1514,21 @@ struct QueryIn {
 void account_iteration(kpax_handle* h, int p)
 {
     if(h->h_status[(size_t)p * (h->Qmax + 1) + h->Qmax] <= 0) return;  // no slot was running
-    // event pairs -> phases prepare, expand, append, propose, score, frontier
+    // event pairs -> phases prepare, expand, append, propose (when it ran),
+    // score (when k_score ran), frontier; each phase ends at its own event
+    // and starts at the last one recorded before it
     constexpr int kPhase[6] = {6, 0, 1, 5, 2, 3};
-    float ms = 0.0f;
+    float ms   = 0.0f;
+    int from   = 0;
     for(int k = 0; k < 6; ++k)
     {
-        if(kPhase[k] == 5 && h->mix <= 0.0f) continue;
-        if(hipEventElapsedTime(&ms, h->ph_ev[p][k], h->ph_ev[p][k + 1]) == hipSuccess)
+        if((k == 3 && !h->ph_ran[p][0]) || (k == 4 && !h->ph_ran[p][1])) continue;
+        if(hipEventElapsedTime(&ms, h->ph_ev[p][from], h->ph_ev[p][k + 1]) == hipSuccess)
         {
             h->stats.phase_ms[kPhase[k]] += ms;
             h->stats.phase_launches[kPhase[k]] += 1;
         }
+        from = k + 1;
     }
 }
 
