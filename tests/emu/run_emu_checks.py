@@ -94,6 +94,19 @@ def plan_checks():
         if near_goal is not None:
             check(r.status == kpax.OK and r.verified == 1 and r.goal_index == o["goal_index"],
                   f"plan workload {k} reaches the goal with a verified path")
+    # profiled phases count the launches that ran: a single-query handle
+    # computes its scores inside the frontier pass and runs no proposal pass
+    cfg, obs, start, goal = workload(1, seed=0, max_tree=1500, max_iter=6)
+    small(cfg)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        p.set_profiling(True)
+        p.reset_stats()
+        r = p.plan(start, goal, seed=77)
+        n = p.stats()["phase_launches"]
+    check(n["score"] == 0 and n["propose"] == 0 and n["init"] == 1
+          and n["prepare"] == n["expand"] == n["append"] == n["frontier"] == r.iterations,
+          f"profiled phases of a {r.iterations}-iteration plan: {n}")
 
 
 def prune_checks():
