@@ -244,6 +244,26 @@ def batch_checks():
                     f"{r.iterations} tree {r.tree_size} vs oracle {o['status']} {o['iterations']} {o['tree_size']}")
     check(any(r.status == kpax.OK for r in res) and any(r.status != kpax.OK for r in res),
           "batch holds both found and unfound queries")
+    # the batched kernels of the other two models
+    for model in (1, 3):
+        cfg, obs, start, goal = workload(model, seed=0, max_tree=3000, max_iter=8 if model == 1 else 5)
+        small(cfg)
+        mid = ((np.asarray(start[:3], np.float32) + np.asarray(goal[:3], np.float32)) / 2).astype(np.float32)
+        mq = [kpax.Query(start, mid, seed=31, budget=2000, env=0), kpax.Query(start, goal, seed=32, budget=900, env=0),
+              kpax.Query(start, goal, seed=33, budget=1, env=0), kpax.Query(start, mid, seed=34, budget=1500, env=0)]
+        with kpax.KPAX(cfg, max_queries=2, node_capacity=5000) as p:
+            p.set_environment(0, obs)
+            mres = p.plan_batch(mq)
+        for i, (q, r) in enumerate(zip(mq, mres)):
+            c = cfg.copy()
+            c.max_tree = q.budget
+            o = oracle.plan(c, obs, q.start, q.goal, seed=q.seed, threads=4)
+            same = (r.status, r.iterations, r.tree_size, r.expansions) == (o["status"], o["iterations"],
+                                                                           o["tree_size"], o["expansions"])
+            if same and r.status == kpax.OK:
+                same = np.array_equal(r.rows.view(np.uint32), oracle_path_rows(o, c).view(np.uint32))
+            check(same, f"model {model} batch query {i} (budget {q.budget}): status {r.status} tree {r.tree_size} "
+                        f"vs oracle {o['status']} {o['tree_size']}")
 
 
 SECTIONS = {"ops": ops_checks, "plans": plan_checks, "batch": batch_checks, "mlp": mlp_checks,

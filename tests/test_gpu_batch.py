@@ -60,6 +60,39 @@ def test_batch_matches_oracle_per_query(kpax, oracle):
     assert kpax.OK in statuses and kpax.BUDGET_EXHAUSTED in statuses
 
 
+@pytest.mark.parametrize("model", [1, 3], ids=["double_integrator", "dubins_airplane"])
+def test_batch_of_the_other_models_matches_oracle_per_query(kpax, oracle, model):
+    """The batched expansion, preparation and scans compiled for the 6D double
+    integrator and the 6D Dubins airplane (the quadcopter's are covered
+    above): each query of a batch with uneven budgets over two slots equals
+    the oracle's plan of that query alone, counters and path rows bit for
+    bit."""
+    cfg, obs, start, goal = workload(model, seed=0, max_iter=25 if model == 1 else 12)
+    mid = ((np.asarray(start[:3], np.float32) + np.asarray(goal[:3], np.float32)) / 2).astype(np.float32)
+    qs = [kpax.Query(start, mid, seed=31, budget=1 << 14, env=0),
+          kpax.Query(start, goal, seed=32, budget=6000, env=0),
+          kpax.Query(start, goal, seed=33, budget=1, env=0),
+          kpax.Query(start, mid, seed=34, budget=20000, env=0),
+          kpax.Query(start, goal, seed=35, budget=1 << 13, env=0)]
+    print(f"model {model}: 5 queries on 2 slots vs the oracle", flush=True)
+    with kpax.KPAX(cfg, max_queries=2, node_capacity=40000) as p:
+        p.set_environment(0, obs)
+        res = p.plan_batch(qs)
+    for i, (q, r) in enumerate(zip(qs, res)):
+        c = cfg.copy()
+        c.max_tree = q.budget
+        o = oracle.plan(c, obs, q.start, q.goal, seed=q.seed, threads=8)
+        print(f"  query {i}: gpu {r.status} it {r.iterations} ts {r.tree_size}; oracle {o['status']} "
+              f"it {o['iterations']} ts {o['tree_size']}", flush=True)
+        assert (r.status, r.iterations, r.tree_size, r.expansions) == \
+            (o["status"], o["iterations"], o["tree_size"], o["expansions"])
+        assert r.goal_index == o["goal_index"]
+        if r.status == kpax.OK:
+            assert np.array_equal(r.rows.view(np.uint32), oracle_path_rows(o, c).view(np.uint32))
+            assert r.verified == 1
+    assert res[2].status == kpax.BUDGET_EXHAUSTED
+
+
 def test_batch_matches_single_plans(kpax):
     cfg, obs, start, goal = workload(2, seed=0, max_iter=60)
     envs = [obs] + _envs(kpax, cfg, start, goal, 3)
