@@ -94,6 +94,25 @@ def plan_checks():
         if near_goal is not None:
             check(r.status == kpax.OK and r.verified == 1 and r.goal_index == o["goal_index"],
                   f"plan workload {k} reaches the goal with a verified path")
+    # the double integrator and the Dubins airplane (in a generated forest:
+    # its maze has no LDS image) on both grid paths
+    for model in (1, 3):
+        cfg, obs, start, goal = workload(model, seed=0, max_tree=2000, max_iter=6 if model == 1 else 4)
+        small(cfg)
+        if model == 3:
+            obs = kpax.generate_environment(2, 5, 40, cfg.ws_min, cfg.ws_max, start[:3],
+                                            list(goal[:3]) + [cfg.goal_radius])
+        o = oracle.plan(cfg, obs, start, goal, seed=41, threads=4)
+        with kpax.KPAX(cfg) as p:
+            p.set_obstacles(obs)
+            has_image = p.lds_grid()[1] > 0
+            for lds in (1, 0):
+                p.set_launch(lds_grid=lds)
+                r = p.plan(start, goal, seed=41)
+                t = p.tree()
+                check(has_image and r.tree_size == o["tree_size"] and np.array_equal(t["parents"], o["parents"])
+                      and np.array_equal(t["states"].view(np.uint32), o["states"].view(np.uint32)),
+                      f"model {model} with lds_grid={lds}: tree {r.tree_size} vs oracle {o['tree_size']}")
     # profiled phases count the launches that ran: a single-query handle
     # computes its scores inside the frontier pass and runs no proposal pass
     cfg, obs, start, goal = workload(1, seed=0, max_tree=1500, max_iter=6)

@@ -140,6 +140,29 @@ def test_plan_identical_with_and_without_lds_grid(kpax):
     assert np.array_equal(t1["states"].view(np.uint32), t0["states"].view(np.uint32))
 
 
+@pytest.mark.parametrize("k", [1, 3], ids=["double_integrator", "dubins_airplane"])
+def test_other_models_with_and_without_lds_grid_bit_exact_vs_oracle(kpax, oracle, k):
+    """The single-query expansion of the double integrator and of the Dubins
+    airplane compiled for both grid paths (its LDS image, and the CSR grid in
+    global memory) plans the oracle's tree; the Dubins airplane flies a
+    generated column forest here, since its maze has no LDS image."""
+    cfg, obs, start, goal = workload(k, seed=0, max_tree=1 << 15, max_iter=10 if k == 1 else 8)
+    if k == 3:
+        obs = kpax.generate_environment(2, 5, 40, cfg.ws_min, cfg.ws_max, start[:3],
+                                        list(goal[:3]) + [cfg.goal_radius])
+    o = oracle.plan(cfg, obs, start, goal, seed=41, threads=8)
+    with kpax.KPAX(cfg) as p:
+        p.set_obstacles(obs)
+        assert p.lds_grid()[1] > 0  # the environment has an LDS image
+        for lds in (1, 0):
+            p.set_launch(lds_grid=lds)
+            r = p.plan(start, goal, seed=41)
+            t = p.tree()
+            assert (r.status, r.iterations, r.tree_size) == (o["status"], o["iterations"], o["tree_size"]), lds
+            assert np.array_equal(t["parents"], o["parents"])
+            assert np.array_equal(t["states"].view(np.uint32), o["states"].view(np.uint32))
+
+
 @pytest.mark.parametrize("k", MODELS)
 def test_plan_identical_with_and_without_wave_priority(kpax, k):
     """Issue priorities reorder the expansion's instruction issue only: the
